@@ -1,0 +1,1362 @@
+// Host runtime and C ABI of libfastoverlap_hip (see include/fastoverlap_hip.h).
+//
+// A context owns one device, a default stream, the byte budget and the
+// device-resident tables (Wigner-d and twiddles per grid side, twiddles per
+// FFT side), built once and shared read-only by every batch.  A batch owns
+// one device allocation holding all of its inputs, intermediates and
+// outputs, and its own stream, so several batches run concurrently.
+#include "fastoverlap_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fo_common.h"
+#include "fo_host.h"
+#include "fo_internal.h"
+#include "fo_refine.h"
+
+namespace fo {
+// launchers (fo_spherical.hip / fo_periodic.hip)
+void launch_wigner_table(hipStream_t, int, int, double*, float*);
+void launch_twiddles(hipStream_t, int, double, double2*, float2*);
+void launch_sph_prep(hipStream_t, int, const SphPair*, const double*, const double*, double*,
+                     double*, double*, int);
+void launch_sph_ylm(hipStream_t, int, const SphPair*, const int*, const double*, const double*,
+                    double2*, double2*);
+void launch_sph_radial(hipStream_t, int, long long, const SphPair*, const SphGroup*, const int*,
+                       const double*, const double*, const double*, double*);
+void launch_sph_contract(hipStream_t, int, int, const SphPair*, const SphGroup*, const double2*,
+                         const double2*, const double*, const double*, double2*, float2*);
+size_t isoft_plane_smem(int, int);
+void launch_isoft_plane(hipStream_t, int, int, int, long long, const SphPair*, const float*,
+                        const float2*, const float2*, const float2*, int, int, PlaneCand*, float*);
+void launch_sph_select(hipStream_t, int, int, long long, int, const SphPair*, const double*,
+                       const double2*, const double2*, const double2*, int, const PlaneCand*,
+                       double2*, long long, SphPeak*);
+void launch_sph_rotopt(hipStream_t, int, int, const SphPair*, const double2*, const double2*, int,
+                       const SphPeak*, SphOpt*);
+void launch_sph_finish(hipStream_t, int, int, const SphPair*, const SphGroup*, const int*,
+                       const double*, const double*, const SphOpt*, int, double*, int*, int*,
+                       double*, SphResult*);
+void launch_c64_to_c32(hipStream_t, long long, const double2*, float2*);
+void launch_sph_energy(hipStream_t, int, const double2*, double, double, double, double*);
+void launch_wigner_D(hipStream_t, int, double, double, double, double2*, double2*);
+void launch_ylm_angles(hipStream_t, int, int, const double*, const double*, double2*);
+size_t per_sf_smem(int);
+void launch_per_sf(hipStream_t, int, int, const PerPair*, const PerGroup*, const int*,
+                   const double*, const double*, double2*, double2*, double2*);
+void launch_per_product(hipStream_t, int, int, const PerPair*, const double2*, const double2*,
+                        double2*);
+size_t per_plane_smem(int, int);
+void launch_per_plane(hipStream_t, int, int, int, const PerPair*, const double2*, const float2*,
+                      PlaneCand*, float*);
+void launch_per_select(hipStream_t, int, int, int, const PerPair*, const double2*, const double2*,
+                       const PlaneCand*, double2*, long long, PerPeak*);
+void launch_per_refine(hipStream_t, int, int, const PerPair*, const PerGroup*, const int*,
+                       const double*, const double*, const PerPeak*, int, int*, int*, int*,
+                       double*, double*, PerResult*);
+void launch_lap_align(hipStream_t, int, const double*, const double*, int, const int*, const int*,
+                      const int*, int, const double*, int, int*, int*, double*);
+}  // namespace fo
+
+using namespace fo;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int set_err(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define FO_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return set_err(FO_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define FO_CHECK_LAUNCH()                                                         \
+  do {                                                                            \
+    hipError_t e_ = hipGetLastError();                                            \
+    if (e_ != hipSuccess)                                                         \
+      return set_err(FO_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+static bool all_finite(const double* p, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(p[i])) return false;
+  return true;
+}
+static constexpr int kMaxL = 63;
+static constexpr int kMaxGroup = 4096;
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct SphTable {
+  int n = 0, Lt = 0;
+  long long Kt = 0;
+  double* tab64 = nullptr;
+  float* tab32 = nullptr;
+  double2* tw64 = nullptr;
+  float2* tw32 = nullptr;
+};
+struct PerTw {
+  int F = 0;
+  double2* tw64 = nullptr;
+  float2* tw32 = nullptr;
+};
+
+struct fo_context {
+  int device = 0;
+  size_t max_lds = 65536;  // bytes of LDS one workgroup may use on this device
+  hipStream_t stream = nullptr;
+  unsigned long long budget = 0;
+  unsigned long long used = 0;
+  std::mutex mu;       // tables + accounting
+  std::mutex call_mu;  // serialises the single-call helpers
+  std::map<int, SphTable> sph;
+  std::map<int, PerTw> per;
+  std::vector<void*> owned;  // table allocations
+};
+
+static int ctx_alloc(fo_context* c, size_t bytes, void** p) {
+  {
+    std::lock_guard<std::mutex> g(c->mu);
+    if (c->budget && c->used + bytes > c->budget)
+      return set_err(FO_ERR_NOMEM, "allocation of " + std::to_string(bytes) +
+                                       " bytes exceeds the context budget");
+    c->used += bytes;
+  }
+  hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+  if (e != hipSuccess) {
+    std::lock_guard<std::mutex> g(c->mu);
+    c->used -= bytes;
+    return set_err(FO_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  return FO_OK;
+}
+static void ctx_free(fo_context* c, void* p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  std::lock_guard<std::mutex> g(c->mu);
+  c->used -= bytes;
+}
+
+static int get_sph_table(fo_context* c, int n, const SphTable** out) {
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->sph.find(n);
+  if (it != c->sph.end()) {
+    *out = &it->second;
+    return FO_OK;
+  }
+  SphTable t;
+  t.n = n;
+  t.Lt = std::min(n / 2 - 1, kMaxL);
+  t.Kt = kcount(t.Lt);
+  const size_t b64 = sizeof(double) * (size_t)t.Kt * n, b32 = sizeof(float) * (size_t)t.Kt * n;
+  const size_t bt = (sizeof(double2) + sizeof(float2)) * (size_t)n;
+  const size_t total = b64 + b32 + bt + 1024;
+  if (c->budget && c->used + total > c->budget)
+    return set_err(FO_ERR_NOMEM, "Wigner table exceeds the context budget");
+  void* base = nullptr;
+  hipError_t e = hipMalloc(&base, total);
+  if (e != hipSuccess) return set_err(FO_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->used += total;
+  c->owned.push_back(base);
+  char* p = (char*)base;
+  t.tab64 = (double*)p;
+  p += (b64 + 255) / 256 * 256;
+  t.tab32 = (float*)p;
+  p += (b32 + 255) / 256 * 256;
+  t.tw64 = (double2*)p;
+  p += (sizeof(double2) * n + 255) / 256 * 256;
+  t.tw32 = (float2*)p;
+  launch_wigner_table(c->stream, n, t.Lt, t.tab64, t.tab32);
+  launch_twiddles(c->stream, n, 1.0, t.tw64, t.tw32);
+  FO_CHECK_LAUNCH();
+  FO_HIP(hipStreamSynchronize(c->stream));
+  c->sph[n] = t;
+  *out = &c->sph[n];
+  return FO_OK;
+}
+
+static int get_per_tw(fo_context* c, int F, const PerTw** out) {
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->per.find(F);
+  if (it != c->per.end()) {
+    *out = &it->second;
+    return FO_OK;
+  }
+  PerTw t;
+  t.F = F;
+  const size_t total = (sizeof(double2) + sizeof(float2)) * (size_t)F + 512;
+  void* base = nullptr;
+  hipError_t e = hipMalloc(&base, total);
+  if (e != hipSuccess) return set_err(FO_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->used += total;
+  c->owned.push_back(base);
+  t.tw64 = (double2*)base;
+  t.tw32 = (float2*)((char*)base + (sizeof(double2) * F + 255) / 256 * 256);
+  launch_twiddles(c->stream, F, -1.0, t.tw64, t.tw32);
+  FO_CHECK_LAUNCH();
+  FO_HIP(hipStreamSynchronize(c->stream));
+  c->per[F] = t;
+  *out = &c->per[F];
+  return FO_OK;
+}
+
+// bump layout inside one allocation
+struct Layout {
+  size_t off = 0;
+  template <class T>
+  size_t add(size_t count) {
+    size_t o = off;
+    off += (sizeof(T) * count + 255) / 256 * 256;
+    return o;
+  }
+};
+
+extern "C" {
+
+int fo_version(void) { return 10000; }
+
+const char* fo_last_error(void) { return g_err.c_str(); }
+
+int fo_device_count(int* count) {
+  if (!count) return set_err(FO_ERR_INVALID, "count is NULL");
+  FO_HIP(hipGetDeviceCount(count));
+  return FO_OK;
+}
+
+int fo_context_create(int device, unsigned long long hbm_budget, fo_context** out) {
+  if (!out) return set_err(FO_ERR_INVALID, "out is NULL");
+  int n = 0;
+  FO_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return set_err(FO_ERR_INVALID, "no such device");
+  FO_HIP(hipSetDevice(device));
+  auto* c = new fo_context();
+  c->device = device;
+  c->budget = hbm_budget;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+      lds > 0)
+    c->max_lds = (size_t)lds;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return set_err(FO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return FO_OK;
+}
+
+int fo_context_destroy(fo_context* c) {
+  if (!c) return FO_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->owned) (void)hipFree(p);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FO_OK;
+}
+
+int fo_context_memory(fo_context* c, unsigned long long* used, unsigned long long* budget) {
+  if (!c) return set_err(FO_ERR_INVALID, "ctx is NULL");
+  std::lock_guard<std::mutex> g(c->mu);
+  if (used) *used = c->used;
+  if (budget) *budget = c->budget;
+  return FO_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// spherical batch
+// ---------------------------------------------------------------------------
+struct fo_sph_batch {
+  fo_context* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  const SphTable* tab = nullptr;
+  SphLaunch lc{};
+  int flags = 0;
+  int maxgroup = 1;
+  long long Ntot = 0, Ktot = 0;
+  std::vector<SphPair> hp;
+  std::vector<SphGroup> hg;
+  std::vector<int> hgath;
+  size_t bytes = 0;
+  void* base = nullptr;
+  // device pointers
+  SphPair* pd = nullptr;
+  SphGroup* grp = nullptr;
+  int* gath = nullptr;
+  double *pos1 = nullptr, *pos2 = nullptr, *X1c = nullptr, *X2c = nullptr, *X2R = nullptr,
+         *X2out = nullptr, *scale = nullptr, *R = nullptr;
+  double2 *Y1 = nullptr, *Y2 = nullptr, *I64A = nullptr, *I64B = nullptr, *scr = nullptr;
+  float2 *I32A = nullptr, *I32B = nullptr;
+  PlaneCand* cand = nullptr;
+  SphPeak* peaks = nullptr;
+  SphOpt* opt = nullptr;
+  SphResult* res = nullptr;
+  int *perm = nullptr, *rowsol = nullptr;
+  float* grid = nullptr;
+  long long scr_stride = 0;
+  // graphs per stage mask
+  std::map<int, hipGraphExec_t> graphs;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[9] = {};
+  bool ev_made = false;
+  // layout (byte offsets inside base) and ownership
+  struct {
+    size_t pd, grp, gath, pos1, pos2, X1c, X2c, X2R, X2out, scale, R, Y1, Y2, I64A, I32A, I64B,
+        I32B, cand, peaks, opt, res, perm, rowsol, scr, grid;
+  } off{};
+  int gsum = 0;
+  bool own_mem = false, own_stream = false;
+};
+
+static int sph_enqueue(fo_sph_batch* b, int stages) {
+  const SphLaunch& L = b->lc;
+  hipStream_t s = b->stream;
+  const int P = L.P;
+  const bool tm = b->timing;
+  auto mark = [&](int i) {
+    if (tm) (void)hipEventRecord(b->ev[i], s);
+  };
+  mark(0);
+  if (stages & FO_STAGE_PREP)
+    launch_sph_prep(s, P, b->pd, b->pos1, b->pos2, b->X1c, b->X2c, b->scale, L.center);
+  mark(1);
+  if (stages & FO_STAGE_COEFF) {
+    launch_sph_ylm(s, P, b->pd, b->gath, b->X1c, b->X2c, b->Y1, b->Y2);
+    mark(2);
+    launch_sph_radial(s, P, L.maxjk, b->pd, b->grp, b->gath, b->X1c, b->X2c, b->scale, b->R);
+    mark(3);
+    launch_sph_contract(s, P, L.Lmax, b->pd, b->grp, b->Y1, b->Y2, b->R, b->scale, b->I64A,
+                        b->I32A);
+  } else {
+    mark(2);
+    mark(3);
+  }
+  mark(4);
+  if (stages & FO_STAGE_GRID) {
+    launch_isoft_plane(s, P, L.n, L.Lmax, b->tab->Kt, b->pd, b->tab->tab32, b->tab->tw32, b->I32A,
+                       b->I32B, L.mode, L.norient, b->cand, b->grid);
+    mark(5);
+    launch_sph_select(s, P, L.n, b->tab->Kt, L.norient, b->pd, b->tab->tab64, b->tab->tw64,
+                      b->I64A, b->I64B, L.mode, b->cand, b->scr, b->scr_stride, b->peaks);
+  } else {
+    mark(5);
+  }
+  mark(6);
+  if (stages & FO_STAGE_ROTOPT) launch_sph_rotopt(s, P, L.norient, b->pd, b->I64A, b->I64B, L.mode, b->peaks,
+                                                 b->opt);
+  mark(7);
+  if (stages & FO_STAGE_FINISH)
+    launch_sph_finish(s, P, b->maxgroup, b->pd, b->grp, b->gath, b->X1c, b->X2c, b->opt,
+                      L.norient, b->X2R, b->perm, b->rowsol, b->X2out, b->res);
+  mark(8);
+  FO_CHECK_LAUNCH();
+  return FO_OK;
+}
+
+namespace fo {
+
+int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out) {
+  if (!c || !d || !out) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (d->npairs < 1) return set_err(FO_ERR_INVALID, "npairs must be >= 1");
+  if (!d->natoms || !d->L || !d->ngroups)
+    return set_err(FO_ERR_INVALID, "natoms, L and ngroups are required");
+  const int P = d->npairs, n = d->grid_n;
+  if (n < 2 || n > 256) return set_err(FO_ERR_UNSUPPORTED, "grid_n must be in [2, 256]");
+  const bool general = d->flags & FO_SPH_GENERAL;
+  auto b = std::make_unique<fo_sph_batch>();
+  b->ctx = c;
+  b->flags = d->flags;
+  int Lmax = 0, maxatoms = 0, maxgath = 0;
+  long long maxjk = 0, Ntot = 0, Ktot = 0, radtot = 0, ylmtot = 0;
+  int gsum = 0, gatot = 0;
+  for (int p = 0; p < P; ++p) gsum += d->ngroups[p];
+  for (int q = 0; q < gsum; ++q) {
+    if (!d->group_sizes || d->group_sizes[q] < 0) return set_err(FO_ERR_INVALID, "bad group size");
+    gatot += d->group_sizes[q];
+  }
+  if (gatot > 0 && !d->group_atoms) return set_err(FO_ERR_INVALID, "group_atoms is NULL");
+  b->hp.resize(P);
+  b->hg.resize(gsum);
+  b->hgath.resize(std::max(gatot, 1));
+  int gq = 0, ga = 0;
+  for (int p = 0; p < P; ++p) {
+    const int N = d->natoms[p], Lp = d->L[p];
+    if (N < 1) return set_err(FO_ERR_INVALID, "pair " + std::to_string(p) + ": natoms < 1");
+    if (Lp < 0 || Lp > kMaxL)
+      return set_err(FO_ERR_UNSUPPORTED, "pair " + std::to_string(p) + ": Jmax must be in [0, 63]");
+    if (n < 2 * (Lp + 1))
+      return set_err(FO_ERR_INVALID, "pair " + std::to_string(p) + ": grid_n < 2(Jmax+1)");
+    SphPair& sp = b->hp[p];
+    sp.natoms = N;
+    sp.atom_off = (int)Ntot;
+    sp.L = Lp;
+    sp.ngroups = d->ngroups[p];
+    sp.group_off = gq;
+    sp.gath_off = ga;
+    sp.scale = d->scale ? d->scale[p] : 0.0;
+    sp.pad0 = 0;
+    std::vector<char> seen(N, 0);
+    long long jk = 0;
+    int ng = 0;
+    for (int q = 0; q < sp.ngroups; ++q) {
+      const int sz = d->group_sizes[gq + q];
+      if (sz > kMaxGroup)
+        return set_err(FO_ERR_UNSUPPORTED, "permutation group larger than 4096 atoms");
+      SphGroup& G = b->hg[gq + q];
+      G.size = sz;
+      G.gath_off = ng;
+      G.jk_off = jk;
+      G.rad_off = (long long)(Lp + 1) * jk;
+      for (int i = 0; i < sz; ++i) {
+        const int a = d->group_atoms[ga + ng + i];
+        if (a < 0 || a >= N) return set_err(FO_ERR_INVALID, "group atom index out of range");
+        if (seen[a]) return set_err(FO_ERR_INVALID, "atom listed in two permutation groups");
+        seen[a] = 1;
+        b->hgath[ga + ng + i] = a;
+      }
+      jk += (long long)sz * sz;
+      ng += sz;
+      b->maxgroup = std::max(b->maxgroup, sz);
+    }
+    sp.ngath = ng;
+    sp.coef_off = Ktot;
+    sp.rad_off = radtot;
+    sp.ylm_off = ylmtot;
+    Ktot += kcount(Lp);
+    radtot += (long long)(Lp + 1) * jk;
+    ylmtot += (long long)(Lp + 1) * (Lp + 1) * ng;
+    Lmax = std::max(Lmax, Lp);
+    maxatoms = std::max(maxatoms, N);
+    maxgath = std::max(maxgath, ng);
+    maxjk = std::max(maxjk, jk);
+    Ntot += N;
+    gq += sp.ngroups;
+    ga += ng;
+  }
+  if (isoft_plane_smem(n, Lmax) > c->max_lds)
+    return set_err(FO_ERR_UNSUPPORTED, "grid_n and Jmax too large for one beta plane in LDS (" +
+                                           std::to_string(isoft_plane_smem(n, Lmax)) + " > " +
+                                           std::to_string(c->max_lds) + " bytes)");
+  if (lap_lds_bytes(b->maxgroup) > c->max_lds)
+    return set_err(FO_ERR_UNSUPPORTED, "permutation group too large for the LDS assignment solver");
+  int rc = get_sph_table(c, n, &b->tab);
+  if (rc) return rc;
+  if (Lmax > b->tab->Lt) return set_err(FO_ERR_UNSUPPORTED, "Jmax exceeds the table degree");
+  SphLaunch& lc = b->lc;
+  lc.P = P;
+  lc.n = n;
+  lc.Lmax = Lmax;
+  lc.Lt = b->tab->Lt;
+  lc.maxgath = maxgath;
+  lc.maxatoms = maxatoms;
+  lc.maxjk = maxjk;
+  lc.norient = (d->flags & FO_SPH_NO_INVERT) ? 1 : 2;
+  lc.mode = general ? 1 : 0;
+  if (general) lc.norient = 2;
+  lc.keep_grid = (d->flags & FO_SPH_KEEP_GRID) ? 1 : 0;
+  lc.center = (d->flags & FO_SPH_NO_CENTER) ? 0 : 1;
+  b->Ntot = Ntot;
+  b->Ktot = Ktot;
+  const int no = lc.norient;
+  b->scr_stride = (long long)(Lmax + 1) * (2 * Lmax + 1) + (long long)(Lmax + 1) * n;
+  b->gsum = gsum;
+  Layout lay;
+  auto& o = b->off;
+  o.pd = lay.add<SphPair>(P);
+  o.grp = lay.add<SphGroup>(std::max(gsum, 1));
+  o.gath = lay.add<int>(b->hgath.size());
+  o.pos1 = lay.add<double>(3 * Ntot);
+  o.pos2 = lay.add<double>(3 * Ntot);
+  o.X1c = lay.add<double>(3 * Ntot);
+  o.X2c = lay.add<double>(3 * Ntot);
+  o.X2R = lay.add<double>(3 * Ntot);
+  o.X2out = lay.add<double>(3 * Ntot);
+  o.scale = lay.add<double>(P);
+  o.R = lay.add<double>(std::max(radtot, 1LL));
+  o.Y1 = lay.add<double2>(std::max(ylmtot, 1LL));
+  o.Y2 = lay.add<double2>(std::max(ylmtot, 1LL));
+  o.I64A = lay.add<double2>(Ktot);
+  o.I32A = lay.add<float2>(Ktot);
+  o.I64B = general ? lay.add<double2>(Ktot) : 0;
+  o.I32B = general ? lay.add<float2>(Ktot) : 0;
+  o.cand = lay.add<PlaneCand>((size_t)P * no * n);
+  o.peaks = lay.add<SphPeak>((size_t)P * no);
+  o.opt = lay.add<SphOpt>((size_t)P * no);
+  o.res = lay.add<SphResult>(P);
+  o.perm = lay.add<int>(Ntot);
+  o.rowsol = lay.add<int>(Ntot + 1);
+  o.scr = lay.add<double2>((size_t)P * no * b->scr_stride);
+  o.grid = lc.keep_grid ? lay.add<float>((size_t)P * no * n * n * n) : 0;
+  b->bytes = lay.off;
+  *out = b.release();
+  return FO_OK;
+}
+
+size_t sph_batch_bytes(const fo_sph_batch* b) { return b->bytes; }
+
+int sph_batch_bind(fo_sph_batch* b, void* mem, hipStream_t stream) {
+  fo_context* c = b->ctx;
+  if (mem) {
+    b->base = mem;
+    b->own_mem = false;
+  } else {
+    int rc = ctx_alloc(c, b->bytes, &b->base);
+    if (rc) return rc;
+    b->own_mem = true;
+  }
+  const bool general = b->lc.mode == 1;
+  const auto& o = b->off;
+  char* B = (char*)b->base;
+  b->pd = (SphPair*)(B + o.pd);
+  b->grp = (SphGroup*)(B + o.grp);
+  b->gath = (int*)(B + o.gath);
+  b->pos1 = (double*)(B + o.pos1);
+  b->pos2 = (double*)(B + o.pos2);
+  b->X1c = (double*)(B + o.X1c);
+  b->X2c = (double*)(B + o.X2c);
+  b->X2R = (double*)(B + o.X2R);
+  b->X2out = (double*)(B + o.X2out);
+  b->scale = (double*)(B + o.scale);
+  b->R = (double*)(B + o.R);
+  b->Y1 = (double2*)(B + o.Y1);
+  b->Y2 = (double2*)(B + o.Y2);
+  b->I64A = (double2*)(B + o.I64A);
+  b->I32A = (float2*)(B + o.I32A);
+  b->I64B = general ? (double2*)(B + o.I64B) : nullptr;
+  b->I32B = general ? (float2*)(B + o.I32B) : nullptr;
+  b->cand = (PlaneCand*)(B + o.cand);
+  b->peaks = (SphPeak*)(B + o.peaks);
+  b->opt = (SphOpt*)(B + o.opt);
+  b->res = (SphResult*)(B + o.res);
+  b->perm = (int*)(B + o.perm);
+  b->rowsol = (int*)(B + o.rowsol);
+  b->scr = (double2*)(B + o.scr);
+  b->grid = b->lc.keep_grid ? (float*)(B + o.grid) : nullptr;
+  if (stream) {
+    b->stream = stream;
+    b->own_stream = false;
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return set_err(FO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    b->own_stream = true;
+  }
+  FO_HIP(hipMemcpyAsync(b->pd, b->hp.data(), sizeof(SphPair) * b->lc.P, hipMemcpyHostToDevice,
+                        b->stream));
+  if (b->gsum)
+    FO_HIP(hipMemcpyAsync(b->grp, b->hg.data(), sizeof(SphGroup) * b->gsum, hipMemcpyHostToDevice,
+                          b->stream));
+  FO_HIP(hipMemcpyAsync(b->gath, b->hgath.data(), sizeof(int) * b->hgath.size(),
+                        hipMemcpyHostToDevice, b->stream));
+  return FO_OK;
+}
+
+}  // namespace fo
+
+extern "C" {
+
+int fo_sph_batch_create(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out) {
+  if (!out) return set_err(FO_ERR_INVALID, "out is NULL");
+  fo_sph_batch* b = nullptr;
+  int rc = sph_batch_plan(c, d, &b);
+  if (rc) return rc;
+  rc = sph_batch_bind(b, nullptr, nullptr);
+  if (rc == FO_OK && hipStreamSynchronize(b->stream) != hipSuccess)
+    rc = set_err(FO_ERR_HIP, "descriptor upload failed");
+  if (rc) {
+    fo_sph_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return FO_OK;
+}
+
+int fo_sph_batch_destroy(fo_sph_batch* b) {
+  if (!b) return FO_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (b->ev_made)
+    for (auto& e : b->ev) (void)hipEventDestroy(e);
+  if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->own_mem) ctx_free(b->ctx, b->base, b->bytes);
+  delete b;
+  return FO_OK;
+}
+
+int fo_sph_batch_footprint(const fo_sph_batch* b, unsigned long long* bytes) {
+  if (!b || !bytes) return set_err(FO_ERR_INVALID, "NULL argument");
+  *bytes = b->bytes;
+  return FO_OK;
+}
+
+int fo_sph_batch_norient(const fo_sph_batch* b) { return b ? b->lc.norient : 0; }
+
+int fo_sph_batch_set_positions(fo_sph_batch* b, const double* p1, const double* p2) {
+  if (!b || !p1 || !p2) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (!all_finite(p1, 3 * b->Ntot) || !all_finite(p2, 3 * b->Ntot))
+    return set_err(FO_ERR_INVALID, "positions must be finite");
+  FO_HIP(hipMemcpyAsync(b->pos1, p1, sizeof(double) * 3 * b->Ntot, hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipMemcpyAsync(b->pos2, p2, sizeof(double) * 3 * b->Ntot, hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_sph_batch_set_positions_device(fo_sph_batch* b, const void* p1, const void* p2) {
+  if (!b || !p1 || !p2) return set_err(FO_ERR_INVALID, "NULL argument");
+  FO_HIP(hipMemcpyAsync(b->pos1, p1, sizeof(double) * 3 * b->Ntot, hipMemcpyDeviceToDevice, b->stream));
+  FO_HIP(hipMemcpyAsync(b->pos2, p2, sizeof(double) * 3 * b->Ntot, hipMemcpyDeviceToDevice, b->stream));
+  return FO_OK;
+}
+
+int fo_sph_batch_set_coeffs(fo_sph_batch* b, const double* A, const double* Bc) {
+  if (!b || !A) return set_err(FO_ERR_INVALID, "NULL argument");
+  FO_HIP(hipMemcpyAsync(b->I64A, A, sizeof(double2) * b->Ktot, hipMemcpyHostToDevice, b->stream));
+  launch_c64_to_c32(b->stream, b->Ktot, b->I64A, b->I32A);
+  if (b->lc.mode == 1) {
+    if (!Bc) return set_err(FO_ERR_INVALID, "general mode needs the second coefficient set");
+    FO_HIP(hipMemcpyAsync(b->I64B, Bc, sizeof(double2) * b->Ktot, hipMemcpyHostToDevice, b->stream));
+    launch_c64_to_c32(b->stream, b->Ktot, b->I64B, b->I32B);
+  }
+  FO_CHECK_LAUNCH();
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_sph_batch_set_start(fo_sph_batch* b, const double* R0) {
+  if (!b || !R0) return set_err(FO_ERR_INVALID, "NULL argument");
+  const int no = b->lc.norient;
+  std::vector<SphPeak> pk((size_t)b->lc.P * no);
+  for (size_t i = 0; i < pk.size(); ++i) {
+    memset(&pk[i], 0, sizeof(SphPeak));
+    for (int k = 0; k < 3; ++k) pk[i].R0[k] = R0[3 * i + k];
+  }
+  FO_HIP(hipMemcpyAsync(b->peaks, pk.data(), sizeof(SphPeak) * pk.size(), hipMemcpyHostToDevice,
+                        b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_sph_batch_set_rotation(fo_sph_batch* b, const double* R) {
+  if (!b || !R) return set_err(FO_ERR_INVALID, "NULL argument");
+  const int no = b->lc.norient;
+  std::vector<SphOpt> op((size_t)b->lc.P * no);
+  for (int p = 0; p < b->lc.P; ++p)
+    for (int o = 0; o < no; ++o) {
+      SphOpt& x = op[(size_t)p * no + o];
+      memset(&x, 0, sizeof(SphOpt));
+      for (int k = 0; k < 3; ++k) x.R[k] = R[3 * p + k];
+      x.f = o == 0 ? 0.0 : HUGE_VAL;
+    }
+  FO_HIP(hipMemcpyAsync(b->opt, op.data(), sizeof(SphOpt) * op.size(), hipMemcpyHostToDevice,
+                        b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_sph_batch_run(fo_sph_batch* b, int stages, int use_graph) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  FO_HIP(hipSetDevice(b->ctx->device));
+  if (use_graph && !b->timing) {
+    auto it = b->graphs.find(stages);
+    if (it == b->graphs.end()) {
+      hipGraph_t g;
+      FO_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = sph_enqueue(b, stages);
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return set_err(FO_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+      hipGraphExec_t ex;
+      FO_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      it = b->graphs.emplace(stages, ex).first;
+    }
+    FO_HIP(hipGraphLaunch(it->second, b->stream));
+    return FO_OK;
+  }
+  return sph_enqueue(b, stages);
+}
+
+int fo_sph_batch_sync(fo_sph_batch* b) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_sph_batch_results(fo_sph_batch* b, double* dist, double* X1, double* X2, int* perm,
+                         double* R, double* rmat, int* inverted) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  const int P = b->lc.P;
+  std::vector<SphResult> r(P);
+  FO_HIP(hipMemcpyAsync(r.data(), b->res, sizeof(SphResult) * P, hipMemcpyDeviceToHost, b->stream));
+  if (X1) FO_HIP(hipMemcpyAsync(X1, b->X1c, sizeof(double) * 3 * b->Ntot, hipMemcpyDeviceToHost, b->stream));
+  if (X2) FO_HIP(hipMemcpyAsync(X2, b->X2out, sizeof(double) * 3 * b->Ntot, hipMemcpyDeviceToHost, b->stream));
+  if (perm) FO_HIP(hipMemcpyAsync(perm, b->perm, sizeof(int) * b->Ntot, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  for (int p = 0; p < P; ++p) {
+    if (dist) dist[p] = r[p].dist;
+    if (R)
+      for (int k = 0; k < 3; ++k) R[3 * p + k] = r[p].R[k];
+    if (rmat)
+      for (int k = 0; k < 9; ++k) rmat[9 * p + k] = r[p].rmat[k];
+    if (inverted) inverted[p] = r[p].inverted;
+  }
+  return FO_OK;
+}
+
+int fo_sph_batch_peaks(fo_sph_batch* b, int* ind, double* fun, double* peak_value, double* margin,
+                       double* delta, int* ncand, int* noverflow) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  const size_t m = (size_t)b->lc.P * b->lc.norient;
+  std::vector<SphPeak> pk(m);
+  std::vector<SphOpt> op(m);
+  FO_HIP(hipMemcpyAsync(pk.data(), b->peaks, sizeof(SphPeak) * m, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipMemcpyAsync(op.data(), b->opt, sizeof(SphOpt) * m, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < m; ++i) {
+    if (ind)
+      for (int k = 0; k < 3; ++k) ind[3 * i + k] = pk[i].ind[k];
+    if (fun) fun[i] = op[i].f;
+    if (peak_value) peak_value[i] = pk[i].value;
+    if (margin) margin[i] = pk[i].margin;
+    if (delta) delta[i] = pk[i].delta;
+    if (ncand) ncand[i] = pk[i].ncand;
+    if (noverflow) noverflow[i] = pk[i].noverflow;
+  }
+  return FO_OK;
+}
+
+int fo_sph_batch_opt(fo_sph_batch* b, double* R, double* fun, double* gnorm, int* iters) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  const size_t m = (size_t)b->lc.P * b->lc.norient;
+  std::vector<SphOpt> op(m);
+  FO_HIP(hipMemcpyAsync(op.data(), b->opt, sizeof(SphOpt) * m, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < m; ++i) {
+    if (R)
+      for (int k = 0; k < 3; ++k) R[3 * i + k] = op[i].R[k];
+    if (fun) fun[i] = op[i].f;
+    if (gnorm) gnorm[i] = op[i].gnorm;
+    if (iters) iters[i] = op[i].iters;
+  }
+  return FO_OK;
+}
+
+int fo_sph_batch_scales(fo_sph_batch* b, double* scale) {
+  if (!b || !scale) return set_err(FO_ERR_INVALID, "NULL argument");
+  FO_HIP(hipMemcpyAsync(scale, b->scale, sizeof(double) * b->lc.P, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_sph_batch_coeffs(fo_sph_batch* b, double* coef) {
+  if (!b || !coef) return set_err(FO_ERR_INVALID, "NULL argument");
+  FO_HIP(hipMemcpyAsync(coef, b->I64A, sizeof(double2) * b->Ktot, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_sph_batch_grid(fo_sph_batch* b, int pair, float* grid) {
+  if (!b || !grid) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (!b->grid) return set_err(FO_ERR_STATE, "batch created without FO_SPH_KEEP_GRID");
+  if (pair < 0 || pair >= b->lc.P) return set_err(FO_ERR_INVALID, "pair out of range");
+  const size_t n3 = (size_t)b->lc.n * b->lc.n * b->lc.n * b->lc.norient;
+  FO_HIP(hipMemcpyAsync(grid, b->grid + n3 * pair, sizeof(float) * n3, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_sph_batch_timing(fo_sph_batch* b, int enable, float* ms8) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  if (enable >= 0) {
+    if (enable && !b->ev_made) {
+      for (auto& e : b->ev) FO_HIP(hipEventCreate(&e));
+      b->ev_made = true;
+    }
+    b->timing = enable != 0;
+  }
+  if (ms8 && b->ev_made) {
+    FO_HIP(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < 8; ++i) FO_HIP(hipEventElapsedTime(&ms8[i], b->ev[i], b->ev[i + 1]));
+  }
+  return FO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// single-call helpers
+// ---------------------------------------------------------------------------
+struct DevScratch {
+  fo_context* c;
+  void* p = nullptr;
+  size_t n = 0;
+  explicit DevScratch(fo_context* cc) : c(cc) {}
+  ~DevScratch() { ctx_free(c, p, n); }
+  int alloc(size_t bytes) {
+    n = bytes;
+    return ctx_alloc(c, bytes, &p);
+  }
+};
+
+int fo_sph_harm(fo_context* c, int L, int npts, const double* theta, const double* phi, double* Y) {
+  if (!c || !theta || !phi || !Y || L < 0 || npts < 0) return set_err(FO_ERR_INVALID, "bad argument");
+  if (npts == 0) return FO_OK;
+  std::lock_guard<std::mutex> g(c->call_mu);
+  FO_HIP(hipSetDevice(c->device));
+  const size_t W = 2 * L + 1, ny = (size_t)(L + 1) * W * npts;
+  DevScratch s(c);
+  int rc = s.alloc(sizeof(double) * 2 * npts + sizeof(double2) * ny + 512);
+  if (rc) return rc;
+  double* dth = (double*)s.p;
+  double* dph = dth + npts;
+  double2* dY = (double2*)((char*)s.p + ((sizeof(double) * 2 * npts + 255) / 256 * 256));
+  FO_HIP(hipMemcpyAsync(dth, theta, sizeof(double) * npts, hipMemcpyHostToDevice, c->stream));
+  FO_HIP(hipMemcpyAsync(dph, phi, sizeof(double) * npts, hipMemcpyHostToDevice, c->stream));
+  launch_ylm_angles(c->stream, L, npts, dth, dph, dY);
+  FO_CHECK_LAUNCH();
+  FO_HIP(hipMemcpyAsync(Y, dY, sizeof(double2) * ny, hipMemcpyDeviceToHost, c->stream));
+  FO_HIP(hipStreamSynchronize(c->stream));
+  return FO_OK;
+}
+
+static void dense_to_compact_conj(int L, const double* X, std::vector<double>& out) {
+  const int W = 2 * L + 1;
+  out.assign(2 * (size_t)kcount(L), 0.0);
+  for (int l = 0; l <= L; ++l)
+    for (int m1 = -l; m1 <= l; ++m1)
+      for (int m2 = -l; m2 <= l; ++m2) {
+        const size_t dd = ((size_t)l * W + (m1 + W) % W) * W + (m2 + W) % W;
+        const size_t k = (size_t)kidx(l, m1, m2);
+        out[2 * k] = X[2 * dd];
+        out[2 * k + 1] = -X[2 * dd + 1];
+      }
+}
+
+int fo_sph_energy(fo_context* c, int L, const double* X, const double* rot, double* f, double* grad) {
+  if (!c || !X || !rot || L < 0 || L > kMaxL) return set_err(FO_ERR_INVALID, "bad argument");
+  std::lock_guard<std::mutex> g(c->call_mu);
+  FO_HIP(hipSetDevice(c->device));
+  std::vector<double> h;
+  dense_to_compact_conj(L, X, h);
+  DevScratch s(c);
+  int rc = s.alloc(sizeof(double) * h.size() + 512);
+  if (rc) return rc;
+  double2* dI = (double2*)s.p;
+  double* dout = (double*)((char*)s.p + (sizeof(double) * h.size() + 255) / 256 * 256);
+  FO_HIP(hipMemcpyAsync(dI, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+  launch_sph_energy(c->stream, L, dI, rot[0], rot[1], rot[2], dout);
+  FO_CHECK_LAUNCH();
+  double o[10];
+  FO_HIP(hipMemcpyAsync(o, dout, sizeof(o), hipMemcpyDeviceToHost, c->stream));
+  FO_HIP(hipStreamSynchronize(c->stream));
+  if (f) *f = o[0];
+  if (grad) {
+    grad[0] = o[1];
+    grad[1] = o[2];
+    grad[2] = o[3];
+  }
+  return FO_OK;
+}
+
+int fo_sph_wigner_D(fo_context* c, int L, const double* rot, double* D, double* grad) {
+  if (!c || !rot || !D || L < 0 || L > kMaxL) return set_err(FO_ERR_INVALID, "bad argument");
+  std::lock_guard<std::mutex> g(c->call_mu);
+  FO_HIP(hipSetDevice(c->device));
+  const size_t W = 2 * L + 1, m = (size_t)(L + 1) * W * W;
+  DevScratch s(c);
+  int rc = s.alloc(sizeof(double2) * 4 * m);
+  if (rc) return rc;
+  double2* dD = (double2*)s.p;
+  double2* dG = dD + m;
+  launch_wigner_D(c->stream, L, rot[0], rot[1], rot[2], dD, dG);
+  FO_CHECK_LAUNCH();
+  FO_HIP(hipMemcpyAsync(D, dD, sizeof(double2) * m, hipMemcpyDeviceToHost, c->stream));
+  if (grad) FO_HIP(hipMemcpyAsync(grad, dG, sizeof(double2) * 3 * m, hipMemcpyDeviceToHost, c->stream));
+  FO_HIP(hipStreamSynchronize(c->stream));
+  return FO_OK;
+}
+
+int fo_sph_wigner_table(fo_context* c, int bw, double* Ds) {
+  if (!c || !Ds || bw < 1 || bw - 1 > kMaxL) return set_err(FO_ERR_INVALID, "bad argument");
+  const SphTable* t = nullptr;
+  int rc = get_sph_table(c, 2 * bw, &t);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(c->call_mu);
+  const int n = 2 * bw, J = bw - 1, W = 2 * bw - 1;
+  std::vector<double> h((size_t)t->Kt * n);
+  FO_HIP(hipMemcpyAsync(h.data(), t->tab64, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  FO_HIP(hipStreamSynchronize(c->stream));
+  std::fill(Ds, Ds + (size_t)bw * W * W * n, 0.0);
+  for (int l = 0; l <= J; ++l)
+    for (int m1 = -l; m1 <= l; ++m1)
+      for (int m2 = -l; m2 <= l; ++m2)
+        for (int k = 0; k < n; ++k)
+          Ds[(((size_t)l * W + (m1 + W) % W) * W + (m2 + W) % W) * n + k] =
+              h[(size_t)k * t->Kt + kidx(l, m1, m2)];
+  return FO_OK;
+}
+
+int fo_lap_align(fo_context* c, int N, const double* X1, const double* X2, int ngroups,
+                 const int* gsizes, const int* gatoms, int metric, const double* box, int* perm,
+                 double* dist) {
+  if (!c || !X1 || !X2 || N < 1 || ngroups < 0) return set_err(FO_ERR_INVALID, "bad argument");
+  if (metric == 1 && !box) return set_err(FO_ERR_INVALID, "periodic metric needs box");
+  std::lock_guard<std::mutex> g(c->call_mu);
+  FO_HIP(hipSetDevice(c->device));
+  std::vector<int> goff(ngroups + 1, 0);
+  int gmax = 1;
+  for (int q = 0; q < ngroups; ++q) {
+    goff[q + 1] = goff[q] + gsizes[q];
+    gmax = std::max(gmax, gsizes[q]);
+  }
+  if (gmax > kMaxGroup || lap_lds_bytes(gmax) > c->max_lds)
+    return set_err(FO_ERR_UNSUPPORTED, "permutation group too large");
+  if (!all_finite(X1, 3 * (size_t)N) || !all_finite(X2, 3 * (size_t)N))
+    return set_err(FO_ERR_INVALID, "positions must be finite");
+  const int ga = goff[ngroups];
+  for (int i = 0; i < ga; ++i)
+    if (gatoms[i] < 0 || gatoms[i] >= N) return set_err(FO_ERR_INVALID, "group atom out of range");
+  Layout lay;
+  size_t o1 = lay.add<double>(3 * N), o2 = lay.add<double>(3 * N), og = lay.add<int>(std::max(ngroups, 1)),
+         oo = lay.add<int>(ngroups + 1), oa = lay.add<int>(std::max(ga, 1)), op = lay.add<int>(N),
+         orr = lay.add<int>(N + 1), od = lay.add<double>(1);
+  DevScratch s(c);
+  int rc = s.alloc(lay.off);
+  if (rc) return rc;
+  char* B = (char*)s.p;
+  FO_HIP(hipMemcpyAsync(B + o1, X1, sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream));
+  FO_HIP(hipMemcpyAsync(B + o2, X2, sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream));
+  if (ngroups) FO_HIP(hipMemcpyAsync(B + og, gsizes, sizeof(int) * ngroups, hipMemcpyHostToDevice, c->stream));
+  FO_HIP(hipMemcpyAsync(B + oo, goff.data(), sizeof(int) * (ngroups + 1), hipMemcpyHostToDevice, c->stream));
+  if (ga) FO_HIP(hipMemcpyAsync(B + oa, gatoms, sizeof(int) * ga, hipMemcpyHostToDevice, c->stream));
+  launch_lap_align(c->stream, N, (double*)(B + o1), (double*)(B + o2), ngroups, (int*)(B + og),
+                   (int*)(B + oo), (int*)(B + oa), metric, box, gmax, (int*)(B + op),
+                   (int*)(B + orr), (double*)(B + od));
+  FO_CHECK_LAUNCH();
+  if (perm) FO_HIP(hipMemcpyAsync(perm, B + op, sizeof(int) * N, hipMemcpyDeviceToHost, c->stream));
+  double dd = 0.0;
+  FO_HIP(hipMemcpyAsync(&dd, B + od, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FO_HIP(hipStreamSynchronize(c->stream));
+  if (dist) *dist = dd;
+  return FO_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// periodic batch
+// ---------------------------------------------------------------------------
+struct fo_per_batch {
+  fo_context* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  const PerTw* tw = nullptr;
+  int P = 0, nk = 0, F = 0, niter = 10, flags = 0, maxgroup = 1, gsum = 0;
+  long long Ntot = 0, cube = 0;
+  std::vector<PerPair> hp;
+  std::vector<PerGroup> hg;
+  std::vector<int> hatoms;
+  size_t bytes = 0;
+  void* base = nullptr;
+  PerPair* pd = nullptr;
+  PerGroup* grp = nullptr;
+  int* atoms = nullptr;
+  double *pos1 = nullptr, *pos2 = nullptr, *X1out = nullptr, *X2out = nullptr;
+  double2 *Cprod = nullptr, *C1 = nullptr, *C2 = nullptr, *scr = nullptr;
+  PlaneCand* cand = nullptr;
+  PerPeak* peaks = nullptr;
+  PerResult* res = nullptr;
+  int *perm = nullptr, *saveperm = nullptr, *rowsol = nullptr;
+  float* grid = nullptr;
+  long long scr_stride = 0;
+  std::map<int, hipGraphExec_t> graphs;
+  bool timing = false;
+  hipEvent_t ev[5] = {};
+  bool ev_made = false;
+  struct {
+    size_t pd, grp, at, p1, p2, x1, x2, cp, c1, c2, cand, pk, res, perm, sp, rs, scr, grid;
+  } off{};
+  bool keepc = false;
+  bool own_mem = false, own_stream = false;
+};
+
+static int per_enqueue(fo_per_batch* b, int stages) {
+  hipStream_t s = b->stream;
+  const bool tm = b->timing;
+  auto mark = [&](int i) {
+    if (tm) (void)hipEventRecord(b->ev[i], s);
+  };
+  mark(0);
+  if (stages & FO_PSTAGE_SF) {
+    if (b->flags & FO_PER_INPUT_COEFFS)
+      launch_per_product(s, b->P, b->nk, b->pd, b->C1, b->C2, b->Cprod);
+    else
+      launch_per_sf(s, b->P, b->nk, b->pd, b->grp, b->atoms, b->pos1, b->pos2, b->Cprod,
+                    (b->flags & FO_PER_KEEP_COEFFS) ? b->C1 : nullptr,
+                    (b->flags & FO_PER_KEEP_COEFFS) ? b->C2 : nullptr);
+  }
+  mark(1);
+  if (stages & FO_PSTAGE_GRID) {
+    launch_per_plane(s, b->P, b->nk, b->F, b->pd, b->Cprod, b->tw->tw32, b->cand, b->grid);
+    mark(2);
+    launch_per_select(s, b->P, b->nk, b->F, b->pd, b->Cprod, b->tw->tw64, b->cand, b->scr,
+                      b->scr_stride, b->peaks);
+  } else {
+    mark(2);
+  }
+  mark(3);
+  if (stages & FO_PSTAGE_REFINE)
+    launch_per_refine(s, b->P, b->maxgroup, b->pd, b->grp, b->atoms, b->pos1, b->pos2, b->peaks,
+                      b->niter, b->perm, b->saveperm, b->rowsol, b->X1out, b->X2out, b->res);
+  mark(4);
+  FO_CHECK_LAUNCH();
+  return FO_OK;
+}
+
+namespace fo {
+
+int per_batch_plan(fo_context* c, const fo_per_batch_desc* d, fo_per_batch** out) {
+  if (!c || !d || !out) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (d->npairs < 1) return set_err(FO_ERR_INVALID, "npairs must be >= 1");
+  if (d->nk < 0 || d->nk > 31) return set_err(FO_ERR_UNSUPPORTED, "nk must be in [0, 31]");
+  if (d->F < 2 * d->nk + 1 || d->F > 512) return set_err(FO_ERR_UNSUPPORTED, "F must be in [2nk+1, 512]");
+  if (!d->natoms || !d->box || !d->scale || !d->ngroups)
+    return set_err(FO_ERR_INVALID, "natoms, box, scale and ngroups are required");
+  auto b = std::make_unique<fo_per_batch>();
+  b->ctx = c;
+  b->P = d->npairs;
+  b->nk = d->nk;
+  b->F = d->F;
+  b->niter = d->niter;
+  b->flags = d->flags;
+  const int NB = 2 * d->nk + 1;
+  b->cube = (long long)NB * NB * NB;
+  int gsum = 0, gatot = 0;
+  for (int p = 0; p < b->P; ++p) gsum += d->ngroups[p];
+  for (int q = 0; q < gsum; ++q) gatot += d->group_sizes[q];
+  b->gsum = gsum;
+  b->hp.resize(b->P);
+  b->hg.resize(std::max(gsum, 1));
+  b->hatoms.resize(std::max(gatot, 1));
+  long long Ntot = 0;
+  int gq = 0, ga = 0;
+  for (int p = 0; p < b->P; ++p) {
+    PerPair& pp = b->hp[p];
+    const int N = d->natoms[p];
+    if (N < 1) return set_err(FO_ERR_INVALID, "natoms < 1");
+    pp.natoms = N;
+    pp.atom_off = (int)Ntot;
+    pp.ngroups = d->ngroups[p];
+    pp.group_off = gq;
+    for (int k = 0; k < 3; ++k) {
+      pp.box[k] = d->box[3 * p + k];
+      if (!(pp.box[k] > 0)) return set_err(FO_ERR_INVALID, "box lengths must be > 0");
+    }
+    pp.scale = d->scale[p];
+    if (!(pp.scale > 0)) return set_err(FO_ERR_INVALID, "scale must be > 0");
+    pp.c_off = (long long)p * b->cube;
+    std::vector<char> seen(N, 0);
+    for (int q = 0; q < pp.ngroups; ++q) {
+      const int sz = d->group_sizes[gq + q];
+      if (sz > kMaxGroup) return set_err(FO_ERR_UNSUPPORTED, "permutation group too large");
+      b->hg[gq + q].size = sz;
+      b->hg[gq + q].atom_list_off = ga;
+      for (int i = 0; i < sz; ++i) {
+        const int a = d->group_atoms[ga + i];
+        if (a < 0 || a >= N) return set_err(FO_ERR_INVALID, "group atom index out of range");
+        if (seen[a]) return set_err(FO_ERR_INVALID, "atom listed in two permutation groups");
+        seen[a] = 1;
+        b->hatoms[ga + i] = a;
+      }
+      ga += sz;
+      b->maxgroup = std::max(b->maxgroup, sz);
+    }
+    gq += pp.ngroups;
+    Ntot += N;
+  }
+  b->Ntot = Ntot;
+  if (per_plane_smem(b->nk, b->F) > c->max_lds || per_sf_smem(b->nk) > c->max_lds)
+    return set_err(FO_ERR_UNSUPPORTED, "nk / F too large for LDS");
+  if (lap_lds_bytes(b->maxgroup) > c->max_lds)
+    return set_err(FO_ERR_UNSUPPORTED, "permutation group too large for the LDS assignment solver");
+  int rc = get_per_tw(c, b->F, &b->tw);
+  if (rc) return rc;
+  b->scr_stride = (long long)NB * NB + (long long)NB * b->F;
+  b->keepc = (b->flags & (FO_PER_KEEP_COEFFS | FO_PER_INPUT_COEFFS)) != 0;
+  Layout lay;
+  auto& o = b->off;
+  o.pd = lay.add<PerPair>(b->P);
+  o.grp = lay.add<PerGroup>(b->hg.size());
+  o.at = lay.add<int>(b->hatoms.size());
+  o.p1 = lay.add<double>(3 * Ntot);
+  o.p2 = lay.add<double>(3 * Ntot);
+  o.x1 = lay.add<double>(3 * Ntot);
+  o.x2 = lay.add<double>(3 * Ntot);
+  o.cp = lay.add<double2>((size_t)b->P * b->cube);
+  o.c1 = b->keepc ? lay.add<double2>((size_t)std::max(gsum, 1) * b->cube) : 0;
+  o.c2 = b->keepc ? lay.add<double2>((size_t)std::max(gsum, 1) * b->cube) : 0;
+  o.cand = lay.add<PlaneCand>((size_t)b->P * b->F);
+  o.pk = lay.add<PerPeak>(b->P);
+  o.res = lay.add<PerResult>(b->P);
+  o.perm = lay.add<int>(Ntot);
+  o.sp = lay.add<int>(Ntot);
+  o.rs = lay.add<int>(Ntot + 1);
+  o.scr = lay.add<double2>((size_t)b->P * b->scr_stride);
+  o.grid = (b->flags & FO_PER_KEEP_GRID) ? lay.add<float>((size_t)b->P * b->F * b->F * b->F) : 0;
+  b->bytes = lay.off;
+  *out = b.release();
+  return FO_OK;
+}
+
+size_t per_batch_bytes(const fo_per_batch* b) { return b->bytes; }
+
+int per_batch_bind(fo_per_batch* b, void* mem, hipStream_t stream) {
+  fo_context* c = b->ctx;
+  if (mem) {
+    b->base = mem;
+    b->own_mem = false;
+  } else {
+    int rc = ctx_alloc(c, b->bytes, &b->base);
+    if (rc) return rc;
+    b->own_mem = true;
+  }
+  const auto& o = b->off;
+  char* B = (char*)b->base;
+  b->pd = (PerPair*)(B + o.pd);
+  b->grp = (PerGroup*)(B + o.grp);
+  b->atoms = (int*)(B + o.at);
+  b->pos1 = (double*)(B + o.p1);
+  b->pos2 = (double*)(B + o.p2);
+  b->X1out = (double*)(B + o.x1);
+  b->X2out = (double*)(B + o.x2);
+  b->Cprod = (double2*)(B + o.cp);
+  b->C1 = b->keepc ? (double2*)(B + o.c1) : nullptr;
+  b->C2 = b->keepc ? (double2*)(B + o.c2) : nullptr;
+  b->cand = (PlaneCand*)(B + o.cand);
+  b->peaks = (PerPeak*)(B + o.pk);
+  b->res = (PerResult*)(B + o.res);
+  b->perm = (int*)(B + o.perm);
+  b->saveperm = (int*)(B + o.sp);
+  b->rowsol = (int*)(B + o.rs);
+  b->scr = (double2*)(B + o.scr);
+  b->grid = (b->flags & FO_PER_KEEP_GRID) ? (float*)(B + o.grid) : nullptr;
+  if (stream) {
+    b->stream = stream;
+    b->own_stream = false;
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return set_err(FO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    b->own_stream = true;
+  }
+  FO_HIP(hipMemcpyAsync(b->pd, b->hp.data(), sizeof(PerPair) * b->P, hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipMemcpyAsync(b->grp, b->hg.data(), sizeof(PerGroup) * b->hg.size(), hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipMemcpyAsync(b->atoms, b->hatoms.data(), sizeof(int) * b->hatoms.size(), hipMemcpyHostToDevice, b->stream));
+  return FO_OK;
+}
+
+int ctx_alloc_bytes(fo_context* c, size_t bytes, void** p) { return ctx_alloc(c, bytes, p); }
+void ctx_free_bytes(fo_context* c, void* p, size_t bytes) { ctx_free(c, p, bytes); }
+int ctx_device(const fo_context* c) { return c->device; }
+int set_error(int code, const char* msg) { return set_err(code, msg); }
+
+}  // namespace fo
+
+extern "C" {
+
+int fo_per_batch_create(fo_context* c, const fo_per_batch_desc* d, fo_per_batch** out) {
+  if (!out) return set_err(FO_ERR_INVALID, "out is NULL");
+  fo_per_batch* b = nullptr;
+  int rc = per_batch_plan(c, d, &b);
+  if (rc) return rc;
+  rc = per_batch_bind(b, nullptr, nullptr);
+  if (rc == FO_OK && hipStreamSynchronize(b->stream) != hipSuccess)
+    rc = set_err(FO_ERR_HIP, "descriptor upload failed");
+  if (rc) {
+    fo_per_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return FO_OK;
+}
+
+int fo_per_batch_destroy(fo_per_batch* b) {
+  if (!b) return FO_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (b->ev_made)
+    for (auto& e : b->ev) (void)hipEventDestroy(e);
+  if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->own_mem) ctx_free(b->ctx, b->base, b->bytes);
+  delete b;
+  return FO_OK;
+}
+
+int fo_per_batch_footprint(const fo_per_batch* b, unsigned long long* bytes) {
+  if (!b || !bytes) return set_err(FO_ERR_INVALID, "NULL argument");
+  *bytes = b->bytes;
+  return FO_OK;
+}
+
+int fo_per_batch_set_positions(fo_per_batch* b, const double* p1, const double* p2) {
+  if (!b || !p1 || !p2) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (!all_finite(p1, 3 * b->Ntot) || !all_finite(p2, 3 * b->Ntot))
+    return set_err(FO_ERR_INVALID, "positions must be finite");
+  FO_HIP(hipMemcpyAsync(b->pos1, p1, sizeof(double) * 3 * b->Ntot, hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipMemcpyAsync(b->pos2, p2, sizeof(double) * 3 * b->Ntot, hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_per_batch_set_positions_device(fo_per_batch* b, const void* p1, const void* p2) {
+  if (!b || !p1 || !p2) return set_err(FO_ERR_INVALID, "NULL argument");
+  FO_HIP(hipMemcpyAsync(b->pos1, p1, sizeof(double) * 3 * b->Ntot, hipMemcpyDeviceToDevice, b->stream));
+  FO_HIP(hipMemcpyAsync(b->pos2, p2, sizeof(double) * 3 * b->Ntot, hipMemcpyDeviceToDevice, b->stream));
+  return FO_OK;
+}
+
+int fo_per_batch_set_coeffs(fo_per_batch* b, const double* C1, const double* C2) {
+  if (!b || !C1 || !C2) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (!b->C1) return set_err(FO_ERR_STATE, "batch created without FO_PER_INPUT_COEFFS");
+  const size_t m = (size_t)b->gsum * b->cube;
+  FO_HIP(hipMemcpyAsync(b->C1, C1, sizeof(double2) * m, hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipMemcpyAsync(b->C2, C2, sizeof(double2) * m, hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_per_batch_set_disp(fo_per_batch* b, const double* disp) {
+  if (!b || !disp) return set_err(FO_ERR_INVALID, "NULL argument");
+  std::vector<PerPeak> pk(b->P);
+  for (int p = 0; p < b->P; ++p) {
+    memset(&pk[p], 0, sizeof(PerPeak));
+    for (int k = 0; k < 3; ++k) pk[p].disp[k] = disp[3 * p + k];
+  }
+  FO_HIP(hipMemcpyAsync(b->peaks, pk.data(), sizeof(PerPeak) * b->P, hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_per_batch_run(fo_per_batch* b, int stages, int use_graph) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  FO_HIP(hipSetDevice(b->ctx->device));
+  if (use_graph && !b->timing) {
+    auto it = b->graphs.find(stages);
+    if (it == b->graphs.end()) {
+      hipGraph_t g;
+      FO_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = per_enqueue(b, stages);
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return set_err(FO_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+      hipGraphExec_t ex;
+      FO_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      it = b->graphs.emplace(stages, ex).first;
+    }
+    FO_HIP(hipGraphLaunch(it->second, b->stream));
+    return FO_OK;
+  }
+  return per_enqueue(b, stages);
+}
+
+int fo_per_batch_sync(fo_per_batch* b) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_per_batch_results(fo_per_batch* b, double* dist, double* X1, double* X2, int* perm,
+                         double* disp) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  std::vector<PerResult> r(b->P);
+  FO_HIP(hipMemcpyAsync(r.data(), b->res, sizeof(PerResult) * b->P, hipMemcpyDeviceToHost, b->stream));
+  if (X1) FO_HIP(hipMemcpyAsync(X1, b->X1out, sizeof(double) * 3 * b->Ntot, hipMemcpyDeviceToHost, b->stream));
+  if (X2) FO_HIP(hipMemcpyAsync(X2, b->X2out, sizeof(double) * 3 * b->Ntot, hipMemcpyDeviceToHost, b->stream));
+  if (perm) FO_HIP(hipMemcpyAsync(perm, b->perm, sizeof(int) * b->Ntot, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  for (int p = 0; p < b->P; ++p) {
+    if (dist) dist[p] = r[p].dist;
+    if (disp)
+      for (int k = 0; k < 3; ++k) disp[3 * p + k] = r[p].disp[k];
+  }
+  return FO_OK;
+}
+
+int fo_per_batch_peaks(fo_per_batch* b, int* ind, double* disp0, double* peak_value, double* margin,
+                       double* delta, int* ncand, int* noverflow) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  std::vector<PerPeak> pk(b->P);
+  FO_HIP(hipMemcpyAsync(pk.data(), b->peaks, sizeof(PerPeak) * b->P, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  for (int p = 0; p < b->P; ++p) {
+    if (ind)
+      for (int k = 0; k < 3; ++k) ind[3 * p + k] = pk[p].ind[k];
+    if (disp0)
+      for (int k = 0; k < 3; ++k) disp0[3 * p + k] = pk[p].disp[k];
+    if (peak_value) peak_value[p] = pk[p].value;
+    if (margin) margin[p] = pk[p].margin;
+    if (delta) delta[p] = pk[p].delta;
+    if (ncand) ncand[p] = pk[p].ncand;
+    if (noverflow) noverflow[p] = pk[p].noverflow;
+  }
+  return FO_OK;
+}
+
+int fo_per_batch_coeffs(fo_per_batch* b, double* C1, double* C2) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  if (!b->C1) return set_err(FO_ERR_STATE, "batch created without FO_PER_KEEP_COEFFS");
+  const size_t m = (size_t)b->gsum * b->cube;
+  if (C1) FO_HIP(hipMemcpyAsync(C1, b->C1, sizeof(double2) * m, hipMemcpyDeviceToHost, b->stream));
+  if (C2) FO_HIP(hipMemcpyAsync(C2, b->C2, sizeof(double2) * m, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_per_batch_product(fo_per_batch* b, double* C) {
+  if (!b || !C) return set_err(FO_ERR_INVALID, "NULL argument");
+  FO_HIP(hipMemcpyAsync(C, b->Cprod, sizeof(double2) * b->P * b->cube, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_per_batch_grid(fo_per_batch* b, int pair, float* g) {
+  if (!b || !g) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (!b->grid) return set_err(FO_ERR_STATE, "batch created without FO_PER_KEEP_GRID");
+  if (pair < 0 || pair >= b->P) return set_err(FO_ERR_INVALID, "pair out of range");
+  const size_t f3 = (size_t)b->F * b->F * b->F;
+  FO_HIP(hipMemcpyAsync(g, b->grid + f3 * pair, sizeof(float) * f3, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_per_batch_timing(fo_per_batch* b, int enable, float* ms4) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  if (enable >= 0) {
+    if (enable && !b->ev_made) {
+      for (auto& e : b->ev) FO_HIP(hipEventCreate(&e));
+      b->ev_made = true;
+    }
+    b->timing = enable != 0;
+  }
+  if (ms4 && b->ev_made) {
+    FO_HIP(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < 4; ++i) FO_HIP(hipEventElapsedTime(&ms4[i], b->ev[i], b->ev[i + 1]));
+  }
+  return FO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// Internal host-side interface between the C ABI (fo_api.cpp) and the
+// serving engine (fo_server.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fastoverlap_hip.h"
+
+namespace fo {
+
+// Plan a batch: validate the descriptor, build host-side descriptors and the
+// memory layout, and resolve device tables; no batch memory is allocated.
+int sph_batch_plan(fo_context* ctx, const fo_sph_batch_desc* d, fo_sph_batch** out);
+int per_batch_plan(fo_context* ctx, const fo_per_batch_desc* d, fo_per_batch** out);
+// Device bytes the planned batch needs.
+size_t sph_batch_bytes(const fo_sph_batch* b);
+size_t per_batch_bytes(const fo_per_batch* b);
+// Bind memory and a stream.  mem == nullptr allocates (counted against the
+// context budget); stream == nullptr creates a private stream.  Uploads the
+// descriptors asynchronously on the stream.
+int sph_batch_bind(fo_sph_batch* b, void* mem, hipStream_t stream);
+int per_batch_bind(fo_per_batch* b, void* mem, hipStream_t stream);
+
+// Context allocation with budget accounting.
+int ctx_alloc_bytes(fo_context* c, size_t bytes, void** p);
+void ctx_free_bytes(fo_context* c, void* p, size_t bytes);
+int ctx_device(const fo_context* c);
+
+int set_error(int code, const char* msg);
+
+}  // namespace fo

@@ -1,0 +1,1284 @@
+// Spherical (cluster) overlap-alignment kernels for gfx950.
+//
+// Pipeline per batch of pairs sharing one SO(3) grid side n = 2B:
+//   k_wigner_table   FP64 Wigner-d table on the beta grid (cached per n)
+//   k_sph_prep       centroids, automatic kernel width
+//   k_sph_ylm        Y_lm of every grouped atom (FP64)
+//   k_sph_radial     scaled i_l(x) * Gaussian radial kernel (FP64)
+//   k_sph_contract   C^l = Y1 . K_l . Y2^H per degree (FP64)
+//   k_isoft_plane    inverse SOFT of both orientations per beta plane (FP32),
+//                    top candidates per plane
+//   k_sph_select     FP64 re-evaluation of every candidate within the FP32
+//                    error bound of the maximum -> exact argmax, parabolic
+//                    offsets, grid Euler angles
+//   k_sph_rotopt     FP64 Newton maximisation of Re sum conj(I) D(R)
+//   k_sph_finish     orientation choice, LAP, quaternion fit (FP64)
+#include "fo_common.h"
+#include "fo_internal.h"
+#include "fo_refine.h"
+#include <algorithm>
+
+namespace fo {
+
+// ---------------------------------------------------------------------------
+// Wigner-d table: tab[k * Kt + kidx(l,m1,m2)] = sqrt((2l+1)/2) d^l_{m1m2}(b_k)
+// with b_k = pi (2k+1) / (2n)  (fastoverlap/soft.py:55, :73-96).  Evaluated
+// with the Kostelec-Rockmore three-term recurrence in l (the method of
+// f90/DSOFT.f90:81-195), seeded at l0 = max(|m1|,|m2|) by the closed form.
+// ---------------------------------------------------------------------------
+FO_DEV void recurrence_terms(int l, int m1, int m2, double& A, double& B, double& C) {
+  const double dl = (double)l;
+  const double t1 = sqrt((2.0 * dl + 3.0) / (2.0 * dl + 1.0));
+  const double t3 = (dl + 1.0) * (2.0 * dl + 1.0);
+  const double t5 = 1.0 / sqrt(((dl + 1.0) * (dl + 1.0) - (double)m1 * m1) *
+                               ((dl + 1.0) * (dl + 1.0) - (double)m2 * m2));
+  B = t1 * t3 * t5;
+  if (l == 0) {
+    A = 0.0;
+    C = 0.0;
+    return;
+  }
+  const double t2 = sqrt((2.0 * dl + 3.0) / (2.0 * dl - 1.0)) * (dl + 1.0) / dl;
+  const double t4 = sqrt((dl * dl - (double)m1 * m1) * (dl * dl - (double)m2 * m2));
+  A = t2 * t4 * t5;
+  C = (double)m1 * m2 / (dl * (dl + 1.0));
+}
+
+// log of sqrt(binomial(mu+nu, mu)) and the sign xi of the edge element
+FO_DEV double seed_log_factor(int mu, int nu) {
+  return 0.5 * (lgamma((double)(mu + nu + 1)) - lgamma((double)(mu + 1)) -
+                lgamma((double)(nu + 1)));
+}
+FO_DEV double seed_xi(int m1, int m2) {
+  return (m2 < m1 && ((m1 - m2) & 1)) ? -1.0 : 1.0;
+}
+
+__global__ void k_wigner_table(int n, int Lt, double* __restrict__ tab64,
+                               float* __restrict__ tab32) {
+  const int W = 2 * Lt + 1;
+  const long long total = (long long)n * W * W;
+  const long long Kt = kcount(Lt);
+  for (long long w = blockIdx.x * (long long)blockDim.x + threadIdx.x; w < total;
+       w += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(w / (W * W));
+    const int rem = (int)(w % (W * W));
+    const int m1 = rem / W - Lt, m2 = rem % W - Lt;
+    const double beta = kPi * (2.0 * k + 1.0) / (2.0 * n);
+    const double c2 = cos(0.5 * beta), s2 = sin(0.5 * beta), cb = cos(beta);
+    const int l0 = max(abs(m1), abs(m2));
+    const int mu = abs(m1 - m2), nu = abs(m1 + m2);
+    const double seed = sqrt((2.0 * l0 + 1.0) / 2.0) * seed_xi(m1, m2) *
+                        exp(seed_log_factor(mu, nu) + mu * log(s2) + nu * log(c2));
+    double prev = 0.0, cur = seed;
+    double* row64 = tab64 + (long long)k * Kt;
+    float* row32 = tab32 + (long long)k * Kt;
+    long long id = kidx(l0, m1, m2);
+    row64[id] = cur;
+    row32[id] = (float)cur;
+    for (int l = l0; l < Lt; ++l) {
+      double A, B, C;
+      recurrence_terms(l, m1, m2, A, B, C);
+      const double nxt = B * (cb - C) * cur - A * prev;
+      prev = cur;
+      cur = nxt;
+      id = kidx(l + 1, m1, m2);
+      row64[id] = cur;
+      row32[id] = (float)cur;
+    }
+  }
+}
+
+// twiddles e^{+2 pi i k / n} (inverse DFT sign of fastoverlap/soft.py:124-125)
+__global__ void k_twiddles(int n, double sign, double2* tw64, float2* tw32) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    double s, c;
+    sincospi(sign * 2.0 * (double)k / (double)n, &s, &c);
+    tw64[k] = make_double2(c, s);
+    tw32[k] = make_float2((float)c, (float)s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Prep: centroids (fastoverlap/sphericalAlignment.py:129-137) and automatic
+// kernel width (mean nearest-neighbour distance / 3 averaged over the two
+// structures, :35-41 and :145-147).
+// ---------------------------------------------------------------------------
+__global__ void k_sph_prep(const SphPair* __restrict__ pd, const double* __restrict__ pos1,
+                           const double* __restrict__ pos2, double* __restrict__ X1c,
+                           double* __restrict__ X2c, double* __restrict__ scale_eff,
+                           int center) {
+  __shared__ double red[16];
+  const SphPair d = pd[blockIdx.x];
+  const int N = d.natoms, t = threadIdx.x, bd = blockDim.x;
+  const double* p1 = pos1 + 3LL * d.atom_off;
+  const double* p2 = pos2 + 3LL * d.atom_off;
+  double c[6] = {0, 0, 0, 0, 0, 0};
+  for (int a = t; a < N; a += bd)
+    for (int q = 0; q < 3; ++q) {
+      c[q] += p1[3 * a + q];
+      c[3 + q] += p2[3 * a + q];
+    }
+  for (int q = 0; q < 6; ++q) c[q] = block_sum(c[q], red) / (double)N;
+  if (!center)
+    for (int q = 0; q < 6; ++q) c[q] = 0.0;
+  double* x1 = X1c + 3LL * d.atom_off;
+  double* x2 = X2c + 3LL * d.atom_off;
+  for (int a = t; a < N; a += bd)
+    for (int q = 0; q < 3; ++q) {
+      x1[3 * a + q] = p1[3 * a + q] - c[q];
+      x2[3 * a + q] = p2[3 * a + q] - c[3 + q];
+    }
+  if (d.scale > 0.0) {
+    if (t == 0) scale_eff[blockIdx.x] = d.scale;
+    return;
+  }
+  double s1 = 0.0, s2 = 0.0;
+  for (int a = t; a < N; a += bd) {
+    double m1 = 1e300, m2 = 1e300;
+    for (int b = 0; b < N; ++b) {
+      if (b == a) continue;
+      double dx = p1[3 * a] - p1[3 * b], dy = p1[3 * a + 1] - p1[3 * b + 1],
+             dz = p1[3 * a + 2] - p1[3 * b + 2];
+      m1 = fmin(m1, dx * dx + dy * dy + dz * dz);
+      dx = p2[3 * a] - p2[3 * b];
+      dy = p2[3 * a + 1] - p2[3 * b + 1];
+      dz = p2[3 * a + 2] - p2[3 * b + 2];
+      m2 = fmin(m2, dx * dx + dy * dy + dz * dz);
+    }
+    s1 += sqrt(m1);
+    s2 += sqrt(m2);
+  }
+  s1 = block_sum(s1, red);
+  s2 = block_sum(s2, red);
+  if (t == 0) scale_eff[blockIdx.x] = (s1 / N + s2 / N) / 6.0;
+}
+
+// ---------------------------------------------------------------------------
+// Y_l^m (orthonormal, Condon-Shortley phase: scipy's sph_harm as called at
+// fastoverlap/sphericalAlignment.py:57-65), FP64, via the normalised
+// associated-Legendre recurrence.  Layout: Y[ylm_off + (l*l+l+m)*ngath + g].
+// An atom at the centre takes theta = phi = 0 (only l = 0 survives there).
+// ---------------------------------------------------------------------------
+__global__ void k_sph_ylm(const SphPair* __restrict__ pd, const int* __restrict__ gath,
+                          const double* __restrict__ X1c, const double* __restrict__ X2c,
+                          double2* __restrict__ Y1, double2* __restrict__ Y2) {
+  const SphPair d = pd[blockIdx.x];
+  const double* X = blockIdx.y ? X2c : X1c;
+  double2* Y = (blockIdx.y ? Y2 : Y1) + d.ylm_off;
+  const int L = d.L, Ng = d.ngath;
+  const double inv4pi = 1.0 / (4.0 * kPi);
+  for (int g = threadIdx.x; g < Ng; g += blockDim.x) {
+    const int a = gath[d.gath_off + g];
+    const double x = X[3LL * (d.atom_off + a)], y = X[3LL * (d.atom_off + a) + 1],
+                 z = X[3LL * (d.atom_off + a) + 2];
+    const double rxy = sqrt(x * x + y * y), r = sqrt(x * x + y * y + z * z);
+    double ct = 1.0, st = 0.0, cp = 1.0, sp = 0.0;
+    if (r > 0.0) {
+      ct = z / r;
+      st = rxy / r;
+      const double phi = atan2(y, x);
+      sincos(phi, &sp, &cp);
+    }
+    double pmm = sqrt(inv4pi);  // P^0_0
+    double2 em = make_double2(1.0, 0.0);  // e^{i m phi}
+    for (int m = 0; m <= L; ++m) {
+      if (m > 0) {
+        pmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m)) * st;
+        em = cmul(em, make_double2(cp, sp));
+      }
+      double p2 = 0.0, p1 = pmm;
+      for (int l = m; l <= L; ++l) {
+        double pl;
+        if (l == m) {
+          pl = pmm;
+        } else if (l == m + 1) {
+          pl = sqrt(2.0 * m + 3.0) * ct * pmm;
+        } else {
+          const double a = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+          const double b = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) /
+                                (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+          pl = a * (ct * p1 - b * p2);
+        }
+        if (l > m) {
+          p2 = p1;
+          p1 = pl;
+        }
+        const double2 yp = cscale(em, pl);
+        Y[(long long)(l * l + l + m) * Ng + g] = yp;
+        if (m > 0) {
+          const double sg = (m & 1) ? -1.0 : 1.0;
+          Y[(long long)(l * l + l - m) * Ng + g] = make_double2(sg * yp.x, -sg * yp.y);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Radial kernel il[l,j,k] = i_l(x) exp(-(r1^2 + r2^2)/4s^2), x = r1 r2/(2s^2)
+// (fastoverlap/sphericalAlignment.py:269-272), evaluated overflow-free as
+// [i_l(x) e^{-x}] exp(-(r1-r2)^2/4s^2) in FP64.  i_l(x) e^{-x}: power series
+// for x < 1; otherwise Miller's backward recurrence started at
+// l_s = ceil(sqrt(L^2 + 40x)) + 10 (where the minimal-solution
+// contamination is below 1e-17) and normalised by i_0(x) e^{-x} =
+// -expm1(-2x)/(2x).  Layout: R[rad_off + grp.rad_off + (l*Ns + j)*Ns + k].
+// ---------------------------------------------------------------------------
+FO_DEV void radial_store(int L, double x, double gauss, double* out, long long stride) {
+  if (x == 0.0) {
+    out[0] = gauss;
+    for (int l = 1; l <= L; ++l) out[l * stride] = 0.0;
+    return;
+  }
+  if (x < 1.0) {
+    const double ex = exp(-x), h = 0.5 * x * x;
+    double lead = 1.0;  // x^l / (2l+1)!!
+    for (int l = 0; l <= L; ++l) {
+      if (l > 0) lead *= x / (2.0 * l + 1.0);
+      double term = 1.0, sum = 1.0;
+      for (int k = 1; k < 40; ++k) {
+        term *= h / (k * (2.0 * l + 2.0 * k + 1.0));
+        sum += term;
+        if (term < 1e-18 * sum) break;
+      }
+      out[l * stride] = lead * sum * ex * gauss;
+    }
+    return;
+  }
+  const int ls = (int)ceil(sqrt((double)L * L + 40.0 * x)) + 10;
+  const double ix = 1.0 / x;
+  // pass 1: normalisation
+  double fp1 = 0.0, f = 1.0;
+  for (int l = ls; l > 0; --l) {
+    const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
+    fp1 = f;
+    f = fm;
+  }
+  const double i0 = -expm1(-2.0 * x) / (2.0 * x);
+  const double scal = i0 / f * gauss;
+  // pass 2: store
+  fp1 = 0.0;
+  f = 1.0;
+  for (int l = ls; l >= 0; --l) {
+    if (l <= L) out[l * stride] = f * scal;
+    if (l == 0) break;
+    const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
+    fp1 = f;
+    f = fm;
+  }
+}
+
+__global__ void k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
+                             const int* __restrict__ gath, const double* __restrict__ X1c,
+                             const double* __restrict__ X2c, const double* __restrict__ scale_eff,
+                             double* __restrict__ R) {
+  const SphPair d = pd[blockIdx.y];
+  const double s = scale_eff[blockIdx.y];
+  const double i2s2 = 1.0 / (2.0 * s * s), i4s2 = 1.0 / (4.0 * s * s);
+  const SphGroup* g = grp + d.group_off;
+  long long total = 0;
+  for (int q = 0; q < d.ngroups; ++q) total += (long long)g[q].size * g[q].size;
+  for (long long w = blockIdx.x * (long long)blockDim.x + threadIdx.x; w < total;
+       w += (long long)gridDim.x * blockDim.x) {
+    int q = 0;
+    while (q + 1 < d.ngroups && w >= g[q + 1].jk_off) ++q;
+    const int Ns = g[q].size;
+    const long long jk = w - g[q].jk_off;
+    const int j = (int)(jk / Ns), k = (int)(jk % Ns);
+    const int a1 = gath[d.gath_off + g[q].gath_off + j];
+    const int a2 = gath[d.gath_off + g[q].gath_off + k];
+    const double* x1 = X1c + 3LL * (d.atom_off + a1);
+    const double* x2 = X2c + 3LL * (d.atom_off + a2);
+    const double r1 = sqrt(x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2]);
+    const double r2 = sqrt(x2[0] * x2[0] + x2[1] * x2[1] + x2[2] * x2[2]);
+    const double x = r1 * r2 * i2s2;
+    const double gauss = exp(-(r1 - r2) * (r1 - r2) * i4s2);
+    radial_store(d.L, x, gauss, R + d.rad_off + g[q].rad_off + (long long)j * Ns + k,
+                 (long long)Ns * Ns);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Contraction I^l_{m1m2} = 4 pi^{5/2} s^3 sum_s sum_{j,k in s}
+//   Y1[l,m1,j] K_l[j,k] conj(Y2[l,m2,k])      (sphericalAlignment.py:273,:175)
+// One workgroup per (pair, l, block of kMB rows m1); per k-tile the
+// intermediate A = Y1 . K_l is formed in LDS, then accumulated against Y2^H.
+// ---------------------------------------------------------------------------
+constexpr int kMB = 16;   // m1 rows per workgroup
+constexpr int kKT = 32;   // atoms of structure 2 per tile
+constexpr int kAccMax = 8;  // kMB * (2L+1) / 256 <= 8  ->  L <= 63
+
+__global__ void __launch_bounds__(256) k_sph_contract(
+    const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
+    const double2* __restrict__ Y1, const double2* __restrict__ Y2,
+    const double* __restrict__ R, const double* __restrict__ scale_eff,
+    double2* __restrict__ I64, float2* __restrict__ I32) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const SphPair d = pd[blockIdx.y];
+  const int L = d.L;
+  // decode (l, m1 block) from blockIdx.x
+  int b = blockIdx.x, l = 0;
+  for (; l <= L; ++l) {
+    const int nb = (2 * l + 1 + kMB - 1) / kMB;
+    if (b < nb) break;
+    b -= nb;
+  }
+  if (l > L) return;
+  const int M = 2 * l + 1;
+  const int m1a = b * kMB;  // row offset within [0, M)
+  const int rows = min(kMB, M - m1a);
+  double2* As = (double2*)smem;          // [kMB][kKT]
+  double2* Ys = As + kMB * kKT;          // [M][kKT]
+  const int t = threadIdx.x;
+  double2 acc[kAccMax];
+  for (int q = 0; q < kAccMax; ++q) acc[q] = make_double2(0.0, 0.0);
+  const int Ng = d.ngath;
+  const double2* y1 = Y1 + d.ylm_off + (long long)(l * l) * Ng;
+  const double2* y2 = Y2 + d.ylm_off + (long long)(l * l) * Ng;
+  for (int q = 0; q < d.ngroups; ++q) {
+    const SphGroup G = grp[d.group_off + q];
+    const int Ns = G.size;
+    const double* Rl = R + d.rad_off + G.rad_off + (long long)l * Ns * Ns;
+    for (int k0 = 0; k0 < Ns; k0 += kKT) {
+      const int kt = min(kKT, Ns - k0);
+      for (int e = t; e < M * kKT; e += blockDim.x) {
+        const int m2 = e / kKT, kk = e % kKT;
+        Ys[e] = kk < kt ? y2[(long long)m2 * Ng + G.gath_off + k0 + kk] : make_double2(0.0, 0.0);
+      }
+      for (int e = t; e < kMB * kKT; e += blockDim.x) {
+        const int r = e / kKT, kk = e % kKT;
+        double2 a = make_double2(0.0, 0.0);
+        if (r < rows && kk < kt) {
+          const double2* yr = y1 + (long long)(m1a + r) * Ng + G.gath_off;
+          const double* rc = Rl + k0 + kk;
+          for (int j = 0; j < Ns; ++j) {
+            const double kv = rc[(long long)j * Ns];
+            const double2 yv = yr[j];
+            a.x += yv.x * kv;
+            a.y += yv.y * kv;
+          }
+        }
+        As[e] = a;
+      }
+      __syncthreads();
+      for (int q2 = 0; q2 < kAccMax; ++q2) {
+        const int e = t + q2 * blockDim.x;
+        if (e >= rows * M) break;
+        const int r = e / M, m2 = e % M;
+        double2 s = acc[q2];
+        for (int kk = 0; kk < kt; ++kk) {
+          const double2 a = As[r * kKT + kk];
+          const double2 y = Ys[m2 * kKT + kk];
+          // a * conj(y)
+          s.x += a.x * y.x + a.y * y.y;
+          s.y += a.y * y.x - a.x * y.y;
+        }
+        acc[q2] = s;
+      }
+      __syncthreads();
+    }
+  }
+  const double sc = scale_eff[blockIdx.y];
+  const double norm = 4.0 * pow(kPi, 2.5) * sc * sc * sc;
+  for (int q2 = 0; q2 < kAccMax; ++q2) {
+    const int e = t + q2 * blockDim.x;
+    if (e >= rows * M) break;
+    const int r = e / M, m2i = e % M;
+    const long long id = d.coef_off + kidx(l, m1a + r - l, m2i - l);
+    const double2 v = cscale(acc[q2], norm);
+    I64[id] = v;
+    I32[id] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Inverse SOFT of two Hermitian coefficient sets, one beta plane per
+// workgroup (fastoverlap/soft.py:115-125).  In alignment mode the second set
+// is the inverted orientation, B^l = (-1)^l A^l, so one beta contraction
+// serves both: with E/O the even/odd-l partial sums, S_A = E + O and
+// S_B = E - O.  Hermitian symmetry S[-m1,-m2] = conj S[m1,m2] makes the
+// grid real, so only m1 >= 0 rows are contracted and transformed:
+//   T[m1][g]  = sum_{m2} S[m1][m2] e^{i m2 g}         (gamma pass)
+//   out[a][g] = Re T[0][g] + 2 sum_{m1>0} Re(T[m1][g] e^{i m1 a})   (alpha pass)
+// Each plane keeps its kCandPerPlane largest values and an upper bound on
+// the rest, for the FP64 selection kernel.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_isoft_plane(
+    const SphPair* __restrict__ pd, int n, long long Kt, const float* __restrict__ tab32,
+    const float2* __restrict__ twg, const float2* __restrict__ IA,
+    const float2* __restrict__ IB, int mode, int norient, PlaneCand* __restrict__ cand,
+    float* __restrict__ grid) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ib = blockIdx.x, p = blockIdx.y;
+  const SphPair d = pd[p];
+  const int L = d.L, W = 2 * L + 1, t = threadIdx.x, bd = blockDim.x;
+  float2* tw = (float2*)smem;                 // [n]
+  float2* S = tw + n;                          // [2][L+1][W]
+  float2* T = S + 2 * (L + 1) * W;             // [2][L+1][n]
+  float* rv = (float*)(T + 2 * (L + 1) * n);   // reduction scratch
+  int* ri = (int*)(rv + 16);
+  for (int k = t; k < n; k += bd) tw[k] = twg[k];
+  const float* tb = tab32 + (long long)ib * Kt;
+  const float2* ia = IA + d.coef_off;
+  const float2* ibb = IB + d.coef_off;
+  // beta contraction
+  for (int e = t; e < (L + 1) * W; e += bd) {
+    const int m1 = e / W, m2 = e % W - L;
+    const int l0 = max(m1, abs(m2));
+    float2 s0 = make_float2(0.f, 0.f), s1 = make_float2(0.f, 0.f);
+    if (mode == 0) {
+      for (int l = l0; l <= L; ++l) {
+        const long long id = kidx(l, m1, m2);
+        const float dv = tb[id];
+        const float2 c = ia[id];
+        if (l & 1) {
+          s1.x += dv * c.x;
+          s1.y += dv * c.y;
+        } else {
+          s0.x += dv * c.x;
+          s0.y += dv * c.y;
+        }
+      }
+      S[e] = make_float2(s0.x + s1.x, s0.y + s1.y);
+      S[(L + 1) * W + e] = make_float2(s0.x - s1.x, s0.y - s1.y);
+    } else {
+      for (int l = l0; l <= L; ++l) {
+        const long long id = kidx(l, m1, m2);
+        const float dv = tb[id];
+        const float2 ca = ia[id], cb = ibb[id];
+        s0.x += dv * ca.x;
+        s0.y += dv * ca.y;
+        s1.x += dv * cb.x;
+        s1.y += dv * cb.y;
+      }
+      S[e] = s0;
+      S[(L + 1) * W + e] = s1;
+    }
+  }
+  __syncthreads();
+  // gamma pass
+  for (int e = t; e < norient * (L + 1) * n; e += bd) {
+    const int o = e / ((L + 1) * n);
+    const int rem = e % ((L + 1) * n);
+    const int m1 = rem / n, g = rem % n;
+    const float2* Sr = S + (o * (L + 1) + m1) * W;
+    float2 acc = make_float2(0.f, 0.f);
+    int ph = (n - (L % n) * g % n) % n;  // (-L * g) mod n
+    for (int m2i = 0; m2i < W; ++m2i) {
+      const float2 w = tw[ph];
+      const float2 s = Sr[m2i];
+      acc.x += s.x * w.x - s.y * w.y;
+      acc.y += s.x * w.y + s.y * w.x;
+      ph += g;
+      if (ph >= n) ph -= n;
+    }
+    T[(o * (L + 1) + m1) * n + g] = acc;
+  }
+  __syncthreads();
+  // alpha pass + per-thread candidates
+  float bv[2] = {-INFINITY, -INFINITY}, dm[2] = {-INFINITY, -INFINITY};
+  int bi[2] = {0x7fffffff, 0x7fffffff};
+  for (int e = t; e < n * n; e += bd) {
+    const int a = e / n, g = e % n;
+    for (int o = 0; o < norient; ++o) {
+      const float2* To = T + o * (L + 1) * n;
+      float acc = 0.f;
+      int ph = 0;
+      for (int m1 = 1; m1 <= L; ++m1) {
+        ph += a;
+        if (ph >= n) ph -= n;
+        const float2 w = tw[ph];
+        const float2 v = To[m1 * n + g];
+        acc += v.x * w.x - v.y * w.y;
+      }
+      const float val = To[g].x + 2.f * acc;
+      const int flat = (a * n + ib) * n + g;
+      if (grid) grid[((long long)p * norient + o) * n * n * n + flat] = val;
+      if (val > bv[o]) {
+        dm[o] = fmaxf(dm[o], bv[o]);
+        bv[o] = val;
+        bi[o] = flat;
+      } else {
+        dm[o] = fmaxf(dm[o], val);
+      }
+    }
+  }
+  // plane top-kCandPerPlane + bound on the rest
+  for (int o = 0; o < norient; ++o) {
+    bool taken = false;
+    PlaneCand out;
+    float fifth = -INFINITY;
+    for (int r = 0; r <= kCandPerPlane; ++r) {
+      float v = taken ? -INFINITY : bv[o];
+      int i = taken ? 0x7fffffff : bi[o];
+      block_argmax(v, i, rv, ri);
+      if (r < kCandPerPlane) {
+        out.val[r] = v;
+        out.idx[r] = i;
+      } else {
+        fifth = v;
+      }
+      if (!taken && i == bi[o] && v == bv[o]) taken = true;
+    }
+    float dmax = dm[o];
+    int dummy = 0;
+    block_argmax(dmax, dummy, rv, ri);
+    out.rest = fmaxf(fifth, dmax);
+    if (t == 0) cand[((long long)p * norient + o) * n + ib] = out;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 evaluation of the overlap grid at one point (workgroup cooperative):
+//   g(a,b,c) = sum_{m1>=0,m2} w_m1 Re( S64[m1,m2](b) e^{i(m1 alpha + m2 gamma)} )
+// with S64 = sum_l sgn^l dtilde64 I64 (alignment mode) or the B set.
+// ---------------------------------------------------------------------------
+struct GridCtx {
+  int n, L;
+  long long Kt;
+  const double* tab64;
+  const double2* tw64;
+  const double2* I;   // pair's FP64 coefficients (A set)
+  const double2* IB;  // B set (general mode) or nullptr
+  double sgn;         // +1 / -1 (alignment mode)
+};
+
+FO_DEV double grid_eval64(const GridCtx& c, int ia, int ib, int ig, double* red) {
+  const int L = c.L, W = 2 * L + 1, n = c.n;
+  const double* tb = c.tab64 + (long long)ib * c.Kt;
+  const double2* I = c.IB ? c.IB : c.I;
+  const double sg = c.IB ? 1.0 : c.sgn;
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < (L + 1) * W; e += blockDim.x) {
+    const int m1 = e / W, m2 = e % W - L;
+    const int l0 = max(m1, abs(m2));
+    double sx = 0.0, sy = 0.0;
+    double sl = (l0 & 1) ? sg : 1.0;
+    for (int l = l0; l <= L; ++l) {
+      const long long id = kidx(l, m1, m2);
+      const double dv = tb[id] * sl;
+      sx += dv * I[id].x;
+      sy += dv * I[id].y;
+      sl *= sg;
+    }
+    int ph = (int)(((long long)m1 * ia + (long long)m2 * ig) % n);
+    if (ph < 0) ph += n;
+    const double2 w = c.tw64[ph];
+    acc += (m1 == 0 ? 1.0 : 2.0) * (sx * w.x - sy * w.y);
+  }
+  return block_sum(acc, red);
+}
+
+// Full FP64 evaluation of one beta plane; returns its argmax (ties -> lower
+// flat index).  Uses global scratch Sg[(L+1)W], Tg[(L+1)n].
+FO_DEV void plane_eval64(const GridCtx& c, int ib, double2* Sg, double2* Tg, double* red,
+                         int* redi, double& bestv, int& besti) {
+  const int L = c.L, W = 2 * L + 1, n = c.n;
+  const double* tb = c.tab64 + (long long)ib * c.Kt;
+  const double2* I = c.IB ? c.IB : c.I;
+  const double sg = c.IB ? 1.0 : c.sgn;
+  for (int e = threadIdx.x; e < (L + 1) * W; e += blockDim.x) {
+    const int m1 = e / W, m2 = e % W - L;
+    const int l0 = max(m1, abs(m2));
+    double sx = 0.0, sy = 0.0, sl = (l0 & 1) ? sg : 1.0;
+    for (int l = l0; l <= L; ++l) {
+      const long long id = kidx(l, m1, m2);
+      const double dv = tb[id] * sl;
+      sx += dv * I[id].x;
+      sy += dv * I[id].y;
+      sl *= sg;
+    }
+    Sg[e] = make_double2(sx, sy);
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (int e = threadIdx.x; e < (L + 1) * n; e += blockDim.x) {
+    const int m1 = e / n, g = e % n;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int m2i = 0; m2i < W; ++m2i) {
+      int ph = (int)(((long long)(m2i - L) * g) % n);
+      if (ph < 0) ph += n;
+      acc = cadd(acc, cmul(Sg[m1 * W + m2i], c.tw64[ph]));
+    }
+    Tg[e] = acc;
+  }
+  __threadfence_block();
+  __syncthreads();
+  double bv = -INFINITY;
+  int bi = 0x7fffffff;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int a = e / n, g = e % n;
+    double acc = Tg[g].x;
+    for (int m1 = 1; m1 <= L; ++m1) {
+      const int ph = (int)(((long long)m1 * a) % n);
+      const double2 v = Tg[m1 * n + g];
+      const double2 w = c.tw64[ph];
+      acc += 2.0 * (v.x * w.x - v.y * w.y);
+    }
+    argmax_merge(bv, bi, acc, (a * n + ib) * n + g);
+  }
+  block_argmax(bv, bi, red, redi);
+  bestv = bv;
+  besti = bi;
+  __syncthreads();
+}
+
+// Selection: exact argmax of the grid of one (pair, orientation).
+// delta bounds the FP32 grid error: |g| <= A = sum_l sqrt((2l+1)/2) sum|I^l|,
+// and every FP32 stage (coefficient and table rounding, <= L+1 products per
+// beta sum, the two DFT passes of <= 2L+1 terms with rounded twiddles) adds
+// at most a few units of 2^-24 A each; delta = (8(L+1) + 32) 2^-24 A covers
+// their sum.  Every grid point within delta of the FP32 maximum is
+// re-evaluated in FP64; planes whose unlisted values could be within delta
+// are re-evaluated in full.
+
+__global__ void __launch_bounds__(256) k_sph_select(
+    const SphPair* __restrict__ pd, int n, long long Kt, const double* __restrict__ tab64,
+    const double2* __restrict__ tw64, const double2* __restrict__ IA,
+    const double2* __restrict__ IB, int mode, int norient, const PlaneCand* __restrict__ cand,
+    double2* __restrict__ scratch, long long scratch_stride, SphPeak* __restrict__ peaks) {
+  __shared__ double red[16];
+  __shared__ int redi[16];
+  __shared__ int s_cnt, s_ovf;
+  __shared__ int s_cand[kMaxCand];
+  __shared__ int s_ovfl[256];
+  const int p = blockIdx.x, o = blockIdx.y;
+  const SphPair d = pd[p];
+  const int L = d.L, t = threadIdx.x;
+  GridCtx c;
+  c.n = n;
+  c.L = L;
+  c.Kt = Kt;
+  c.tab64 = tab64;
+  c.tw64 = tw64;
+  c.I = IA + d.coef_off;
+  c.IB = (mode == 1 && o == 1) ? IB + d.coef_off : nullptr;
+  c.sgn = (mode == 0 && o == 1) ? -1.0 : 1.0;
+  const PlaneCand* pc = cand + ((long long)p * norient + o) * n;
+  // FP32 maximum over planes
+  float mv = -INFINITY;
+  int mi = 0x7fffffff;
+  for (int b = t; b < n; b += blockDim.x) argmax_merge(mv, mi, pc[b].val[0], pc[b].idx[0]);
+  {
+    __shared__ float rv[16];
+    __shared__ int ri[16];
+    block_argmax(mv, mi, rv, ri);
+  }
+  // error bound
+  const double2* Iuse = c.IB ? c.IB : c.I;
+  double A = 0.0;
+  const long long K = kcount(L);
+  for (long long e = t; e < K; e += blockDim.x) {
+    int l = 0;
+    while (kbase(l + 1) <= e) ++l;
+    A += sqrt((2.0 * l + 1.0) * 0.5) * hypot(Iuse[e].x, Iuse[e].y);
+  }
+  A = block_sum(A, red);
+  const double delta = (8.0 * (L + 1) + 32.0) * 5.9604644775390625e-08 * A;
+  const double thr = (double)mv - delta;
+  if (t == 0) {
+    s_cnt = 0;
+    s_ovf = 0;
+  }
+  __syncthreads();
+  for (int b = t; b < n; b += blockDim.x) {
+    if ((double)pc[b].rest >= thr) {
+      int q = atomicAdd(&s_ovf, 1);
+      s_ovfl[q] = b;
+    } else {
+      for (int r = 0; r < kCandPerPlane; ++r)
+        if ((double)pc[b].val[r] >= thr) {
+          int q = atomicAdd(&s_cnt, 1);
+          if (q < kMaxCand) s_cand[q] = pc[b].idx[r];
+        }
+    }
+  }
+  __syncthreads();
+  const int ncand = min(s_cnt, kMaxCand), novf = s_ovf;
+  const bool cand_overflow = s_cnt > kMaxCand;
+  // sort candidate list (thread 0; small) for a launch-independent order
+  if (t == 0) {
+    for (int i = 1; i < ncand; ++i) {
+      int v = s_cand[i], j = i - 1;
+      while (j >= 0 && s_cand[j] > v) {
+        s_cand[j + 1] = s_cand[j];
+        --j;
+      }
+      s_cand[j + 1] = v;
+    }
+    for (int i = 1; i < novf; ++i) {
+      int v = s_ovfl[i], j = i - 1;
+      while (j >= 0 && s_ovfl[j] > v) {
+        s_ovfl[j + 1] = s_ovfl[j];
+        --j;
+      }
+      s_ovfl[j + 1] = v;
+    }
+  }
+  __syncthreads();
+  double best = -INFINITY, second = -INFINITY;
+  int besti = 0x7fffffff;
+  double2* Sg = scratch + ((long long)p * norient + o) * scratch_stride;
+  double2* Tg = Sg + (L + 1) * (2 * L + 1);
+  if (cand_overflow) {
+    // too many near-maximal points to list: evaluate every plane in FP64
+    for (int b = 0; b < n; ++b) {
+      double v;
+      int i;
+      plane_eval64(c, b, Sg, Tg, red, redi, v, i);
+      if (v > best || (v == best && i < besti)) {
+        second = best;
+        best = v;
+        besti = i;
+      } else if (v > second) {
+        second = v;
+      }
+    }
+  } else {
+    for (int q = 0; q < ncand; ++q) {
+      const int f = s_cand[q];
+      const int ia = f / (n * n), ib = (f / n) % n, ig = f % n;
+      const double v = grid_eval64(c, ia, ib, ig, red);
+      if (v > best || (v == best && f < besti)) {
+        second = best;
+        best = v;
+        besti = f;
+      } else if (v > second) {
+        second = v;
+      }
+    }
+    for (int q = 0; q < novf; ++q) {
+      double v;
+      int i;
+      plane_eval64(c, s_ovfl[q], Sg, Tg, red, redi, v, i);
+      if (v > best || (v == best && i < besti)) {
+        second = best;
+        best = v;
+        besti = i;
+      } else if (v > second) {
+        second = v;
+      }
+    }
+  }
+  // parabolic offsets on |g| with periodic wrap (fastoverlap/utils.py:319-338)
+  const int ind[3] = {besti / (n * n), (besti / n) % n, besti % n};
+  double off[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    int ip[3] = {ind[0], ind[1], ind[2]}, im[3] = {ind[0], ind[1], ind[2]};
+    ip[ax] = (ind[ax] + 1) % n;
+    im[ax] = (ind[ax] + n - 1) % n;
+    const double y1 = fabs(grid_eval64(c, ip[0], ip[1], ip[2], red));
+    const double y3 = fabs(grid_eval64(c, im[0], im[1], im[2], red));
+    const double y2 = fabs(best);
+    off[ax] = (y3 - y1) / (2.0 * (2.0 * y2 - y1 - y3));
+  }
+  if (t == 0) {
+    SphPeak pk;
+    for (int ax = 0; ax < 3; ++ax) pk.ind[ax] = ind[ax];
+    pk.ncand = cand_overflow ? -1 : ncand;
+    pk.noverflow = cand_overflow ? n : novf;
+    pk.pad = 0;
+    pk.value = best;
+    pk.R0[0] = 2.0 * kPi / n * (ind[0] - off[0]);
+    pk.R0[1] = kPi / n * (ind[1] - off[1]) + 0.5 * kPi / n;
+    pk.R0[2] = 2.0 * kPi / n * (ind[2] - off[2]);
+    pk.margin = best - second;
+    pk.delta = delta;
+    peaks[(long long)p * norient + o] = pk;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Rotation refinement: minimise f(R) = -Re sum conj(I^l_{m1m2}) D^l_{m1m2}(R),
+// D = e^{-i m1 a} d^l_{m1m2}(b) e^{-i m2 y} with the unnormalised d
+// (fastoverlap/sphericalAlignment.py:67-113).  The reference runs scipy
+// L-BFGS-B from the grid angle; here a damped Newton iteration with the
+// exact gradient and Hessian (d, d', d'' by the differentiated recurrence)
+// converges to the same local optimum to full FP64 precision.
+// ---------------------------------------------------------------------------
+struct FGH {
+  double v[10];  // f, fa, fb, fy, faa, fab, fay, fbb, fby, fyy
+};
+
+FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a, double b,
+                     double y, FGH& out, double* red) {
+  const int W = 2 * L + 1;
+  const double cb = cos(b), sb = sin(b), c2 = cos(0.5 * b), s2 = sin(0.5 * b);
+  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int e = threadIdx.x; e < (L + 1) * W; e += blockDim.x) {
+    const int m1 = e / W, m2 = e % W - L;
+    const int l0 = max(m1, abs(m2));
+    const int mu = abs(m1 - m2), nu = abs(m1 + m2);
+    // seed g = s2^mu c2^nu and its b-derivatives
+    const double fct = sqrt((2.0 * l0 + 1.0) * 0.5) * seed_xi(m1, m2) * exp(seed_log_factor(mu, nu));
+    const double g0 = ipow(s2, mu) * ipow(c2, nu);
+    const double g1 = 0.5 * ((mu ? mu * ipow(s2, mu - 1) * ipow(c2, nu + 1) : 0.0) -
+                             (nu ? nu * ipow(s2, mu + 1) * ipow(c2, nu - 1) : 0.0));
+    auto dterm = [&](int pp, int qq) {  // d/db (s2^pp c2^qq)
+      return 0.5 * ((pp ? pp * ipow(s2, pp - 1) * ipow(c2, qq + 1) : 0.0) -
+                    (qq ? qq * ipow(s2, pp + 1) * ipow(c2, qq - 1) : 0.0));
+    };
+    const double g2 = 0.5 * ((mu ? mu * dterm(mu - 1, nu + 1) : 0.0) -
+                             (nu ? nu * dterm(mu + 1, nu - 1) : 0.0));
+    double c0x = 0, c0y = 0, c1x = 0, c1y = 0, c2x = 0, c2y = 0;
+    double d0 = fct * g0, d1 = fct * g1, dd2 = fct * g2;
+    double p0 = 0, p1 = 0, p2 = 0;
+    double sl = (l0 & 1) ? sgn : 1.0;
+    for (int l = l0; l <= L; ++l) {
+      const double un = sqrt(2.0 / (2.0 * l + 1.0)) * sl;
+      const double2 Iv = I[kidx(l, m1, m2)];
+      // conj(sgn^l I) * d
+      c0x += un * d0 * Iv.x;
+      c0y -= un * d0 * Iv.y;
+      c1x += un * d1 * Iv.x;
+      c1y -= un * d1 * Iv.y;
+      c2x += un * dd2 * Iv.x;
+      c2y -= un * dd2 * Iv.y;
+      sl *= sgn;
+      if (l == L) break;
+      double A, B, C;
+      recurrence_terms(l, m1, m2, A, B, C);
+      const double n0 = B * (cb - C) * d0 - A * p0;
+      const double n1 = B * (-sb * d0 + (cb - C) * d1) - A * p1;
+      const double n2 = B * (-cb * d0 - 2.0 * sb * d1 + (cb - C) * dd2) - A * p2;
+      p0 = d0;
+      p1 = d1;
+      p2 = dd2;
+      d0 = n0;
+      d1 = n1;
+      dd2 = n2;
+    }
+    double sp, cp;
+    sincos(-(m1 * a + m2 * y), &sp, &cp);
+    const double w = (m1 == 0) ? 1.0 : 2.0;
+    // Re(c E) and Re(-i c E) = Im(c E)
+    const double r0 = c0x * cp - c0y * sp, i0 = c0x * sp + c0y * cp;
+    const double r1 = c1x * cp - c1y * sp, i1 = c1x * sp + c1y * cp;
+    const double r2 = c2x * cp - c2y * sp;
+    const double M1 = m1, M2 = m2;
+    acc[0] -= w * r0;
+    acc[1] -= w * M1 * i0;          // d/da: Re(-i m1 c E) = m1 Im(cE)
+    acc[2] -= w * r1;
+    acc[3] -= w * M2 * i0;
+    acc[4] -= w * (-M1 * M1 * r0);
+    acc[5] -= w * M1 * i1;
+    acc[6] -= w * (-M1 * M2 * r0);
+    acc[7] -= w * r2;
+    acc[8] -= w * M2 * i1;
+    acc[9] -= w * (-M2 * M2 * r0);
+  }
+  for (int q = 0; q < 10; ++q) out.v[q] = block_sum(acc[q], red);
+}
+
+FO_DEV bool chol3_solve(const double H[6], double lam, const double g[3], double x[3]) {
+  // H = [haa hab hay; hab hbb hby; hay hby hyy] stored (aa,ab,ay,bb,by,yy)
+  double a00 = H[0] + lam, a10 = H[1], a20 = H[2], a11 = H[3] + lam, a21 = H[4], a22 = H[5] + lam;
+  if (!(a00 > 0)) return false;
+  const double l00 = sqrt(a00), l10 = a10 / l00, l20 = a20 / l00;
+  const double t11 = a11 - l10 * l10;
+  if (!(t11 > 0)) return false;
+  const double l11 = sqrt(t11), l21 = (a21 - l20 * l10) / l11;
+  const double t22 = a22 - l20 * l20 - l21 * l21;
+  if (!(t22 > 0)) return false;
+  const double l22 = sqrt(t22);
+  // solve L L^T x = -g
+  const double z0 = -g[0] / l00, z1 = (-g[1] - l10 * z0) / l11, z2 = (-g[2] - l20 * z0 - l21 * z1) / l22;
+  x[2] = z2 / l22;
+  x[1] = (z1 - l21 * x[2]) / l11;
+  x[0] = (z0 - l10 * x[1] - l20 * x[2]) / l00;
+  return true;
+}
+
+__global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ pd,
+                                                    const double2* __restrict__ IA,
+                                                    const double2* __restrict__ IB, int mode,
+                                                    const SphPeak* __restrict__ peaks, int norient,
+                                                    SphOpt* __restrict__ opt) {
+  __shared__ double red[16];
+  const int p = blockIdx.x, o = blockIdx.y;
+  const SphPair d = pd[p];
+  const int L = d.L;
+  const double sgn = (mode == 0 && o == 1) ? -1.0 : 1.0;
+  const double2* I = ((mode == 1 && o == 1) ? IB : IA) + d.coef_off;
+  const SphPeak pk = peaks[(long long)p * norient + o];
+  double x[3] = {pk.R0[0], pk.R0[1], pk.R0[2]};
+  FGH cur;
+  eval_fgh(I, sgn, L, x[0], x[1], x[2], cur, red);
+  int it = 0;
+  const double tol = 1e-12 * (fabs(cur.v[0]) + 1.0) * (L + 1);
+  for (; it < 100; ++it) {
+    const double g[3] = {cur.v[1], cur.v[2], cur.v[3]};
+    const double gn = fmax(fabs(g[0]), fmax(fabs(g[1]), fabs(g[2])));
+    if (gn < tol) break;
+    // Newton direction with Levenberg damping until positive definite
+    const double H[6] = {cur.v[4], cur.v[5], cur.v[6], cur.v[7], cur.v[8], cur.v[9]};
+    const double hs = fmax(fabs(H[0]), fmax(fabs(H[3]), fmax(fabs(H[5]), 1e-30)));
+    double dx[3];
+    double lam = 0.0;
+    bool ok = chol3_solve(H, lam, g, dx);
+    for (int k = 0; !ok && k < 12; ++k) {
+      lam = hs * pow(10.0, -8.0 + k);
+      ok = chol3_solve(H, lam, g, dx);
+    }
+    if (!ok) {
+      dx[0] = -g[0] / hs;
+      dx[1] = -g[1] / hs;
+      dx[2] = -g[2] / hs;
+    }
+    const double slope = g[0] * dx[0] + g[1] * dx[1] + g[2] * dx[2];
+    double step = 1.0;
+    FGH nxt;
+    bool acc = false;
+    for (int ls = 0; ls < 40; ++ls) {
+      eval_fgh(I, sgn, L, x[0] + step * dx[0], x[1] + step * dx[1], x[2] + step * dx[2], nxt, red);
+      if (nxt.v[0] <= cur.v[0] + 1e-4 * step * slope) {
+        acc = true;
+        break;
+      }
+      step *= 0.5;
+    }
+    if (!acc) break;
+    for (int q = 0; q < 3; ++q) x[q] += step * dx[q];
+    const double sn = step * fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
+    cur = nxt;
+    if (sn < 1e-15) {
+      ++it;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) {
+    SphOpt r;
+    r.R[0] = x[0];
+    r.R[1] = x[1];
+    r.R[2] = x[2];
+    r.f = cur.v[0];
+    r.gnorm = fmax(fabs(cur.v[1]), fmax(fabs(cur.v[2]), fabs(cur.v[3])));
+    r.iters = it;
+    r.pad = 0;
+    opt[(long long)p * norient + o] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Finish: orientation choice (inverted wins on a strictly smaller objective,
+// sphericalAlignment.py:184-187), X2 <- (+-X2) EulerM(R) (:124), per-group
+// LAP on squared distances with rows = X1 (utils.py:49-56, :82-167), and the
+// quaternion fit (utils.py:169-275): X2out = X2R[perm] MR^T (:126-127).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_sph_finish(
+    const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp, const int* __restrict__ gath,
+    const double* __restrict__ X1c, const double* __restrict__ X2c, const SphOpt* __restrict__ opt,
+    int norient, int lap_nmax, double* __restrict__ X2R, int* __restrict__ perm,
+    int* __restrict__ rowsol, double* __restrict__ X2out, SphResult* __restrict__ res) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[16];
+  __shared__ int redi[16];
+  const int p = blockIdx.x, t = threadIdx.x;
+  const SphPair d = pd[p];
+  const int N = d.natoms;
+  int inv = 0;
+  if (norient == 2 && opt[(long long)p * 2 + 1].f < opt[(long long)p * 2].f) inv = 1;
+  const SphOpt ob = opt[(long long)p * norient + inv];
+  double M[9];
+  euler_matrix(ob.R[0], ob.R[1], ob.R[2], M);
+  const double sg = inv ? -1.0 : 1.0;
+  const double* x1 = X1c + 3LL * d.atom_off;
+  const double* x2 = X2c + 3LL * d.atom_off;
+  double* xr = X2R + 3LL * d.atom_off;
+  int* pm = perm + d.atom_off;
+  for (int a = t; a < N; a += blockDim.x) {
+    const double u = sg * x2[3 * a], v = sg * x2[3 * a + 1], w = sg * x2[3 * a + 2];
+    // row vector times M
+    xr[3 * a] = u * M[0] + v * M[3] + w * M[6];
+    xr[3 * a + 1] = u * M[1] + v * M[4] + w * M[7];
+    xr[3 * a + 2] = u * M[2] + v * M[5] + w * M[8];
+    pm[a] = a;
+  }
+  __threadfence_block();
+  __syncthreads();
+  LapShared ls = lap_carve(smem, lap_nmax);
+  int* rs = rowsol + d.atom_off;
+  for (int q = 0; q < d.ngroups; ++q) {
+    const SphGroup G = grp[d.group_off + q];
+    const int* ga = gath + d.gath_off + G.gath_off;
+    auto cost = [&](int i, int j) {
+      const double* A = x1 + 3 * ga[i];
+      const double* B = xr + 3 * ga[j];
+      const double dx = A[0] - B[0], dy = A[1] - B[1], dz = A[2] - B[2];
+      return dx * dx + dy * dy + dz * dz;
+    };
+    lap_solve(G.size, cost, ls, rs, red, redi);
+    for (int i = t; i < G.size; i += blockDim.x) pm[ga[i]] = ga[rs[i]];
+    __threadfence_block();
+    __syncthreads();
+  }
+  // quaternion fit on all atoms
+  double c[6] = {0, 0, 0, 0, 0, 0};
+  for (int a = t; a < N; a += blockDim.x)
+    for (int k = 0; k < 3; ++k) {
+      c[k] += x1[3 * a + k];
+      c[3 + k] += xr[3 * pm[a] + k];
+    }
+  for (int k = 0; k < 6; ++k) c[k] = block_sum(c[k], red) / (double)N;
+  double q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int a = t; a < N; a += blockDim.x) {
+    const double* B = xr + 3 * pm[a];
+    qmat_accum(q, x1[3 * a] - c[0], x1[3 * a + 1] - c[1], x1[3 * a + 2] - c[2], B[0] - c[3],
+               B[1] - c[4], B[2] - c[5]);
+  }
+  for (int k = 0; k < 10; ++k) q[k] = block_sum(q[k], red);
+  __shared__ double sMR[9];
+  __shared__ double sdist;
+  if (t == 0) {
+    double MR[9], dist;
+    kearsley_solve(q, &dist, MR);
+    for (int k = 0; k < 9; ++k) sMR[k] = MR[k];
+    sdist = dist;
+  }
+  __syncthreads();
+  double* xo = X2out + 3LL * d.atom_off;
+  for (int a = t; a < N; a += blockDim.x) {
+    const double* B = xr + 3 * pm[a];
+    for (int i = 0; i < 3; ++i) xo[3 * a + i] = B[0] * sMR[3 * i] + B[1] * sMR[3 * i + 1] + B[2] * sMR[3 * i + 2];
+  }
+  if (t == 0) {
+    SphResult r;
+    r.dist = sdist;
+    // total map for row vectors: x -> sg * x M MR^T
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) s += M[3 * i + k] * sMR[3 * j + k];
+        r.rmat[3 * i + j] = sg * s;
+      }
+    r.R[0] = ob.R[0];
+    r.R[1] = ob.R[1];
+    r.R[2] = ob.R[2];
+    r.inverted = inv;
+    r.status = 0;
+    res[p] = r;
+  }
+}
+
+}  // namespace fo
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+namespace fo {
+
+static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+void launch_wigner_table(hipStream_t s, int n, int Lt, double* tab64, float* tab32) {
+  const long long total = (long long)n * (2 * Lt + 1) * (2 * Lt + 1);
+  const int blocks = (int)std::min<long long>(cdiv(total, 256), 65535);
+  hipLaunchKernelGGL(k_wigner_table, dim3(blocks), dim3(256), 0, s, n, Lt, tab64, tab32);
+}
+
+void launch_twiddles(hipStream_t s, int n, double sign, double2* tw64, float2* tw32) {
+  hipLaunchKernelGGL(k_twiddles, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sign, tw64, tw32);
+}
+
+void launch_sph_prep(hipStream_t s, int P, const SphPair* pd, const double* pos1,
+                     const double* pos2, double* X1c, double* X2c, double* scale_eff,
+                     int center) {
+  hipLaunchKernelGGL(k_sph_prep, dim3(P), dim3(256), 0, s, pd, pos1, pos2, X1c, X2c, scale_eff,
+                     center);
+}
+
+void launch_sph_ylm(hipStream_t s, int P, const SphPair* pd, const int* gath, const double* X1c,
+                    const double* X2c, double2* Y1, double2* Y2) {
+  hipLaunchKernelGGL(k_sph_ylm, dim3(P, 2), dim3(128), 0, s, pd, gath, X1c, X2c, Y1, Y2);
+}
+
+void launch_sph_radial(hipStream_t s, int P, long long maxjk, const SphPair* pd,
+                       const SphGroup* grp, const int* gath, const double* X1c,
+                       const double* X2c, const double* scale_eff, double* R) {
+  const int bx = std::max(1, std::min(cdiv(maxjk, 256), 1024));
+  hipLaunchKernelGGL(k_sph_radial, dim3(bx, P), dim3(256), 0, s, pd, grp, gath, X1c, X2c,
+                     scale_eff, R);
+}
+
+void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, const SphGroup* grp,
+                         const double2* Y1, const double2* Y2, const double* R,
+                         const double* scale_eff, double2* I64, float2* I32) {
+  int nb = 0;
+  for (int l = 0; l <= Lmax; ++l) nb += (2 * l + 1 + kMB - 1) / kMB;
+  const size_t smem = sizeof(double2) * (size_t)(kMB + 2 * Lmax + 1) * kKT;
+  hipLaunchKernelGGL(k_sph_contract, dim3(nb, P), dim3(256), smem, s, pd, grp, Y1, Y2, R,
+                     scale_eff, I64, I32);
+}
+
+size_t isoft_plane_smem(int n, int Lmax) {
+  return sizeof(float2) * ((size_t)n + 2 * (size_t)(Lmax + 1) * (2 * Lmax + 1) +
+                           2 * (size_t)(Lmax + 1) * n) +
+         16 * sizeof(float) + 16 * sizeof(int);
+}
+
+void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, long long Kt, const SphPair* pd,
+                        const float* tab32, const float2* tw32, const float2* IA,
+                        const float2* IB, int mode, int norient, PlaneCand* cand, float* grid) {
+  hipLaunchKernelGGL(k_isoft_plane, dim3(n, P), dim3(256), isoft_plane_smem(n, Lmax), s, pd, n,
+                     Kt, tab32, tw32, IA, IB, mode, norient, cand, grid);
+}
+
+void launch_sph_select(hipStream_t s, int P, int n, long long Kt, int norient, const SphPair* pd,
+                       const double* tab64, const double2* tw64, const double2* IA,
+                       const double2* IB, int mode, const PlaneCand* cand, double2* scratch,
+                       long long scratch_stride, SphPeak* peaks) {
+  hipLaunchKernelGGL(k_sph_select, dim3(P, norient), dim3(256), 0, s, pd, n, Kt, tab64, tw64, IA,
+                     IB, mode, norient, cand, scratch, scratch_stride, peaks);
+}
+
+void launch_sph_rotopt(hipStream_t s, int P, int norient, const SphPair* pd, const double2* IA,
+                       const double2* IB, int mode, const SphPeak* peaks, SphOpt* opt) {
+  hipLaunchKernelGGL(k_sph_rotopt, dim3(P, norient), dim3(256), 0, s, pd, IA, IB, mode, peaks,
+                     norient, opt);
+}
+
+void launch_sph_finish(hipStream_t s, int P, int lap_nmax, const SphPair* pd, const SphGroup* grp,
+                       const int* gath, const double* X1c, const double* X2c, const SphOpt* opt,
+                       int norient, double* X2R, int* perm, int* rowsol, double* X2out,
+                       SphResult* res) {
+  hipLaunchKernelGGL(k_sph_finish, dim3(P), dim3(256), lap_lds_bytes(lap_nmax), s, pd, grp, gath,
+                     X1c, X2c, opt, norient, lap_nmax, X2R, perm, rowsol, X2out, res);
+}
+
+}  // namespace fo
+
+// ---------------------------------------------------------------------------
+// Single-call helpers behind the drop-in class methods.
+// ---------------------------------------------------------------------------
+namespace fo {
+
+// FP64 -> FP32 copy of compact coefficients
+__global__ void k_c64_to_c32(long long n, const double2* __restrict__ a, float2* __restrict__ b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+// getEnergyGradient (fastoverlap/sphericalAlignment.py:98-100) for one
+// coefficient set given as conj(X) in compact layout.
+__global__ void __launch_bounds__(256) k_sph_energy(int L, const double2* __restrict__ Iconj,
+                                                    double a, double b, double y,
+                                                    double* __restrict__ out) {
+  __shared__ double red[16];
+  FGH r;
+  eval_fgh(Iconj, 1.0, L, a, b, y, r, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 10; ++q) out[q] = r.v[q];
+}
+
+// calcWignerMatrices(rot) (fastoverlap/sphericalAlignment.py:67-96):
+// D = e^{-i m1 a} d e^{-i m2 y} (unnormalised d) and its gradient, dense
+// (l, m1 mod W, m2 mod W) complex.
+__global__ void k_wigner_D(int L, double a, double b, double y, double2* __restrict__ D,
+                           double2* __restrict__ G) {
+  const int W = 2 * L + 1;
+  const double cb = cos(b), sb = sin(b), c2 = cos(0.5 * b), s2 = sin(0.5 * b);
+  const long long plane = (long long)(L + 1) * W * W;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < W * W; e += gridDim.x * blockDim.x) {
+    const int m1 = e / W - L, m2 = e % W - L;
+    const int l0 = max(abs(m1), abs(m2));
+    const int mu = abs(m1 - m2), nu = abs(m1 + m2);
+    const double fct = sqrt((2.0 * l0 + 1.0) * 0.5) * seed_xi(m1, m2) * exp(seed_log_factor(mu, nu));
+    double d0 = fct * ipow(s2, mu) * ipow(c2, nu);
+    double d1 = fct * 0.5 *
+                ((mu ? mu * ipow(s2, mu - 1) * ipow(c2, nu + 1) : 0.0) -
+                 (nu ? nu * ipow(s2, mu + 1) * ipow(c2, nu - 1) : 0.0));
+    double p0 = 0.0, p1 = 0.0;
+    double sa, ca, sy, cy;
+    sincos(-(double)m1 * a, &sa, &ca);
+    sincos(-(double)m2 * y, &sy, &cy);
+    const double2 ph = cmul(make_double2(ca, sa), make_double2(cy, sy));
+    for (int l = 0; l <= L; ++l) {
+      const long long id = (long long)l * W * W + (long long)((m1 + W) % W) * W + (m2 + W) % W;
+      if (l < l0) {
+        D[id] = make_double2(0.0, 0.0);
+        for (int k = 0; k < 3; ++k) G[k * plane + id] = make_double2(0.0, 0.0);
+        continue;
+      }
+      const double un = sqrt(2.0 / (2.0 * l + 1.0));
+      const double2 Dv = cscale(ph, un * d0);
+      D[id] = Dv;
+      G[id] = make_double2(m1 * Dv.y, -m1 * Dv.x);                // -i m1 D
+      G[plane + id] = cscale(ph, un * d1);
+      G[2 * plane + id] = make_double2(m2 * Dv.y, -m2 * Dv.x);    // -i m2 D
+      if (l == L) break;
+      double A, B, C;
+      recurrence_terms(l, m1, m2, A, B, C);
+      const double n0 = B * (cb - C) * d0 - A * p0;
+      const double n1 = B * (-sb * d0 + (cb - C) * d1) - A * p1;
+      p0 = d0;
+      p1 = d1;
+      d0 = n0;
+      d1 = n1;
+    }
+  }
+}
+
+// sphHarm(theta, phi) (fastoverlap/sphericalAlignment.py:57-65):
+// Y[(l*W + m mod W)*npts + j].
+__global__ void k_ylm_angles(int L, int npts, const double* __restrict__ theta,
+                             const double* __restrict__ phi, double2* __restrict__ Y) {
+  const int W = 2 * L + 1;
+  const double inv4pi = 1.0 / (4.0 * kPi);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < npts; j += gridDim.x * blockDim.x) {
+    double st, ct, sp, cp;
+    sincos(theta[j], &st, &ct);
+    sincos(phi[j], &sp, &cp);
+    for (int l = 0; l <= L; ++l)
+      for (int m = 0; m < W; ++m) Y[((long long)l * W + m) * npts + j] = make_double2(0.0, 0.0);
+    double pmm = sqrt(inv4pi);
+    double2 em = make_double2(1.0, 0.0);
+    for (int m = 0; m <= L; ++m) {
+      if (m > 0) {
+        pmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m)) * st;
+        em = cmul(em, make_double2(cp, sp));
+      }
+      double p2 = 0.0, p1 = pmm;
+      for (int l = m; l <= L; ++l) {
+        double pl;
+        if (l == m) {
+          pl = pmm;
+        } else if (l == m + 1) {
+          pl = sqrt(2.0 * m + 3.0) * ct * pmm;
+        } else {
+          const double aa = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+          const double bb = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) /
+                                 (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+          pl = aa * (ct * p1 - bb * p2);
+        }
+        if (l > m) {
+          p2 = p1;
+          p1 = pl;
+        }
+        const double2 yp = cscale(em, pl);
+        Y[((long long)l * W + m) * npts + j] = yp;
+        if (m > 0) {
+          const double sg = (m & 1) ? -1.0 : 1.0;
+          Y[((long long)l * W + (W - m)) * npts + j] = make_double2(sg * yp.x, -sg * yp.y);
+        }
+      }
+    }
+  }
+}
+
+void launch_c64_to_c32(hipStream_t s, long long n, const double2* a, float2* b) {
+  const int blocks = (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_c64_to_c32, dim3(blocks), dim3(256), 0, s, n, a, b);
+}
+void launch_sph_energy(hipStream_t s, int L, const double2* Iconj, double a, double b, double y,
+                       double* out) {
+  hipLaunchKernelGGL(k_sph_energy, dim3(1), dim3(256), 0, s, L, Iconj, a, b, y, out);
+}
+void launch_wigner_D(hipStream_t s, int L, double a, double b, double y, double2* D, double2* G) {
+  const int W = 2 * L + 1;
+  hipLaunchKernelGGL(k_wigner_D, dim3((W * W + 127) / 128), dim3(128), 0, s, L, a, b, y, D, G);
+}
+void launch_ylm_angles(hipStream_t s, int L, int npts, const double* th, const double* ph,
+                       double2* Y) {
+  hipLaunchKernelGGL(k_ylm_angles, dim3((npts + 127) / 128), dim3(128), 0, s, L, npts, th, ph, Y);
+}
+
+}  // namespace fo

@@ -1,0 +1,261 @@
+/*
+ * fastoverlap_hip.h -- C ABI of libfastoverlap_hip.so, the MI355X (gfx950)
+ * implementation of fastoverlap's overlap-maximisation hot path.
+ *
+ * The entry points replace the numerical core behind the reference's
+ * drop-in classes and Fortran modules:
+ *
+ *   fastoverlap/sphericalAlignment.py:160-188, :237-273  SphericalAlign
+ *   fastoverlap/soft.py:115-130                          SOFT.iSOFT
+ *   fastoverlap/periodicAlignment.py:27-110, :362-460    PeriodicAlign
+ *   f90/fastclusters.f90:129-269                         CLUSTERFASTOVERLAP::ALIGN
+ *   f90/fastbulk.f90:279-334                             BULKFASTOVERLAP::ALIGN
+ *   f90/fastutils.f90:117-167                            SETPERM (group layout)
+ *
+ * Conventions
+ *  - Every function returns FO_OK (0) or a positive FO_ERR_* code and never
+ *    terminates the caller; fo_last_error() returns a thread-local message.
+ *  - Positions are row-major (natoms, 3) FP64 -- identical to Fortran's flat
+ *    3*NATOMS arrays.  Several pairs are concatenated atom-wise.
+ *  - Permutation groups ("species") use the SETPERM layout but 0-based:
+ *    ngroups[p] groups per pair, group_sizes concatenated over pairs,
+ *    group_atoms concatenated (atom indices local to the pair).
+ *  - perm[i] (output) = atom of structure 2 matched to atom i of structure 1,
+ *    so X2[perm] aligns with X1 (0-based).
+ *  - C^l coefficients use the compact l-major order: for l = 0..L,
+ *    m1 = -l..l, m2 = -l..l, complex FP64 interleaved (re, im).
+ *  - Host buffers are copied in and out; nothing is retained after a
+ *    synchronous call returns.  Distinct contexts may be used from
+ *    different threads; one context serialises its own calls.
+ */
+#ifndef FASTOVERLAP_HIP_H
+#define FASTOVERLAP_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FO_OK 0
+#define FO_ERR_INVALID 1      /* bad argument / shape */
+#define FO_ERR_HIP 2          /* HIP runtime error */
+#define FO_ERR_NOMEM 3        /* exceeds device memory or the context budget */
+#define FO_ERR_UNSUPPORTED 4  /* size outside the supported range */
+#define FO_ERR_STATE 5        /* call sequence error (e.g. missing stage) */
+#define FO_ERR_NOTFOUND 6     /* unknown request id */
+#define FO_ERR_CANCELLED 7    /* request cancelled / server stopped */
+
+typedef struct fo_context fo_context;
+typedef struct fo_sph_batch fo_sph_batch;
+typedef struct fo_per_batch fo_per_batch;
+typedef struct fo_server fo_server;
+
+/* ---- library / context ------------------------------------------------ */
+int fo_version(void);
+const char* fo_last_error(void);
+int fo_device_count(int* count);
+/* hbm_budget: bytes this context may allocate (0 = no limit). */
+int fo_context_create(int device, unsigned long long hbm_budget, fo_context** out);
+int fo_context_destroy(fo_context* ctx);
+int fo_context_memory(fo_context* ctx, unsigned long long* used, unsigned long long* budget);
+
+/* ---- spherical batches --------------------------------------------------
+ * A batch holds npairs cluster pairs sharing one SO(3) grid side grid_n
+ * (= 2B; the reference uses 2*(Jmax+1), fastoverlap/sphericalAlignment.py:50).
+ * Per pair: natoms, Jmax L (grid_n >= 2(L+1), L <= 63) and kernel width
+ * scale (<= 0: mean nearest-neighbour distance / 3, as the reference's
+ * calcScale path, :145-147).
+ */
+#define FO_SPH_NO_INVERT 1     /* only the given orientation (invert=False) */
+#define FO_SPH_KEEP_GRID 2     /* keep the FP32 overlap grid(s) */
+#define FO_SPH_NO_CENTER 4     /* positions are used as given */
+#define FO_SPH_GENERAL 8       /* two independent Hermitian coefficient sets */
+
+/* stages for fo_sph_batch_run (bit mask) */
+#define FO_STAGE_PREP 1        /* centroids, kernel width */
+#define FO_STAGE_COEFF 2       /* Y_lm, radial kernel, C^l (FP64) */
+#define FO_STAGE_GRID 4        /* inverse SOFT (FP32) + exact FP64 argmax */
+#define FO_STAGE_ROTOPT 8      /* FP64 rotation refinement from the grid peak */
+#define FO_STAGE_FINISH 16     /* orientation choice, LAP, quaternion fit */
+#define FO_STAGE_ALL 31
+
+typedef struct {
+  int npairs;
+  int grid_n;
+  const int* natoms;      /* [npairs] */
+  const int* L;           /* [npairs] */
+  const double* scale;    /* [npairs] */
+  const int* ngroups;     /* [npairs] */
+  const int* group_sizes; /* [sum ngroups] */
+  const int* group_atoms; /* [sum group_sizes] */
+  int flags;
+} fo_sph_batch_desc;
+
+int fo_sph_batch_create(fo_context* ctx, const fo_sph_batch_desc* desc, fo_sph_batch** out);
+int fo_sph_batch_destroy(fo_sph_batch* b);
+/* bytes of device memory the batch holds */
+int fo_sph_batch_footprint(const fo_sph_batch* b, unsigned long long* bytes);
+/* positions: host arrays of 3*sum(natoms) doubles */
+int fo_sph_batch_set_positions(fo_sph_batch* b, const double* pos1, const double* pos2);
+/* positions already in device memory (device-to-device copy, async on the batch stream) */
+int fo_sph_batch_set_positions_device(fo_sph_batch* b, const void* dpos1, const void* dpos2);
+/* compact FP64 coefficients, set A (and B in FO_SPH_GENERAL mode; may be NULL) */
+int fo_sph_batch_set_coeffs(fo_sph_batch* b, const double* coefA, const double* coefB);
+/* grid peaks / start angles for FO_STAGE_ROTOPT without FO_STAGE_GRID:
+ * R0[npairs*norient*3] */
+int fo_sph_batch_set_start(fo_sph_batch* b, const double* R0);
+/* chosen rotation for FO_STAGE_FINISH alone: R[npairs*3]; the objective of
+ * orientation 0 is set to 0 and, with inversion, orientation 1 to +inf */
+int fo_sph_batch_set_rotation(fo_sph_batch* b, const double* R);
+/* enqueue the stages on the batch stream; use_graph=1 captures/replays a
+ * hipGraph of the launches */
+int fo_sph_batch_run(fo_sph_batch* b, int stages, int use_graph);
+int fo_sph_batch_sync(fo_sph_batch* b);
+/* outputs (any pointer may be NULL):
+ *  dist[npairs], X1[3N], X2[3N], perm[N], R[npairs*3], rmat[npairs*9],
+ *  inverted[npairs] */
+int fo_sph_batch_results(fo_sph_batch* b, double* dist, double* X1, double* X2, int* perm,
+                         double* R, double* rmat, int* inverted);
+/* per (pair, orientation): ind[3], fun, grid peak value, margin, error bound,
+ * FP64 candidates, fully re-evaluated planes */
+int fo_sph_batch_peaks(fo_sph_batch* b, int* ind, double* fun, double* peak_value,
+                       double* margin, double* delta, int* ncand, int* noverflow);
+/* rotation refinement per (pair, orientation): R[3], objective, final
+ * gradient max-norm, Newton iterations */
+int fo_sph_batch_opt(fo_sph_batch* b, double* R, double* fun, double* gnorm, int* iters);
+/* kernel width used per pair (resolved on the device when <= 0 was given) */
+int fo_sph_batch_scales(fo_sph_batch* b, double* scale);
+int fo_sph_batch_coeffs(fo_sph_batch* b, double* coef);
+/* FP32 grid(s) of one pair: norient * n^3 floats, axes (alpha, beta, gamma) */
+int fo_sph_batch_grid(fo_sph_batch* b, int pair, float* grid);
+int fo_sph_batch_norient(const fo_sph_batch* b);
+/* kernel times (ms) of the last run: [prep, ylm, radial, contract, plane,
+ * select, rotopt, finish]; requires timing enabled */
+int fo_sph_batch_timing(fo_sph_batch* b, int enable, float* ms8);
+
+/* ---- single-call spherical helpers (drop-in SphericalAlign) ------------ */
+/* sphHarm: Y[l, m mod (2L+1), j], complex interleaved, (L+1)*(2L+1)*npts */
+int fo_sph_harm(fo_context* ctx, int L, int npts, const double* theta, const double* phi,
+                double* Y);
+/* getEnergyGradient: f = -Re sum X * D(rot), grad[3]; X dense complex
+ * (L+1, 2L+1, 2L+1) with m wrapped mod 2L+1 */
+int fo_sph_energy(fo_context* ctx, int L, const double* Xdense, const double* rot, double* f,
+                  double* grad);
+/* calcWignerMatrices(rot): D and grad[3] dense (L+1, 2L+1, 2L+1) complex */
+int fo_sph_wigner_D(fo_context* ctx, int L, const double* rot, double* D, double* grad);
+/* Wigner table of SOFT(bw) in the dense layout Ds[l, m1, m2, k] (FP64) */
+int fo_sph_wigner_table(fo_context* ctx, int bw, double* Ds);
+
+/* ---- linear assignment -------------------------------------------------
+ * metric 0: squared Euclidean; 1: Euclidean with minimum image in box[3].
+ * perm[i] = row i's column (per group); dist = |X1 - X2[perm]| (periodic for
+ * metric 1) */
+int fo_lap_align(fo_context* ctx, int natoms, const double* X1, const double* X2, int ngroups,
+                 const int* group_sizes, const int* group_atoms, int metric, const double* box,
+                 int* perm, double* dist);
+
+/* ---- periodic batches ---------------------------------------------------
+ * All pairs share the k-cube half width nk (2nk+1 vectors per axis) and the
+ * FFT side F; per pair natoms, box[3] and scale (> 0).
+ */
+#define FO_PER_KEEP_GRID 2      /* keep |f| (FP32) */
+#define FO_PER_INPUT_COEFFS 4   /* coefficients supplied (quickAlign path) */
+#define FO_PER_KEEP_COEFFS 8    /* keep per-group structure factors */
+
+#define FO_PSTAGE_SF 1          /* structure factors + weighted product */
+#define FO_PSTAGE_GRID 2        /* 3D DFT planes + exact FP64 argmax */
+#define FO_PSTAGE_REFINE 4      /* LAP + mean-shift iteration */
+#define FO_PSTAGE_ALL 7
+
+typedef struct {
+  int npairs;
+  int nk;
+  int F;
+  const int* natoms;
+  const double* box;      /* [npairs*3] */
+  const double* scale;    /* [npairs] */
+  const int* ngroups;
+  const int* group_sizes;
+  const int* group_atoms;
+  int niter;              /* refinement iterations (reference default 10) */
+  int flags;
+} fo_per_batch_desc;
+
+int fo_per_batch_create(fo_context* ctx, const fo_per_batch_desc* desc, fo_per_batch** out);
+int fo_per_batch_destroy(fo_per_batch* b);
+int fo_per_batch_footprint(const fo_per_batch* b, unsigned long long* bytes);
+int fo_per_batch_set_positions(fo_per_batch* b, const double* pos1, const double* pos2);
+int fo_per_batch_set_positions_device(fo_per_batch* b, const void* dpos1, const void* dpos2);
+/* per-group coefficients (sum ngroups, 2nk+1, 2nk+1, 2nk+1) complex, for
+ * FO_PER_INPUT_COEFFS */
+int fo_per_batch_set_coeffs(fo_per_batch* b, const double* C1, const double* C2);
+/* initial displacement for FO_PSTAGE_REFINE alone: disp[npairs*3] */
+int fo_per_batch_set_disp(fo_per_batch* b, const double* disp);
+int fo_per_batch_run(fo_per_batch* b, int stages, int use_graph);
+int fo_per_batch_sync(fo_per_batch* b);
+int fo_per_batch_results(fo_per_batch* b, double* dist, double* X1, double* X2, int* perm,
+                         double* disp);
+int fo_per_batch_peaks(fo_per_batch* b, int* ind, double* disp0, double* peak_value,
+                       double* margin, double* delta, int* ncand, int* noverflow);
+int fo_per_batch_coeffs(fo_per_batch* b, double* C1, double* C2);
+int fo_per_batch_product(fo_per_batch* b, double* C);
+int fo_per_batch_grid(fo_per_batch* b, int pair, float* fabs_grid);
+int fo_per_batch_timing(fo_per_batch* b, int enable, float* ms4);
+
+/* ---- serving --------------------------------------------------------------
+ * A continuous-batching server: requests are queued by shape class, admitted
+ * while their device footprint fits the budget, batched and run on several
+ * streams so that batches overlap on the device.
+ */
+typedef struct {
+  int kind;               /* 0 spherical, 1 periodic */
+  int natoms;
+  const double* pos1;
+  const double* pos2;
+  int ngroups;
+  const int* group_sizes;
+  const int* group_atoms;
+  /* spherical */
+  int L;
+  int grid_n;             /* 0: next power of two >= 2(L+1) */
+  int invert;
+  /* periodic */
+  double box[3];
+  int nk;
+  int F;
+  double scale;           /* <= 0: reference default */
+} fo_request;
+
+typedef struct {
+  int status;
+  double dist;
+  double R[3];            /* spherical Euler angles */
+  double rmat[9];
+  int inverted;
+  double disp[3];         /* periodic displacement */
+  int ind[6];             /* argmax per orientation (spherical) or [0:3] */
+  double latency_ms;      /* submit -> result ready */
+  double queue_ms;        /* submit -> admitted to a batch */
+  int batch_size;         /* pairs in the batch that served it */
+} fo_result;
+
+typedef struct {
+  int max_batch;          /* pairs per batch */
+  int streams;            /* concurrent batches */
+  double max_wait_ms;     /* how long a partial batch waits for more work */
+  unsigned long long hbm_budget; /* admission limit (bytes) */
+} fo_server_config;
+
+int fo_server_create(fo_context* ctx, const fo_server_config* cfg, fo_server** out);
+int fo_server_destroy(fo_server* s);
+/* copies the request; returns an id */
+int fo_server_submit(fo_server* s, const fo_request* req, long long* id);
+/* wait (timeout_ms < 0: forever) for a result; X2/perm may be NULL */
+int fo_server_wait(fo_server* s, long long id, double timeout_ms, fo_result* res, double* X2,
+                   int* perm);
+int fo_server_stats(fo_server* s, long long* completed, long long* batches,
+                    unsigned long long* peak_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTOVERLAP_HIP_H */

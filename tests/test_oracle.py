@@ -1,0 +1,314 @@
+"""Pinning of the FP64 oracle (CPU).  Each check is chosen so that a plausible
+mistake -- a dropped term, a wrong sign or index, a transposed operand --
+fails it: known answers from the reference's own example data, closed forms,
+library routines, invariants and brute force on tiny inputs."""
+
+import itertools
+import math
+
+import numpy as np
+import pytest
+from scipy.fft import next_fast_len as scipy_next_fast_len
+from scipy.special import spherical_in
+
+from oracle import fo_oracle as O
+
+
+# ---- Wigner-d -----------------------------------------------------------------
+
+def test_wigner_small_l_closed_forms():
+    b = np.array([0.3, 1.1, 2.7])
+    d = O.wigner_d_jacobi(1, b, normalised=False)
+    n = 3
+    c, s = np.cos(b), np.sin(b)
+    # textbook d^1 (z-y-z convention): d^1_{11} = (1+c)/2, d^1_{10} = -s/sqrt2,
+    # d^1_{1,-1} = (1-c)/2, d^1_{00} = c, d^1_{01} = s/sqrt2
+    np.testing.assert_allclose(d[1, 1, 1], (1 + c) / 2, atol=1e-15)
+    np.testing.assert_allclose(d[1, 1, 0], -s / math.sqrt(2), atol=1e-15)
+    np.testing.assert_allclose(d[1, 1, -1 % n], (1 - c) / 2, atol=1e-15)
+    np.testing.assert_allclose(d[1, 0, 0], c, atol=1e-15)
+    np.testing.assert_allclose(d[1, 0, 1], s / math.sqrt(2), atol=1e-15)
+    np.testing.assert_allclose(d[0, 0, 0], 1.0, atol=1e-15)
+
+
+def test_wigner_unitary():
+    L = 12
+    d = O.wigner_d_jacobi(L, np.array([0.83]), normalised=False)[..., 0]
+    n = 2 * L + 1
+    for l in (1, 5, 12):
+        idx = [m % n for m in range(-l, l + 1)]
+        M = d[l][np.ix_(idx, idx)]
+        np.testing.assert_allclose(M @ M.T, np.eye(2 * l + 1), atol=1e-12)
+
+
+def test_wigner_quadrature_orthogonality():
+    # SOFT quadrature: sum_k w_k dt^l dt^l' (b_k) = delta_{ll'} for |m| <= l
+    bw = 10
+    soft = O.SOFTOracle(bw)
+    Ds, w = soft.Ds, soft.weights
+    n = 2 * bw - 1
+    for m1, m2 in [(0, 0), (2, -1), (3, 3), (-4, 1)]:
+        l0 = max(abs(m1), abs(m2))
+        G = np.array([[np.sum(w * Ds[l, m1 % n, m2 % n] * Ds[lp, m1 % n, m2 % n])
+                       for lp in range(l0, bw)] for l in range(l0, bw)])
+        np.testing.assert_allclose(G, np.eye(bw - l0), atol=1e-12)
+
+
+def test_wigner_recurrence_matches_jacobi():
+    bw = 31
+    b = O.soft_grid(bw)[1]
+    np.testing.assert_allclose(O.wigner_d_recurrence(30, b), O.wigner_d_jacobi(30, b),
+                               atol=5e-12)
+
+
+# ---- SOFT ---------------------------------------------------------------------------
+
+def _random_flmm(rng, bw):
+    f = np.zeros((bw, 2 * bw - 1, 2 * bw - 1), complex)
+    n = 2 * bw - 1
+    for l in range(bw):
+        for m1 in range(-l, l + 1):
+            for m2 in range(-l, l + 1):
+                f[l, m1 % n, m2 % n] = rng.normal() + 1j * rng.normal()
+    return f
+
+
+def test_soft_round_trip():
+    rng = np.random.default_rng(0)
+    soft = O.SOFTOracle(6)
+    f = _random_flmm(rng, 6)
+    np.testing.assert_allclose(soft.SOFT(soft.iSOFT(f)), f, atol=1e-11)
+
+
+def test_isoft_delta_coefficient():
+    bw = 5
+    soft = O.SOFTOracle(bw)
+    f = np.zeros((bw, 2 * bw - 1, 2 * bw - 1), complex)
+    l, m1, m2 = 3, 2, -1
+    f[l, m1, m2] = 1.0
+    g = soft.iSOFT(f)
+    a, b, y = soft.a, soft.b, soft.y
+    expect = (np.exp(1j * m1 * a)[:, None, None] * soft.Ds[l, m1, m2][None, :, None]
+              * np.exp(1j * m2 * y)[None, None, :])
+    np.testing.assert_allclose(g, expect, atol=1e-13)
+
+
+# ---- coefficients -----------------------------------------------------------------
+
+def test_sph_harm_basics():
+    th = np.array([0.4, 1.3, 2.9])
+    ph = np.array([0.1, -2.0, 3.0])
+    Y = O.sph_harm_table(3, th, ph)
+    np.testing.assert_allclose(Y[0, 0], 1 / math.sqrt(4 * math.pi))
+    # Y_1^0 = sqrt(3/4pi) cos(theta); Y_1^{-1} = (-1)^1 conj(Y_1^1)
+    np.testing.assert_allclose(Y[1, 0], math.sqrt(3 / (4 * math.pi)) * np.cos(th))
+    np.testing.assert_allclose(Y[1, -1 % 7], -np.conj(Y[1, 1]))
+    # Y_1^1 = -sqrt(3/8pi) sin(theta) e^{i phi} (Condon-Shortley)
+    np.testing.assert_allclose(Y[1, 1], -math.sqrt(3 / (8 * math.pi)) * np.sin(th)
+                               * np.exp(1j * ph))
+
+
+def test_sph_harm_orthonormal_by_quadrature():
+    L = 6
+    xg, wg = np.polynomial.legendre.leggauss(2 * L + 2)
+    th = np.arccos(xg)
+    nphi = 4 * L + 4
+    ph = 2 * np.pi * np.arange(nphi) / nphi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    W = np.repeat(wg, nphi) * (2 * np.pi / nphi)
+    Y = O.sph_harm_table(L, T.ravel(), P.ravel()).reshape((L + 1) * (2 * L + 1), -1)
+    G = (Y * W) @ Y.conj().T
+    keep = np.abs(np.diag(G)) > 0.5
+    np.testing.assert_allclose(G[np.ix_(keep, keep)], np.eye(keep.sum()), atol=1e-12)
+    assert keep.sum() == (L + 1) ** 2
+
+
+def test_radial_kernel_matches_spherical_in():
+    r1 = np.array([0.0, 0.3, 1.2, 2.0])
+    r2 = np.array([0.5, 1.0, 1.7, 2.2])
+    s = 0.4
+    il = O.radial_kernel(10, r1, r2, s)
+    x = r1[:, None] * r2[None] / s ** 2 / 2
+    for l in range(11):
+        expect = spherical_in(l, x) * np.exp(-(r1[:, None] ** 2 + r2[None] ** 2) / 4 / s ** 2)
+        np.testing.assert_allclose(il[l], expect, rtol=1e-12, atol=1e-300)
+    # x = 0 limit: i_0(0) = 1, i_l(0) = 0
+    assert il[0, 0, 0] == pytest.approx(np.exp(-r2[0] ** 2 / 4 / s ** 2))
+    assert np.all(il[1:, 0, :] == 0)
+
+
+def _cloud(rng, n, R=1.4):
+    X = rng.normal(size=(n, 3))
+    X *= R / np.linalg.norm(X, axis=1).max()
+    return X - X.mean(0)
+
+
+def test_coeffs_hermitian_and_inversion():
+    rng = np.random.default_rng(3)
+    X1, X2 = _cloud(rng, 9), _cloud(rng, 9)
+    L = 6
+    n = 2 * L + 1
+    I = O.so3_coeffs(X1, X2, L, 0.35)
+    for l in range(L + 1):
+        for m1 in range(-l, l + 1):
+            for m2 in range(-l, l + 1):
+                assert I[l, -m1 % n, -m2 % n] == pytest.approx(
+                    (-1) ** (m1 + m2) * np.conj(I[l, m1 % n, m2 % n]), abs=1e-13)
+    Iinv = O.so3_coeffs(X1, -X2, L, 0.35)
+    np.testing.assert_allclose(Iinv, ((-1.0) ** np.arange(L + 1))[:, None, None] * I,
+                               atol=1e-13)
+    np.testing.assert_allclose(O.so3_coeffs(X1, X2, L, 0.35, fast=True), I, atol=1e-13)
+
+
+def test_coeffs_brute_force_overlap_identity():
+    # Re sum conj(I) D(R) equals the direct Gaussian overlap of X1 with X2
+    # rotated as refine() rotates it (X2 . EulerM(R)), up to truncation in l.
+    rng = np.random.default_rng(7)
+    X1, X2 = _cloud(rng, 6, 1.0), _cloud(rng, 6, 1.0)
+    s = 0.5
+    L = 22
+    I = O.so3_coeffs(X1, X2, L, s)
+    for R in [np.array([0.3, 1.2, -0.7]), np.array([2.0, 0.4, 1.0])]:
+        f, _ = O.energy_gradient(R, I.conj(), L)
+        brute = O.brute_overlap(X1, X2.dot(O.euler_matrix(*R)), s)
+        assert -f == pytest.approx(brute, rel=1e-9)
+
+
+# ---- peak search / refinement ------------------------------------------------------
+
+def test_find_max_parabola_exact():
+    n = 12
+    i = np.arange(n)
+    c = np.array([4.3, 7.8, 2.2])
+    g = -sum(((np.meshgrid(i, i, i, indexing="ij")[k] - c[k]) ** 2) for k in range(3)) + 100
+    ind, f = O.find_max(g)
+    np.testing.assert_array_equal(ind, [4, 8, 2])
+    np.testing.assert_allclose(f, c, atol=1e-12)
+
+
+def test_find_max_complex_lexicographic_first_occurrence():
+    a = np.zeros((2, 2, 2), complex)
+    a[0, 1, 1] = 3 + 0j
+    a[1, 0, 0] = 3 + 1e-9j
+    a[1, 1, 0] = 3 + 1e-9j
+    ind, _ = O.find_max(a)
+    np.testing.assert_array_equal(ind, [1, 0, 0])
+
+
+def test_energy_gradient_finite_difference_and_recurrence():
+    rng = np.random.default_rng(1)
+    L = 8
+    X = rng.normal(size=(L + 1, 2 * L + 1, 2 * L + 1)) + 1j * rng.normal(
+        size=(L + 1, 2 * L + 1, 2 * L + 1))
+    rot = np.array([0.7, 1.1, 2.3])
+    f, g = O.energy_gradient(rot, X, L)
+    f2, g2 = O.energy_gradient_recurrence(rot, X, L)
+    h = 1e-6
+    fd = [(O.energy_gradient(rot + h * e, X, L)[0] - O.energy_gradient(rot - h * e, X, L)[0])
+          / (2 * h) for e in np.eye(3)]
+    assert f == pytest.approx(f2, rel=1e-12)
+    np.testing.assert_allclose(g, g2, rtol=1e-11)
+    np.testing.assert_allclose(g, fd, rtol=1e-7)
+
+
+def test_lap_brute_force():
+    rng = np.random.default_rng(5)
+    for n in (3, 5, 7):
+        X1, X2 = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+        _, perm = O.find_best_permutation(X1, X2, [np.arange(n)])
+        cost = ((X1[:, None] - X2[None]) ** 2).sum(-1)
+        best = min(itertools.permutations(range(n)),
+                   key=lambda p: cost[np.arange(n), list(p)].sum())
+        np.testing.assert_array_equal(perm, best)
+
+
+def test_kabsch_matches_svd_and_exact_rotation():
+    rng = np.random.default_rng(9)
+    X = rng.normal(size=(20, 3))
+    X -= X.mean(0)
+    Rm = O.euler_matrix(0.4, 1.9, -1.2)
+    dist, MR = O.kabsch_quaternion(X, X.dot(Rm.T))
+    assert dist < 1e-6
+    Y = rng.normal(size=(20, 3))
+    Y -= Y.mean(0)
+    dist, MR = O.kabsch_quaternion(X, Y)
+    U, S, Vt = np.linalg.svd(Y.T @ X)
+    dsign = np.sign(np.linalg.det(U @ Vt))
+    Rsvd = U @ np.diag([1, 1, dsign]) @ Vt
+    assert dist == pytest.approx(np.linalg.norm(X - Y @ Rsvd), rel=1e-10)
+    np.testing.assert_allclose(np.linalg.norm(X - Y.dot(MR.T)), dist, rtol=1e-10)
+
+
+# ---- end to end: the reference's known answers ------------------------------------
+
+def test_lj38_known_answer(lj38):
+    # fastoverlap/sphericalAlignment.py:680-693: "distance should = 1.4767"
+    p1, p2 = lj38
+    r = O.spherical_align(p1, p2, 14, 0.3, fast=True)
+    # the reference prints the answer to 4 decimals; its L-BFGS-B end point
+    # only enters through the permutation, so agreement to 1e-4 pins it
+    assert r.dist == pytest.approx(1.4767, abs=1e-4)
+    assert np.linalg.norm(r.X1 - r.X2) == pytest.approx(r.dist, rel=1e-10)
+
+
+def test_random_cloud_exact_rotation_gives_zero():
+    # fastoverlap/sphericalAlignment.py:711-723: "distance should ~ 0"
+    rng = np.random.default_rng(11)
+    X = rng.normal(size=(30, 3)) * 1.2
+    X -= X.mean(0)
+    Rm = O.euler_matrix(1.0, 0.7, 2.5)
+    Y = X.dot(Rm.T)[rng.permutation(30)]
+    r = O.spherical_align(X, Y, 12, 0.3, fast=True)
+    assert r.dist < 1e-6
+
+
+def test_blj256_known_answer(blj256):
+    # fastoverlap/periodicAlignment.py:609-633: "distance should = 1.559"
+    p1, p2 = blj256
+    P = O.PeriodicOracle(256, np.ones(3) * 5.975206329, [np.arange(204), np.arange(204, 256)])
+    (dist, X1, X2, perm, disp), ind, _ = P.align(p1, p2)
+    assert dist == pytest.approx(1.559, abs=1e-3)
+    assert P.fshape == (40, 40, 40) and P.n == 9
+
+
+def test_periodic_grid_peak_and_corner_padding():
+    rng = np.random.default_rng(2)
+    box = np.full(3, 5.0)
+    X = rng.random((20, 3)) * box
+    shift = np.array([1.0, 2.5, 3.75])       # multiples of box/F with F = 20
+    P = O.PeriodicOracle(20, box, n=4, fshape=(20, 20, 20))
+    C, fabs = P.overlap_grid(X, X + shift)
+    ind, find = O.find_max(fabs)
+    np.testing.assert_array_equal(ind, np.round(shift / box * 20).astype(int))
+    # |fftn| of the corner-padded product equals a direct DFT
+    x = np.array([3, 7, 11])
+    k = np.arange(9)
+    ph = np.exp(-2j * np.pi * np.add.outer(np.add.outer(k * x[0], k * x[1]), k * x[2]) / 20)
+    assert abs((C * ph).sum()) == pytest.approx(fabs[tuple(x)], rel=1e-12)
+
+
+def test_periodic_exact_copy_aligns_to_zero():
+    rng = np.random.default_rng(4)
+    box = np.full(3, 6.0)
+    X = rng.random((40, 3)) * box
+    perm = rng.permutation(40)
+    Y = (X + np.array([1.1, -2.2, 0.7]))[perm]
+    P = O.PeriodicOracle(40, box)
+    (dist, X1, X2, p, disp), _, _ = P.align(X, Y)
+    assert dist < 1e-9
+    np.testing.assert_array_equal(perm[p], np.arange(40))
+
+
+def test_next_fast_len():
+    def smooth(v):
+        for p in (2, 3, 5):
+            while v % p == 0:
+                v //= p
+        return v == 1
+    for t in range(7, 400):
+        assert O.next_fast_len(t) == next(v for v in itertools.count(t) if smooth(v))
+    for t in range(1, 7):
+        assert O.next_fast_len(t) == t
+    # scipy's rule (which also admits 7 and 11) agrees on 2,3,5-smooth inputs
+    assert all(scipy_next_fast_len(O.next_fast_len(t)) == O.next_fast_len(t)
+               for t in range(7, 400))
