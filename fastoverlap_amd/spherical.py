@@ -60,13 +60,14 @@ class BaseSphericalAlignment:
         # the kernel returns (avg1 + avg2) / 6 with avg1 == avg2
         return float(s[0] * 3.0)
 
-    def setJ(self, Jmax):
-        """``sphericalAlignment.py:44-55``."""
+    def setJ(self, Jmax, bw=None):
+        """``sphericalAlignment.py:44-55``.  ``bw`` (extension, default
+        Jmax+1 as in the reference) sets the SOFT bandwidth: grid side 2*bw."""
         self.Jmax = Jmax
         self.J = np.arange(Jmax + 1)
         self.Jinds = np.array([j for j in range(Jmax + 1) for _ in range(-j, j + 1)])
         self.Minds = np.array([m for j in range(Jmax + 1) for m in range(-j, j + 1)])
-        self.soft = SOFT(Jmax + 1)
+        self.soft = SOFT(Jmax + 1 if bw is None else int(bw))
         trip = [(l, m1, m2) for l in range(Jmax + 1)
                 for m1 in range(-l, l + 1) for m2 in range(-l, l + 1)]
         self.Js, self.m1s, self.m2s = (np.array(v) for v in zip(*trip))
@@ -161,7 +162,7 @@ class BaseSphericalAlignment:
 
     # -- the hot path -------------------------------------------------------
     def _grid_n(self):
-        return 2 * (self.Jmax + 1)
+        return self.soft.n
 
     def align(self, pos1, pos2, perm=None, invert=True, calcCoeffs=None):
         """``sphericalAlignment.py:160-188``: returns (dist, X1, X2)."""
@@ -205,7 +206,8 @@ class BaseSphericalAlignment:
         """iSOFT -> exact argmax -> refinement (``:190-194``): (R, fun)."""
         Ilmm = np.asarray(Ilmm, np.complex128)
         L = Ilmm.shape[0] - 1
-        with SphericalBatch([1], [L], [1.0], self._grid_n(), None, _lib.SPH_NO_INVERT) as b:
+        with SphericalBatch([1], [L], [1.0], max(self._grid_n(), 2 * (L + 1)), None,
+                            _lib.SPH_NO_INVERT) as b:
             b.set_coeffs(dense_to_compact(Ilmm))
             b.run(_lib.STAGE_GRID | _lib.STAGE_ROTOPT)
             o = b.opt()
@@ -233,15 +235,16 @@ class BaseSphericalAlignment:
 
 class SphericalAlign(BaseSphericalAlignment):
     """``SphericalAlign(scale=None, Jmax=15, perm=None)``
-    (``sphericalAlignment.py:250-273``)."""
+    (``sphericalAlignment.py:250-273``).  ``bw`` is an extension: the SOFT
+    bandwidth (grid side 2*bw), default Jmax+1 as in the reference."""
 
-    def __init__(self, scale=None, Jmax=15, perm=None):
+    def __init__(self, scale=None, Jmax=15, perm=None, bw=None):
         if scale is not None:
             self.scale = scale
             self.calcScale = False
         else:
             self.calcScale = True
-        self.setJ(Jmax)
+        self.setJ(Jmax, bw)
         self.perm = perm
 
     def calcSO3Coeffs(self, pos1, pos2):
