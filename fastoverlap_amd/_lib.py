@@ -122,6 +122,10 @@ _PROTOS = {
     "fo_server_wait": (C.c_int, [_vp, C.c_longlong, C.c_double, C.POINTER(Result), _pd, _pi]),
     "fo_server_stats": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                   C.POINTER(C.c_ulonglong)]),
+    "fo_f90_cluster_align": (C.c_int, [_pd, _pd, C.c_int, C.c_int, _pd, C.c_int, C.c_int, _pi,
+                                       _pi, _pd, _pd, _pd, _pi, _pi]),
+    "fo_f90_bulk_align": (C.c_int, [_pd, _pd, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_int, _pi, _pi, _pd, _pd, _pi, _pd]),
 }
 
 _lib = None

@@ -21,7 +21,9 @@ LIB = os.path.join(HERE, "libfastoverlap_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FO_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["fo_spherical.hip", "fo_periodic.hip", "fo_api.cpp", "fo_server.cpp"]
+SOURCES = ["fo_spherical.hip", "fo_periodic.hip", "fo_api.cpp", "fo_server.cpp", "fo_f90.cpp"]
+FORTRAN = os.path.join(HERE, "fortran")
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -59,7 +61,26 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    if os.path.exists(FLANG):
+        build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the iso_c_binding modules and the Fortran test caller
+    (fortran/_build/test_align, linked against the in-tree library)."""
+    out = os.path.join(FORTRAN, "_build")
+    os.makedirs(out, exist_ok=True)
+    exe = os.path.join(out, "test_align")
+    cmd = [FLANG, "-O2", "-J", out, os.path.join(FORTRAN, "fastoverlap_hip.f90"),
+           os.path.join(FORTRAN, "test_align.f90"), "-o", exe, "-L", HERE,
+           "-lfastoverlap_hip", f"-Wl,-rpath,$ORIGIN/../..", f"-Wl,-rpath,{HERE}"]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=out)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stderr}")
+    if verbose:
+        print(f"built {exe}")
+    return exe
 
 
 if __name__ == "__main__":

@@ -1,0 +1,183 @@
+// Fortran-facing entry points, bound by the iso_c_binding modules in
+// fastoverlap_amd/fortran/fastoverlap_hip.f90.  They mirror the argument
+// meaning of the reference's Fortran ALIGN routines:
+//   CLUSTERFASTOVERLAP::ALIGN (f90/fastclusters.f90:129-269)
+//   BULKFASTOVERLAP::ALIGN    (f90/fastbulk.f90:279-334)
+// with permutation groups in the 1-based SETPERM layout
+// (f90/fastutils.f90:117-167).  Errors are returned as status codes (the
+// reference STOPs); the numerical work is the batch pipeline of fo_api.cpp.
+#include <cmath>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fastoverlap_hip.h"
+#include "fo_host.h"
+
+namespace {
+
+std::mutex g_mu;
+fo_context* g_ctx = nullptr;
+
+int default_ctx(fo_context** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_ctx) {
+    const char* dev = std::getenv("FASTOVERLAP_DEVICE");
+    int rc = fo_context_create(dev ? std::atoi(dev) : 0, 0, &g_ctx);
+    if (rc) return rc;
+  }
+  *out = g_ctx;
+  return FO_OK;
+}
+
+int groups_from_f90(int natoms, int ngroups, const int* npermsize, const int* permgroup,
+                    std::vector<int>& sizes, std::vector<int>& atoms) {
+  if (ngroups <= 0) {
+    sizes.assign(1, natoms);
+    atoms.resize(natoms);
+    for (int a = 0; a < natoms; ++a) atoms[a] = a;
+    return FO_OK;
+  }
+  int tot = 0;
+  for (int g = 0; g < ngroups; ++g) {
+    sizes.push_back(npermsize[g]);
+    tot += npermsize[g];
+  }
+  atoms.resize(tot);
+  for (int i = 0; i < tot; ++i) {
+    atoms[i] = permgroup[i] - 1;
+    if (atoms[i] < 0 || atoms[i] >= natoms)
+      return fo::set_error(FO_ERR_INVALID, "PERMGROUP entry out of range (1-based indices)");
+  }
+  return FO_OK;
+}
+
+int next_fast_len(int target) {
+  if (target <= 6) return target;
+  long long best = 1LL << 40;
+  for (long long p5 = 1; p5 < 2LL * target; p5 *= 5)
+    for (long long p35 = p5; p35 < 2LL * target; p35 *= 3) {
+      long long N = p35;
+      while (N < target) N *= 2;
+      if (N < best) best = N;
+    }
+  return (int)best;
+}
+
+}  // namespace
+
+extern "C" {
+
+// COORDSB (target) and COORDSA are returned centred; COORDSA is replaced by
+// the aligned, permuted (and possibly inverted) structure.  KWIDTH <= 0
+// selects the automatic width and returns it.  RMATBEST (column-major 3x3)
+// is the rotation applied to the (inverted, if *inverted) COORDSA.
+int fo_f90_cluster_align(double* coordsb, double* coordsa, int natoms, int L, double* kwidth,
+                         int invert, int ngroups, const int* npermsize, const int* permgroup,
+                         double* distance, double* dist2, double* rmatbest, int* bestperm,
+                         int* inverted) {
+  if (!coordsa || !coordsb || !kwidth || natoms < 1)
+    return fo::set_error(FO_ERR_INVALID, "bad argument");
+  fo_context* ctx = nullptr;
+  int rc = default_ctx(&ctx);
+  if (rc) return rc;
+  std::vector<int> sizes, atoms;
+  rc = groups_from_f90(natoms, ngroups, npermsize, permgroup, sizes, atoms);
+  if (rc) return rc;
+  int nat = natoms, ng = (int)sizes.size();
+  double sc = *kwidth;
+  fo_sph_batch_desc d{};
+  d.npairs = 1;
+  d.grid_n = 2 * (L + 1);
+  d.natoms = &nat;
+  d.L = &L;
+  d.scale = &sc;
+  d.ngroups = &ng;
+  d.group_sizes = sizes.data();
+  d.group_atoms = atoms.data();
+  d.flags = invert ? 0 : FO_SPH_NO_INVERT;
+  fo_sph_batch* b = nullptr;
+  rc = fo_sph_batch_create(ctx, &d, &b);
+  if (rc) return rc;
+  std::vector<double> X1(3 * (size_t)natoms), X2(3 * (size_t)natoms);
+  std::vector<int> perm(natoms);
+  double dist = 0, R[3], rmat[9];
+  int inv = 0;
+  rc = fo_sph_batch_set_positions(b, coordsb, coordsa);
+  if (!rc) rc = fo_sph_batch_run(b, FO_STAGE_ALL, 0);
+  if (!rc) rc = fo_sph_batch_results(b, &dist, X1.data(), X2.data(), perm.data(), R, rmat, &inv);
+  if (!rc) rc = fo_sph_batch_scales(b, &sc);
+  fo_sph_batch_destroy(b);
+  if (rc) return rc;
+  for (size_t i = 0; i < X1.size(); ++i) {
+    coordsb[i] = X1[i];
+    coordsa[i] = X2[i];
+  }
+  *kwidth = sc;
+  if (distance) *distance = dist;
+  if (dist2) *dist2 = dist * dist;
+  const double sg = inv ? -1.0 : 1.0;
+  // row-vector map X2c -> X2out is rmat (row-major); as a column-vector map
+  // of the (inverted) input it is (rmat/sg)^T, whose column-major storage is
+  // the row-major rmat / sg
+  if (rmatbest)
+    for (int k = 0; k < 9; ++k) rmatbest[k] = rmat[k] / sg;
+  if (bestperm)
+    for (int i = 0; i < natoms; ++i) bestperm[i] = perm[i] + 1;
+  if (inverted) *inverted = inv;
+  return FO_OK;
+}
+
+// COORDSA is replaced by the aligned, permuted structure (wrapped into the
+// box); COORDSB is unchanged.  KERNELWIDTH <= 0 selects (V/N)^(1/3)/3;
+// NWAVE = ceil(1.3 N^(1/3)) and the FFT side is the next 2,3,5-smooth size
+// >= 4 NWAVE + 3 (f90/fastbulk.f90:293-319).
+int fo_f90_bulk_align(double* coordsb, double* coordsa, int natoms, double boxlx, double boxly,
+                      double boxlz, double kernelwidth, int ngroups, const int* npermsize,
+                      const int* permgroup, double* distance, double* dist2, int* bestperm,
+                      double* disp_out) {
+  if (!coordsa || !coordsb || natoms < 1) return fo::set_error(FO_ERR_INVALID, "bad argument");
+  fo_context* ctx = nullptr;
+  int rc = default_ctx(&ctx);
+  if (rc) return rc;
+  std::vector<int> sizes, atoms;
+  rc = groups_from_f90(natoms, ngroups, npermsize, permgroup, sizes, atoms);
+  if (rc) return rc;
+  int nat = natoms, ng = (int)sizes.size();
+  double box[3] = {boxlx, boxly, boxlz};
+  double sc = kernelwidth > 0 ? kernelwidth : std::cbrt(boxlx * boxly * boxlz / natoms) / 3.0;
+  const int nk = (int)std::ceil(1.3 * std::cbrt((double)natoms));
+  fo_per_batch_desc d{};
+  d.npairs = 1;
+  d.nk = nk;
+  d.F = next_fast_len(4 * nk + 3);
+  d.natoms = &nat;
+  d.box = box;
+  d.scale = &sc;
+  d.ngroups = &ng;
+  d.group_sizes = sizes.data();
+  d.group_atoms = atoms.data();
+  d.niter = 10;
+  d.flags = 0;
+  fo_per_batch* b = nullptr;
+  rc = fo_per_batch_create(ctx, &d, &b);
+  if (rc) return rc;
+  std::vector<double> X2(3 * (size_t)natoms);
+  std::vector<int> perm(natoms);
+  double dist = 0, disp[3];
+  rc = fo_per_batch_set_positions(b, coordsb, coordsa);
+  if (!rc) rc = fo_per_batch_run(b, FO_PSTAGE_ALL, 0);
+  if (!rc) rc = fo_per_batch_results(b, &dist, nullptr, X2.data(), perm.data(), disp);
+  fo_per_batch_destroy(b);
+  if (rc) return rc;
+  for (size_t i = 0; i < X2.size(); ++i) coordsa[i] = X2[i];
+  if (distance) *distance = dist;
+  if (dist2) *dist2 = dist * dist;
+  if (bestperm)
+    for (int i = 0; i < natoms; ++i) bestperm[i] = perm[i] + 1;
+  if (disp_out)
+    for (int k = 0; k < 3; ++k) disp_out[k] = disp[k];
+  return FO_OK;
+}
+
+}  // extern "C"

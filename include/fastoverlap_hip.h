@@ -255,6 +255,25 @@ int fo_server_wait(fo_server* s, long long id, double timeout_ms, fo_result* res
 int fo_server_stats(fo_server* s, long long* completed, long long* batches,
                     unsigned long long* peak_bytes);
 
+/* ---- Fortran callers ------------------------------------------------------
+ * Bound by the iso_c_binding modules CLUSTERFASTOVERLAP_HIP,
+ * BULKFASTOVERLAP_HIP and FASTOVERLAPUTILS_HIP (fastoverlap_amd/fortran).
+ * Argument meaning follows CLUSTERFASTOVERLAP::ALIGN
+ * (f90/fastclusters.f90:129-155) and BULKFASTOVERLAP::ALIGN
+ * (f90/fastbulk.f90:279-289); permgroup is 1-based as in SETPERM
+ * (f90/fastutils.f90:117-167); ngroups = 0 means one group of all atoms.
+ * RMATBEST is column-major and acts on column vectors of the (inverted,
+ * when *inverted) input; bestperm is 1-based.
+ */
+int fo_f90_cluster_align(double* coordsb, double* coordsa, int natoms, int L, double* kwidth,
+                         int invert, int ngroups, const int* npermsize, const int* permgroup,
+                         double* distance, double* dist2, double* rmatbest, int* bestperm,
+                         int* inverted);
+int fo_f90_bulk_align(double* coordsb, double* coordsa, int natoms, double boxlx, double boxly,
+                      double boxlz, double kernelwidth, int ngroups, const int* npermsize,
+                      const int* permgroup, double* distance, double* dist2, int* bestperm,
+                      double* disp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,39 @@
+"""The Fortran caller (iso_c_binding modules, built in-tree by
+fastoverlap_amd/build.py) reproduces the Python drop-in results."""
+
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import fastoverlap_amd as fo
+from oracle import fo_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "fastoverlap_amd", "fortran", "_build", "test_align")
+
+
+def test_fortran_align_matches_dropin(gpu, lj38, blj256):
+    assert os.path.exists(EXE), "build the Fortran caller with python -m fastoverlap_amd.build"
+    print("case: Fortran test_align LJ38 (L=14) and BLJ256", flush=True)
+    r = subprocess.run([EXE, os.path.join(ROOT, "tests", "golden")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.splitlines() if ln.strip()}
+    cdist, cstat = float(out["CLUSTER"][0]), int(out["CLUSTER"][1])
+    bdist, bstat = float(out["BULK"][0]), int(out["BULK"][1])
+    assert cstat == 0 and bstat == 0
+    p1, p2 = lj38
+    o = O.spherical_align(p1, p2, 14, 0.3, fast=True)
+    assert abs(cdist - o.dist) <= 1e-8
+    cperm = np.array([int(v) for v in out["CPERM"]]) - 1
+    np.testing.assert_array_equal(cperm, o.perm)
+    q1, q2 = blj256
+    box = np.ones(3) * 5.975206329
+    groups = [np.arange(204), np.arange(204, 256)]
+    dist = fo.PeriodicAlign(256, box, groups)(q1, q2)[0]
+    assert abs(bdist - dist) <= 1e-8
