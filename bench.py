@@ -175,6 +175,11 @@ def main():
 
     # batched results kept for the plain-path comparison
     bres = {}
+    its = []
+    for n, sel, b in batches:
+        its.append(b.opt()["iters"].ravel())
+    its = np.concatenate(its)
+    log(f"rotation refinement: Newton iterations mean {its.mean():.2f}, max {its.max()}")
     for n, sel, b in batches:
         r = b.results()
         pk = b.peaks()

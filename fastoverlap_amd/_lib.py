@@ -21,7 +21,7 @@ ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "unsuppo
           5: "state", 6: "not found", 7: "cancelled"}
 
 # spherical flags / stages
-SPH_NO_INVERT, SPH_KEEP_GRID, SPH_NO_CENTER, SPH_GENERAL = 1, 2, 4, 8
+SPH_NO_INVERT, SPH_KEEP_GRID, SPH_NO_CENTER, SPH_GENERAL, SPH_DFT = 1, 2, 4, 8, 16
 STAGE_PREP, STAGE_COEFF, STAGE_GRID, STAGE_ROTOPT, STAGE_FINISH, STAGE_ALL = 1, 2, 4, 8, 16, 31
 # periodic flags / stages
 PER_KEEP_GRID, PER_INPUT_COEFFS, PER_KEEP_COEFFS = 2, 4, 8

@@ -33,19 +33,22 @@ void launch_sph_radial(hipStream_t, int, long long, const SphPair*, const SphGro
                        const double*, const double*, const double*, double*);
 void launch_sph_contract(hipStream_t, int, int, const SphPair*, const SphGroup*, const double2*,
                          const double2*, const double*, const double*, double2*, float2*);
-size_t isoft_plane_smem(int, int);
-void launch_isoft_plane(hipStream_t, int, int, int, long long, const SphPair*, const float*,
+size_t isoft_plane_smem(int, int, int);
+int isoft_plane_chunk(int, int, size_t);
+void launch_isoft_plane(hipStream_t, int, int, int, int, long long, const SphPair*, const float*,
                         const float2*, const float2*, const float2*, int, int, PlaneCand*, float*);
 void launch_sph_select(hipStream_t, int, int, long long, int, const SphPair*, const double*,
                        const double2*, const double2*, const double2*, int, const PlaneCand*,
                        double2*, long long, SphPeak*);
 void launch_sph_rotopt(hipStream_t, int, int, const SphPair*, const double2*, const double2*, int,
-                       const SphPeak*, SphOpt*);
+                       const SphPeak*, const double*, const double*, int, SphOpt*);
+void launch_rec_table(hipStream_t, int, double*, double*);
 void launch_sph_finish(hipStream_t, int, int, const SphPair*, const SphGroup*, const int*,
                        const double*, const double*, const SphOpt*, int, double*, int*, int*,
                        double*, SphResult*);
 void launch_c64_to_c32(hipStream_t, long long, const double2*, float2*);
-void launch_sph_energy(hipStream_t, int, const double2*, double, double, double, double*);
+void launch_sph_energy(hipStream_t, int, const double2*, double, double, double, const double*,
+                       const double*, int, double*);
 void launch_wigner_D(hipStream_t, int, double, double, double, double2*, double2*);
 void launch_ylm_angles(hipStream_t, int, int, const double*, const double*, double2*);
 size_t per_sf_smem(int);
@@ -110,6 +113,14 @@ struct SphTable {
   double2* tw64 = nullptr;
   float2* tw32 = nullptr;
 };
+// Recurrence coefficients of the unnormalised Wigner d for every (l, m1, m2)
+// with l <= kRecL, shared by all batches of a context.
+static constexpr int kRecL = 63;
+struct RecTable {
+  double* rec = nullptr;    // 3 * K(kRecL)
+  double* seedu = nullptr;  // (2 kRecL + 1)^2
+};
+
 struct PerTw {
   int F = 0;
   double2* tw64 = nullptr;
@@ -126,6 +137,7 @@ struct fo_context {
   std::mutex call_mu;  // serialises the single-call helpers
   std::map<int, SphTable> sph;
   std::map<int, PerTw> per;
+  RecTable rect;
   std::vector<void*> owned;  // table allocations
 };
 
@@ -187,6 +199,26 @@ static int get_sph_table(fo_context* c, int n, const SphTable** out) {
   FO_HIP(hipStreamSynchronize(c->stream));
   c->sph[n] = t;
   *out = &c->sph[n];
+  return FO_OK;
+}
+
+static int get_rec_table(fo_context* c, const RecTable** out) {
+  std::lock_guard<std::mutex> g(c->mu);
+  if (!c->rect.rec) {
+    const size_t b1 = sizeof(double) * 3 * (size_t)kcount(kRecL);
+    const size_t b2 = sizeof(double) * (size_t)(2 * kRecL + 1) * (2 * kRecL + 1);
+    void* base = nullptr;
+    hipError_t e = hipMalloc(&base, b1 + b2 + 256);
+    if (e != hipSuccess) return set_err(FO_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->used += b1 + b2 + 256;
+    c->owned.push_back(base);
+    c->rect.rec = (double*)base;
+    c->rect.seedu = (double*)((char*)base + (b1 + 255) / 256 * 256);
+    launch_rec_table(c->stream, kRecL, c->rect.rec, c->rect.seedu);
+    FO_CHECK_LAUNCH();
+    FO_HIP(hipStreamSynchronize(c->stream));
+  }
+  *out = &c->rect;
   return FO_OK;
 }
 
@@ -287,9 +319,11 @@ struct fo_sph_batch {
   fo_context* ctx = nullptr;
   hipStream_t stream = nullptr;
   const SphTable* tab = nullptr;
+  const RecTable* rect = nullptr;
   SphLaunch lc{};
   int flags = 0;
   int maxgroup = 1;
+  int fft_chunk = 0;  // rows per FFT pass of the plane kernel (0: direct DFT)
   long long Ntot = 0, Ktot = 0;
   std::vector<SphPair> hp;
   std::vector<SphGroup> hg;
@@ -351,8 +385,8 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
   }
   mark(4);
   if (stages & FO_STAGE_GRID) {
-    launch_isoft_plane(s, P, L.n, L.Lmax, b->tab->Kt, b->pd, b->tab->tab32, b->tab->tw32, b->I32A,
-                       b->I32B, L.mode, L.norient, b->cand, b->grid);
+    launch_isoft_plane(s, P, L.n, L.Lmax, b->fft_chunk, b->tab->Kt, b->pd, b->tab->tab32,
+                       b->tab->tw32, b->I32A, b->I32B, L.mode, L.norient, b->cand, b->grid);
     mark(5);
     launch_sph_select(s, P, L.n, b->tab->Kt, L.norient, b->pd, b->tab->tab64, b->tab->tw64,
                       b->I64A, b->I64B, L.mode, b->cand, b->scr, b->scr_stride, b->peaks);
@@ -360,8 +394,9 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
     mark(5);
   }
   mark(6);
-  if (stages & FO_STAGE_ROTOPT) launch_sph_rotopt(s, P, L.norient, b->pd, b->I64A, b->I64B, L.mode, b->peaks,
-                                                 b->opt);
+  if (stages & FO_STAGE_ROTOPT)
+    launch_sph_rotopt(s, P, L.norient, b->pd, b->I64A, b->I64B, L.mode, b->peaks, b->rect->rec,
+                      b->rect->seedu, kRecL, b->opt);
   mark(7);
   if (stages & FO_STAGE_FINISH)
     launch_sph_finish(s, P, b->maxgroup, b->pd, b->grp, b->gath, b->X1c, b->X2c, b->opt,
@@ -451,13 +486,16 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
     gq += sp.ngroups;
     ga += ng;
   }
-  if (isoft_plane_smem(n, Lmax) > c->max_lds)
+  if (isoft_plane_smem(n, Lmax, 0) > c->max_lds)
     return set_err(FO_ERR_UNSUPPORTED, "grid_n and Jmax too large for one beta plane in LDS (" +
-                                           std::to_string(isoft_plane_smem(n, Lmax)) + " > " +
+                                           std::to_string(isoft_plane_smem(n, Lmax, 0)) + " > " +
                                            std::to_string(c->max_lds) + " bytes)");
+  b->fft_chunk = (d->flags & FO_SPH_DFT) ? 0 : isoft_plane_chunk(n, Lmax, c->max_lds);
   if (lap_lds_bytes(b->maxgroup) > c->max_lds)
     return set_err(FO_ERR_UNSUPPORTED, "permutation group too large for the LDS assignment solver");
   int rc = get_sph_table(c, n, &b->tab);
+  if (rc) return rc;
+  rc = get_rec_table(c, &b->rect);
   if (rc) return rc;
   if (Lmax > b->tab->Lt) return set_err(FO_ERR_UNSUPPORTED, "Jmax exceeds the table degree");
   SphLaunch& lc = b->lc;
@@ -849,6 +887,9 @@ static void dense_to_compact_conj(int L, const double* X, std::vector<double>& o
 
 int fo_sph_energy(fo_context* c, int L, const double* X, const double* rot, double* f, double* grad) {
   if (!c || !X || !rot || L < 0 || L > kMaxL) return set_err(FO_ERR_INVALID, "bad argument");
+  const RecTable* rt = nullptr;
+  int rc0 = get_rec_table(c, &rt);
+  if (rc0) return rc0;
   std::lock_guard<std::mutex> g(c->call_mu);
   FO_HIP(hipSetDevice(c->device));
   std::vector<double> h;
@@ -859,7 +900,7 @@ int fo_sph_energy(fo_context* c, int L, const double* X, const double* rot, doub
   double2* dI = (double2*)s.p;
   double* dout = (double*)((char*)s.p + (sizeof(double) * h.size() + 255) / 256 * 256);
   FO_HIP(hipMemcpyAsync(dI, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
-  launch_sph_energy(c->stream, L, dI, rot[0], rot[1], rot[2], dout);
+  launch_sph_energy(c->stream, L, dI, rot[0], rot[1], rot[2], rt->rec, rt->seedu, kRecL, dout);
   FO_CHECK_LAUNCH();
   double o[10];
   FO_HIP(hipMemcpyAsync(o, dout, sizeof(o), hipMemcpyDeviceToHost, c->stream));

@@ -84,6 +84,27 @@ FO_DEV T block_sum(T v, T* scratch) {
   return r;
 }
 
+// Deterministic block sum of NV values at once (two barriers).  `scratch`
+// must hold NV * blockDim.x/64 elements.  All threads receive the sums.
+template <int NV, typename T>
+FO_DEV void block_sum_n(T (&v)[NV], T* scratch) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) scratch[wid * NV + k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    T r = 0;
+    for (int w = 0; w < nw; ++w) r += scratch[w * NV + k];
+    v[k] = r;
+  }
+}
+
 // Block argmax on (value, index): larger value wins, ties -> smaller index.
 FO_DEV void argmax_merge(float& v, int& i, float ov, int oi) {
   if (ov > v || (ov == v && oi < i)) {

@@ -402,11 +402,43 @@ __global__ void __launch_bounds__(256) k_sph_contract(
 // Each plane keeps its kCandPerPlane largest values and an upper bound on
 // the rest, for the FP64 selection kernel.
 // ---------------------------------------------------------------------------
+// Inverse radix-2 Stockham FFTs (e^{+2 pi i jk/n}, unnormalised) of `rows`
+// rows of length n = 2^logn held in a[rows][n]; b is scratch of the same
+// size.  Returns the buffer holding the result (natural order).
+FO_DEV float2* fft_rows_inv(float2* a, float2* b, int rows, int n, int logn,
+                            const float2* __restrict__ tw) {
+  const int half = n >> 1;
+  for (int st = 0; st < logn; ++st) {
+    const int Ns = 1 << st;
+    const int tws = n >> (st + 1);
+    for (int e = threadIdx.x; e < rows * half; e += blockDim.x) {
+      const int r = e >> (logn - 1);
+      const int j = e & (half - 1);
+      const float2* in = a + r * n;
+      float2* out = b + r * n;
+      const int k = j & (Ns - 1);
+      const float2 v0 = in[j];
+      const float2 v1 = cmul(in[j + half], tw[k * tws]);
+      const int o = ((j - k) << 1) + k;
+      out[o] = make_float2(v0.x + v1.x, v0.y + v1.y);
+      out[o + Ns] = make_float2(v0.x - v1.x, v0.y - v1.y);
+    }
+    __syncthreads();
+    float2* tmp = a;
+    a = b;
+    b = tmp;
+  }
+  return a;
+}
+
+// logn >= 0: n = 2^logn, the gamma and alpha passes are FFTs processed in
+// chunks of `chunk` rows / columns through the workspace WS[2][chunk][n];
+// logn < 0: direct DFT passes (any n).
 __global__ void __launch_bounds__(256) k_isoft_plane(
-    const SphPair* __restrict__ pd, int n, long long Kt, const float* __restrict__ tab32,
-    const float2* __restrict__ twg, const float2* __restrict__ IA,
-    const float2* __restrict__ IB, int mode, int norient, PlaneCand* __restrict__ cand,
-    float* __restrict__ grid) {
+    const SphPair* __restrict__ pd, int n, int logn, int chunk, long long Kt,
+    const float* __restrict__ tab32, const float2* __restrict__ twg,
+    const float2* __restrict__ IA, const float2* __restrict__ IB, int mode, int norient,
+    PlaneCand* __restrict__ cand, float* __restrict__ grid) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int ib = blockIdx.x, p = blockIdx.y;
   const SphPair d = pd[p];
@@ -414,7 +446,8 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
   float2* tw = (float2*)smem;                 // [n]
   float2* S = tw + n;                          // [2][L+1][W]
   float2* T = S + 2 * (L + 1) * W;             // [2][L+1][n]
-  float* rv = (float*)(T + 2 * (L + 1) * n);   // reduction scratch
+  float2* WS = T + 2 * (L + 1) * n;            // [2][chunk][n] (FFT path)
+  float* rv = (float*)(WS + (logn >= 0 ? 2 * chunk * n : 0));  // reduction scratch
   int* ri = (int*)(rv + 16);
   for (int k = t; k < n; k += bd) tw[k] = twg[k];
   const float* tb = tab32 + (long long)ib * Kt;
@@ -455,50 +488,107 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
     }
   }
   __syncthreads();
-  // gamma pass
-  for (int e = t; e < norient * (L + 1) * n; e += bd) {
-    const int o = e / ((L + 1) * n);
-    const int rem = e % ((L + 1) * n);
-    const int m1 = rem / n, g = rem % n;
-    const float2* Sr = S + (o * (L + 1) + m1) * W;
-    float2 acc = make_float2(0.f, 0.f);
-    int ph = (n - (L % n) * g % n) % n;  // (-L * g) mod n
-    for (int m2i = 0; m2i < W; ++m2i) {
-      const float2 w = tw[ph];
-      const float2 s = Sr[m2i];
-      acc.x += s.x * w.x - s.y * w.y;
-      acc.y += s.x * w.y + s.y * w.x;
-      ph += g;
-      if (ph >= n) ph -= n;
-    }
-    T[(o * (L + 1) + m1) * n + g] = acc;
-  }
-  __syncthreads();
-  // alpha pass + per-thread candidates
   float bv[2] = {-INFINITY, -INFINITY}, dm[2] = {-INFINITY, -INFINITY};
   int bi[2] = {0x7fffffff, 0x7fffffff};
-  for (int e = t; e < n * n; e += bd) {
-    const int a = e / n, g = e % n;
-    for (int o = 0; o < norient; ++o) {
-      const float2* To = T + o * (L + 1) * n;
-      float acc = 0.f;
-      int ph = 0;
-      for (int m1 = 1; m1 <= L; ++m1) {
-        ph += a;
-        if (ph >= n) ph -= n;
-        const float2 w = tw[ph];
-        const float2 v = To[m1 * n + g];
-        acc += v.x * w.x - v.y * w.y;
+  auto consider = [&](int o, int a, int g, float val) {
+    const int flat = (a * n + ib) * n + g;
+    if (grid) grid[((long long)p * norient + o) * n * n * n + flat] = val;
+    if (val > bv[o]) {
+      dm[o] = fmaxf(dm[o], bv[o]);
+      bv[o] = val;
+      bi[o] = flat;
+    } else {
+      dm[o] = fmaxf(dm[o], val);
+    }
+  };
+  if (logn >= 0) {
+    float2* wa = WS;
+    float2* wb = WS + chunk * n;
+    // gamma pass: rows (o, m1), input S[o][m1][m2] at index m2 mod n
+    const int nrows = norient * (L + 1);
+    for (int r0 = 0; r0 < nrows; r0 += chunk) {
+      const int rows = min(chunk, nrows - r0);
+      for (int e = t; e < rows * n; e += bd) {
+        const int r = e / n, k = e % n;
+        const int m2 = k <= L ? k : k - n;
+        wa[e] = (m2 >= -L) ? S[(r0 + r) * W + m2 + L] : make_float2(0.f, 0.f);
       }
-      const float val = To[g].x + 2.f * acc;
-      const int flat = (a * n + ib) * n + g;
-      if (grid) grid[((long long)p * norient + o) * n * n * n + flat] = val;
-      if (val > bv[o]) {
-        dm[o] = fmaxf(dm[o], bv[o]);
-        bv[o] = val;
-        bi[o] = flat;
-      } else {
-        dm[o] = fmaxf(dm[o], val);
+      __syncthreads();
+      const float2* res = fft_rows_inv(wa, wb, rows, n, logn, tw);
+      for (int e = t; e < rows * n; e += bd) T[r0 * n + e] = res[e];
+      __syncthreads();
+    }
+    // alpha pass: column g holds X[m1] = T_A[m1][g] + i T_B[m1][g] for
+    // m1 >= 0 and its Hermitian extension for m1 < 0; Re / Im of the inverse
+    // transform are the two real grids.
+    const float2* TA = T;
+    const float2* TB = T + (L + 1) * n;
+    for (int g0 = 0; g0 < n; g0 += chunk) {
+      const int cols = min(chunk, n - g0);
+      for (int e = t; e < cols * n; e += bd) {
+        const int c = e / n, k = e % n;
+        const int g = g0 + c;
+        const int m1 = k <= L ? k : k - n;
+        float2 v = make_float2(0.f, 0.f);
+        if (m1 == 0) {
+          // T[0][g] is real for a Hermitian set: keep the real parts only
+          v = make_float2(TA[g].x, norient == 2 ? TB[g].x : 0.f);
+        } else if (m1 > 0) {
+          const float2 A = TA[m1 * n + g];
+          const float2 B = norient == 2 ? TB[m1 * n + g] : make_float2(0.f, 0.f);
+          v = make_float2(A.x - B.y, A.y + B.x);
+        } else if (m1 >= -L) {
+          const float2 A = TA[-m1 * n + g];
+          const float2 B = norient == 2 ? TB[-m1 * n + g] : make_float2(0.f, 0.f);
+          v = make_float2(A.x + B.y, B.x - A.y);
+        }
+        wa[e] = v;
+      }
+      __syncthreads();
+      const float2* res = fft_rows_inv(wa, wb, cols, n, logn, tw);
+      for (int e = t; e < cols * n; e += bd) {
+        const int c = e / n, a = e % n;
+        const float2 v = res[e];
+        consider(0, a, g0 + c, v.x);
+        if (norient == 2) consider(1, a, g0 + c, v.y);
+      }
+      __syncthreads();
+    }
+  } else {
+    // gamma pass
+    for (int e = t; e < norient * (L + 1) * n; e += bd) {
+      const int o = e / ((L + 1) * n);
+      const int rem = e % ((L + 1) * n);
+      const int m1 = rem / n, g = rem % n;
+      const float2* Sr = S + (o * (L + 1) + m1) * W;
+      float2 acc = make_float2(0.f, 0.f);
+      int ph = (n - (L % n) * g % n) % n;  // (-L * g) mod n
+      for (int m2i = 0; m2i < W; ++m2i) {
+        const float2 w = tw[ph];
+        const float2 sv = Sr[m2i];
+        acc.x += sv.x * w.x - sv.y * w.y;
+        acc.y += sv.x * w.y + sv.y * w.x;
+        ph += g;
+        if (ph >= n) ph -= n;
+      }
+      T[(o * (L + 1) + m1) * n + g] = acc;
+    }
+    __syncthreads();
+    // alpha pass
+    for (int e = t; e < n * n; e += bd) {
+      const int a = e / n, g = e % n;
+      for (int o = 0; o < norient; ++o) {
+        const float2* To = T + o * (L + 1) * n;
+        float acc = 0.f;
+        int ph = 0;
+        for (int m1 = 1; m1 <= L; ++m1) {
+          ph += a;
+          if (ph >= n) ph -= n;
+          const float2 w = tw[ph];
+          const float2 v = To[m1 * n + g];
+          acc += v.x * w.x - v.y * w.y;
+        }
+        consider(o, a, g, To[g].x + 2.f * acc);
       }
     }
   }
@@ -794,58 +884,87 @@ __global__ void __launch_bounds__(256) k_sph_select(
 // L-BFGS-B from the grid angle; here a damped Newton iteration with the
 // exact gradient and Hessian (d, d', d'' by the differentiated recurrence)
 // converges to the same local optimum to full FP64 precision.
+//
+// The recurrence coefficients of the unnormalised d,
+//   d^{l+1} = Bu_l (cos b - C_l) d^l - Au_l d^{l-1},
+//   Bu_l = B_l sqrt((2l+1)/(2l+3)),  Au_l = A_l sqrt((2l-1)/(2l+3)),
+// and the edge prefactor xi sqrt(binom(mu+nu, mu)) are precomputed once per
+// context (k_rec_table), so the inner loop has no FP64 square root or
+// division.
 // ---------------------------------------------------------------------------
+__global__ void k_rec_table(int Lt, double* __restrict__ rec, double* __restrict__ seedu) {
+  const int W = 2 * Lt + 1;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < W * W; e += gridDim.x * blockDim.x) {
+    const int m1 = e / W - Lt, m2 = e % W - Lt;
+    const int l0 = max(abs(m1), abs(m2));
+    const int mu = abs(m1 - m2), nu = abs(m1 + m2);
+    seedu[e] = seed_xi(m1, m2) * exp(seed_log_factor(mu, nu));
+    for (int l = l0; l <= Lt; ++l) {
+      double A = 0.0, B = 0.0, C = 0.0;
+      if (l < Lt) recurrence_terms(l, m1, m2, A, B, C);
+      double* r = rec + 3 * kidx(l, m1, m2);
+      r[0] = A * sqrt((2.0 * l - 1.0) / (2.0 * l + 3.0));
+      r[1] = B * sqrt((2.0 * l + 1.0) / (2.0 * l + 3.0));
+      r[2] = C;
+    }
+  }
+}
+
 struct FGH {
   double v[10];  // f, fa, fb, fy, faa, fab, fay, fbb, fby, fyy
 };
 
 FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a, double b,
-                     double y, FGH& out, double* red) {
-  const int W = 2 * L + 1;
+                     double y, const double* __restrict__ rec, const double* __restrict__ seedu,
+                     int Lt, FGH& out, double* red) {
+  const int W = 2 * L + 1, Wt = 2 * Lt + 1;
   const double cb = cos(b), sb = sin(b), c2 = cos(0.5 * b), s2 = sin(0.5 * b);
   double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int e = threadIdx.x; e < (L + 1) * W; e += blockDim.x) {
     const int m1 = e / W, m2 = e % W - L;
     const int l0 = max(m1, abs(m2));
     const int mu = abs(m1 - m2), nu = abs(m1 + m2);
-    // seed g = s2^mu c2^nu and its b-derivatives
-    const double fct = sqrt((2.0 * l0 + 1.0) * 0.5) * seed_xi(m1, m2) * exp(seed_log_factor(mu, nu));
-    const double g0 = ipow(s2, mu) * ipow(c2, nu);
-    const double g1 = 0.5 * ((mu ? mu * ipow(s2, mu - 1) * ipow(c2, nu + 1) : 0.0) -
-                             (nu ? nu * ipow(s2, mu + 1) * ipow(c2, nu - 1) : 0.0));
-    auto dterm = [&](int pp, int qq) {  // d/db (s2^pp c2^qq)
-      return 0.5 * ((pp ? pp * ipow(s2, pp - 1) * ipow(c2, qq + 1) : 0.0) -
-                    (qq ? qq * ipow(s2, pp + 1) * ipow(c2, qq - 1) : 0.0));
-    };
-    const double g2 = 0.5 * ((mu ? mu * dterm(mu - 1, nu + 1) : 0.0) -
-                             (nu ? nu * dterm(mu + 1, nu - 1) : 0.0));
+    // edge element g = s2^mu c2^nu and its b-derivatives
+    const double fct = seedu[(m1 + Lt) * Wt + (m2 + Lt)];
+    const double smu = ipow(s2, mu), cnu = ipow(c2, nu);
+    const double smu1 = mu ? ipow(s2, mu - 1) : 0.0, cnu1 = nu ? ipow(c2, nu - 1) : 0.0;
+    const double smu2 = mu > 1 ? ipow(s2, mu - 2) : 0.0, cnu2 = nu > 1 ? ipow(c2, nu - 2) : 0.0;
+    const double g0 = smu * cnu;
+    const double g1 = 0.5 * (mu * smu1 * cnu * c2 - nu * smu * s2 * cnu1);
+    // g'' = 1/4 [mu(mu-1) s^{mu-2} c^{nu+2} - (mu(nu+1) + nu(mu+1)) s^mu c^nu
+    //            + nu(nu-1) s^{mu+2} c^{nu-2}]
+    const double g2 = 0.25 * (mu * (mu - 1) * smu2 * cnu * c2 * c2 -
+                              (mu * (nu + 1.0) + nu * (mu + 1.0)) * smu * cnu +
+                              nu * (nu - 1) * smu * s2 * s2 * cnu2);
     double c0x = 0, c0y = 0, c1x = 0, c1y = 0, c2x = 0, c2y = 0;
     double d0 = fct * g0, d1 = fct * g1, dd2 = fct * g2;
     double p0 = 0, p1 = 0, p2 = 0;
     double sl = (l0 & 1) ? sgn : 1.0;
+    long long id = kidx(l0, m1, m2);
     for (int l = l0; l <= L; ++l) {
-      const double un = sqrt(2.0 / (2.0 * l + 1.0)) * sl;
-      const double2 Iv = I[kidx(l, m1, m2)];
+      const double2 Iv = I[id];
       // conj(sgn^l I) * d
-      c0x += un * d0 * Iv.x;
-      c0y -= un * d0 * Iv.y;
-      c1x += un * d1 * Iv.x;
-      c1y -= un * d1 * Iv.y;
-      c2x += un * dd2 * Iv.x;
-      c2y -= un * dd2 * Iv.y;
+      const double ix = sl * Iv.x, iy = -sl * Iv.y;
+      c0x += d0 * ix;
+      c0y += d0 * iy;
+      c1x += d1 * ix;
+      c1y += d1 * iy;
+      c2x += dd2 * ix;
+      c2y += dd2 * iy;
       sl *= sgn;
       if (l == L) break;
-      double A, B, C;
-      recurrence_terms(l, m1, m2, A, B, C);
-      const double n0 = B * (cb - C) * d0 - A * p0;
-      const double n1 = B * (-sb * d0 + (cb - C) * d1) - A * p1;
-      const double n2 = B * (-cb * d0 - 2.0 * sb * d1 + (cb - C) * dd2) - A * p2;
+      const double Au = rec[3 * id], Bu = rec[3 * id + 1], C = rec[3 * id + 2];
+      const double bc = Bu * (cb - C);
+      const double n0 = bc * d0 - Au * p0;
+      const double n1 = bc * d1 - Bu * sb * d0 - Au * p1;
+      const double n2 = bc * dd2 - Bu * (cb * d0 + 2.0 * sb * d1) - Au * p2;
       p0 = d0;
       p1 = d1;
       p2 = dd2;
       d0 = n0;
       d1 = n1;
       dd2 = n2;
+      id = kidx(l + 1, m1, m2);
     }
     double sp, cp;
     sincos(-(m1 * a + m2 * y), &sp, &cp);
@@ -866,7 +985,8 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
     acc[8] -= w * M2 * i1;
     acc[9] -= w * (-M2 * M2 * r0);
   }
-  for (int q = 0; q < 10; ++q) out.v[q] = block_sum(acc[q], red);
+  block_sum_n<10>(acc, red);
+  for (int q = 0; q < 10; ++q) out.v[q] = acc[q];
 }
 
 FO_DEV bool chol3_solve(const double H[6], double lam, const double g[3], double x[3]) {
@@ -888,12 +1008,16 @@ FO_DEV bool chol3_solve(const double H[6], double lam, const double g[3], double
   return true;
 }
 
+// Converged when the gradient or the Newton model decrease -g.dx/2 is at
+// rounding level of f; the line search halves at most 30 times.
 __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ pd,
                                                     const double2* __restrict__ IA,
                                                     const double2* __restrict__ IB, int mode,
                                                     const SphPeak* __restrict__ peaks, int norient,
+                                                    const double* __restrict__ rec,
+                                                    const double* __restrict__ seedu, int Lt,
                                                     SphOpt* __restrict__ opt) {
-  __shared__ double red[16];
+  __shared__ double red[16 * 10];
   const int p = blockIdx.x, o = blockIdx.y;
   const SphPair d = pd[p];
   const int L = d.L;
@@ -902,13 +1026,13 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
   const SphPeak pk = peaks[(long long)p * norient + o];
   double x[3] = {pk.R0[0], pk.R0[1], pk.R0[2]};
   FGH cur;
-  eval_fgh(I, sgn, L, x[0], x[1], x[2], cur, red);
+  eval_fgh(I, sgn, L, x[0], x[1], x[2], rec, seedu, Lt, cur, red);
   int it = 0;
-  const double tol = 1e-12 * (fabs(cur.v[0]) + 1.0) * (L + 1);
   for (; it < 100; ++it) {
+    const double fscale = fabs(cur.v[0]) + 1e-300;
     const double g[3] = {cur.v[1], cur.v[2], cur.v[3]};
     const double gn = fmax(fabs(g[0]), fmax(fabs(g[1]), fabs(g[2])));
-    if (gn < tol) break;
+    if (gn <= 1e-13 * fscale) break;
     // Newton direction with Levenberg damping until positive definite
     const double H[6] = {cur.v[4], cur.v[5], cur.v[6], cur.v[7], cur.v[8], cur.v[9]};
     const double hs = fmax(fabs(H[0]), fmax(fabs(H[3]), fmax(fabs(H[5]), 1e-30)));
@@ -925,11 +1049,13 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
       dx[2] = -g[2] / hs;
     }
     const double slope = g[0] * dx[0] + g[1] * dx[1] + g[2] * dx[2];
+    if (-0.5 * slope <= 4e-16 * fscale) break;  // predicted decrease below rounding
     double step = 1.0;
     FGH nxt;
     bool acc = false;
-    for (int ls = 0; ls < 40; ++ls) {
-      eval_fgh(I, sgn, L, x[0] + step * dx[0], x[1] + step * dx[1], x[2] + step * dx[2], nxt, red);
+    for (int ls = 0; ls < 30; ++ls) {
+      eval_fgh(I, sgn, L, x[0] + step * dx[0], x[1] + step * dx[1], x[2] + step * dx[2], rec,
+               seedu, Lt, nxt, red);
       if (nxt.v[0] <= cur.v[0] + 1e-4 * step * slope) {
         acc = true;
         break;
@@ -938,12 +1064,7 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
     }
     if (!acc) break;
     for (int q = 0; q < 3; ++q) x[q] += step * dx[q];
-    const double sn = step * fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
     cur = nxt;
-    if (sn < 1e-15) {
-      ++it;
-      break;
-    }
   }
   if (threadIdx.x == 0) {
     SphOpt r;
@@ -1108,17 +1229,33 @@ void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, cons
                      scale_eff, I64, I32);
 }
 
-size_t isoft_plane_smem(int n, int Lmax) {
+static int log2_exact(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return (1 << l) == n ? l : -1;
+}
+
+size_t isoft_plane_smem(int n, int Lmax, int chunk) {
   return sizeof(float2) * ((size_t)n + 2 * (size_t)(Lmax + 1) * (2 * Lmax + 1) +
-                           2 * (size_t)(Lmax + 1) * n) +
+                           2 * (size_t)(Lmax + 1) * n + 2 * (size_t)chunk * n) +
          16 * sizeof(float) + 16 * sizeof(int);
 }
 
-void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, long long Kt, const SphPair* pd,
-                        const float* tab32, const float2* tw32, const float2* IA,
-                        const float2* IB, int mode, int norient, PlaneCand* cand, float* grid) {
-  hipLaunchKernelGGL(k_isoft_plane, dim3(n, P), dim3(256), isoft_plane_smem(n, Lmax), s, pd, n,
-                     Kt, tab32, tw32, IA, IB, mode, norient, cand, grid);
+// FFT chunk (rows per pass) that fits max_lds; 0 selects the direct DFT.
+int isoft_plane_chunk(int n, int Lmax, size_t max_lds) {
+  if (log2_exact(n) < 1) return 0;
+  for (int c = 64; c >= 1; c >>= 1)
+    if (isoft_plane_smem(n, Lmax, c) <= max_lds) return c;
+  return 0;
+}
+
+void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int chunk, long long Kt,
+                        const SphPair* pd, const float* tab32, const float2* tw32,
+                        const float2* IA, const float2* IB, int mode, int norient,
+                        PlaneCand* cand, float* grid) {
+  const int logn = chunk > 0 ? log2_exact(n) : -1;
+  hipLaunchKernelGGL(k_isoft_plane, dim3(n, P), dim3(256), isoft_plane_smem(n, Lmax, chunk), s,
+                     pd, n, logn, chunk, Kt, tab32, tw32, IA, IB, mode, norient, cand, grid);
 }
 
 void launch_sph_select(hipStream_t s, int P, int n, long long Kt, int norient, const SphPair* pd,
@@ -1130,9 +1267,10 @@ void launch_sph_select(hipStream_t s, int P, int n, long long Kt, int norient, c
 }
 
 void launch_sph_rotopt(hipStream_t s, int P, int norient, const SphPair* pd, const double2* IA,
-                       const double2* IB, int mode, const SphPeak* peaks, SphOpt* opt) {
+                       const double2* IB, int mode, const SphPeak* peaks, const double* rec,
+                       const double* seedu, int Lt, SphOpt* opt) {
   hipLaunchKernelGGL(k_sph_rotopt, dim3(P, norient), dim3(256), 0, s, pd, IA, IB, mode, peaks,
-                     norient, opt);
+                     norient, rec, seedu, Lt, opt);
 }
 
 void launch_sph_finish(hipStream_t s, int P, int lap_nmax, const SphPair* pd, const SphGroup* grp,
@@ -1161,10 +1299,12 @@ __global__ void k_c64_to_c32(long long n, const double2* __restrict__ a, float2*
 // coefficient set given as conj(X) in compact layout.
 __global__ void __launch_bounds__(256) k_sph_energy(int L, const double2* __restrict__ Iconj,
                                                     double a, double b, double y,
+                                                    const double* __restrict__ rec,
+                                                    const double* __restrict__ seedu, int Lt,
                                                     double* __restrict__ out) {
-  __shared__ double red[16];
+  __shared__ double red[16 * 10];
   FGH r;
-  eval_fgh(Iconj, 1.0, L, a, b, y, r, red);
+  eval_fgh(Iconj, 1.0, L, a, b, y, rec, seedu, Lt, r, red);
   if (threadIdx.x == 0)
     for (int q = 0; q < 10; ++q) out[q] = r.v[q];
 }
@@ -1269,8 +1409,14 @@ void launch_c64_to_c32(hipStream_t s, long long n, const double2* a, float2* b) 
   hipLaunchKernelGGL(k_c64_to_c32, dim3(blocks), dim3(256), 0, s, n, a, b);
 }
 void launch_sph_energy(hipStream_t s, int L, const double2* Iconj, double a, double b, double y,
-                       double* out) {
-  hipLaunchKernelGGL(k_sph_energy, dim3(1), dim3(256), 0, s, L, Iconj, a, b, y, out);
+                       const double* rec, const double* seedu, int Lt, double* out) {
+  hipLaunchKernelGGL(k_sph_energy, dim3(1), dim3(256), 0, s, L, Iconj, a, b, y, rec, seedu, Lt,
+                     out);
+}
+
+void launch_rec_table(hipStream_t s, int Lt, double* rec, double* seedu) {
+  const int W = 2 * Lt + 1;
+  hipLaunchKernelGGL(k_rec_table, dim3((W * W + 127) / 128), dim3(128), 0, s, Lt, rec, seedu);
 }
 void launch_wigner_D(hipStream_t s, int L, double a, double b, double y, double2* D, double2* G) {
   const int W = 2 * L + 1;

@@ -69,6 +69,7 @@ int fo_context_memory(fo_context* ctx, unsigned long long* used, unsigned long l
 #define FO_SPH_KEEP_GRID 2     /* keep the FP32 overlap grid(s) */
 #define FO_SPH_NO_CENTER 4     /* positions are used as given */
 #define FO_SPH_GENERAL 8       /* two independent Hermitian coefficient sets */
+#define FO_SPH_DFT 16          /* direct DFT planes even when grid_n is a power of two */
 
 /* stages for fo_sph_batch_run (bit mask) */
 #define FO_STAGE_PREP 1        /* centroids, kernel width */

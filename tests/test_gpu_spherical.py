@@ -111,6 +111,22 @@ def test_c0_single_pair_parity(gpu):
     assert r["dist"][0] < 0.5
 
 
+def test_fft_and_dft_planes_agree(gpu):
+    c = W.config2(7)
+    n = c["grid_n"]
+    say(f"case: C2 pair 7 n={n} L={c['L']}: FFT planes vs direct-DFT planes")
+    o = _oracle_pair(c, keep_grids=True)
+    out = {}
+    for name, flag in (("fft", 0), ("dft", _lib.SPH_DFT)):
+        with SphericalBatch([len(c["pos1"])], [c["L"]], 0.3, n, [c["groups"]],
+                            _lib.SPH_KEEP_GRID | flag) as b:
+            b.set_positions(c["pos1"], c["pos2"])
+            b.run()
+            out[name] = (b.results(), b.peaks(), b.grid(0))
+    for name, (r, pk, g) in out.items():
+        _check_pair(name, o, r["dist"][0], r["perm"][0], r["inverted"][0], pk["ind"][0], g)
+
+
 def test_lj38_known_answer_dropin(gpu, lj38):
     say("case: LJ38 SphericalAlign(0.3, Jmax=14) drop-in (n=30, generic DFT path)")
     p1, p2 = lj38
