@@ -300,12 +300,18 @@ __global__ void k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __r
 // ---------------------------------------------------------------------------
 // Contraction I^l_{m1m2} = 4 pi^{5/2} s^3 sum_s sum_{j,k in s}
 //   Y1[l,m1,j] K_l[j,k] conj(Y2[l,m2,k])      (sphericalAlignment.py:273,:175)
-// One workgroup per (pair, l, block of kMB rows m1); per k-tile the
-// intermediate A = Y1 . K_l is formed in LDS, then accumulated against Y2^H.
+// One workgroup (256 threads) per (pair, l, block of kMB rows m1), an LDS
+// tiled FP64 GEMM chain.  Per tile of kKT atoms k of structure 2:
+//   A[r][k]  = sum_j Y1[m1a+r][j] K_l[j][k]   (j in tiles of kJT; 2x2
+//              complex-by-real register tile per thread, A -> LDS)
+//   acc[r][m2] += sum_k A[r][k] conj(Y2[m2][k])  (4x4 complex register tile;
+//              Y2 staged transposed so lanes read consecutive m2)
+// Each output is reduced in the same order for any batch, so results do
+// not depend on the batch a pair is in.
 // ---------------------------------------------------------------------------
-constexpr int kMB = 16;   // m1 rows per workgroup
+constexpr int kMB = 32;   // m1 rows per workgroup
 constexpr int kKT = 32;   // atoms of structure 2 per tile
-constexpr int kAccMax = 8;  // kMB * (2L+1) / 256 <= 8  ->  L <= 63
+constexpr int kJT = 32;   // atoms of structure 1 per step
 
 __global__ void __launch_bounds__(256) k_sph_contract(
     const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
@@ -326,68 +332,99 @@ __global__ void __launch_bounds__(256) k_sph_contract(
   const int M = 2 * l + 1;
   const int m1a = b * kMB;  // row offset within [0, M)
   const int rows = min(kMB, M - m1a);
-  double2* As = (double2*)smem;          // [kMB][kKT]
-  double2* Ys = As + kMB * kKT;          // [M][kKT]
+  double2* Y1s = (double2*)smem;                 // [kMB][kJT]
+  double* Ks = (double*)(Y1s + kMB * kJT);       // [kJT][kKT]
+  double2* As = (double2*)(Ks + kJT * kKT);      // [kMB][kKT]
+  double2* Y2t = As + kMB * kKT;                 // [kKT][M] (transposed)
   const int t = threadIdx.x;
-  double2 acc[kAccMax];
-  for (int q = 0; q < kAccMax; ++q) acc[q] = make_double2(0.0, 0.0);
+  // phase A thread tile: rows {ar, ar+16}, cols {2ac, 2ac+1}
+  const int ar = t >> 4, ac = t & 15;
+  // phase B thread tile: rows {br + 8i}, cols {bc + 32q}, i, q < 4
+  const int br = t >> 5, bc = t & 31;
+  double2 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[i][q] = make_double2(0.0, 0.0);
   const int Ng = d.ngath;
   const double2* y1 = Y1 + d.ylm_off + (long long)(l * l) * Ng;
   const double2* y2 = Y2 + d.ylm_off + (long long)(l * l) * Ng;
-  for (int q = 0; q < d.ngroups; ++q) {
-    const SphGroup G = grp[d.group_off + q];
+  for (int g = 0; g < d.ngroups; ++g) {
+    const SphGroup G = grp[d.group_off + g];
     const int Ns = G.size;
     const double* Rl = R + d.rad_off + G.rad_off + (long long)l * Ns * Ns;
     for (int k0 = 0; k0 < Ns; k0 += kKT) {
       const int kt = min(kKT, Ns - k0);
-      for (int e = t; e < M * kKT; e += blockDim.x) {
-        const int m2 = e / kKT, kk = e % kKT;
-        Ys[e] = kk < kt ? y2[(long long)m2 * Ng + G.gath_off + k0 + kk] : make_double2(0.0, 0.0);
+      __syncthreads();  // previous tile's readers of Y2t / As are done
+      for (int e = t; e < kKT * M; e += blockDim.x) {
+        const int kk = e / M, m2 = e % M;
+        Y2t[e] = kk < kt ? y2[(long long)m2 * Ng + G.gath_off + k0 + kk] : make_double2(0.0, 0.0);
       }
-      for (int e = t; e < kMB * kKT; e += blockDim.x) {
-        const int r = e / kKT, kk = e % kKT;
-        double2 a = make_double2(0.0, 0.0);
-        if (r < rows && kk < kt) {
-          const double2* yr = y1 + (long long)(m1a + r) * Ng + G.gath_off;
-          const double* rc = Rl + k0 + kk;
-          for (int j = 0; j < Ns; ++j) {
-            const double kv = rc[(long long)j * Ns];
-            const double2 yv = yr[j];
-            a.x += yv.x * kv;
-            a.y += yv.y * kv;
+      double2 a00 = make_double2(0.0, 0.0), a01 = a00, a10 = a00, a11 = a00;
+      for (int j0 = 0; j0 < Ns; j0 += kJT) {
+        const int jt = min(kJT, Ns - j0);
+        __syncthreads();
+        for (int e = t; e < kMB * kJT; e += blockDim.x) {
+          const int r = e / kJT, jj = e % kJT;
+          Y1s[e] = (r < rows && jj < jt) ? y1[(long long)(m1a + r) * Ng + G.gath_off + j0 + jj]
+                                         : make_double2(0.0, 0.0);
+        }
+        for (int e = t; e < kJT * kKT; e += blockDim.x) {
+          const int jj = e / kKT, kk = e % kKT;
+          Ks[e] = (jj < jt && kk < kt) ? Rl[(long long)(j0 + jj) * Ns + k0 + kk] : 0.0;
+        }
+        __syncthreads();
+        for (int jj = 0; jj < jt; ++jj) {
+          const double2 y0 = Y1s[ar * kJT + jj], yb = Y1s[(ar + 16) * kJT + jj];
+          const double k0v = Ks[jj * kKT + 2 * ac], k1v = Ks[jj * kKT + 2 * ac + 1];
+          a00.x += y0.x * k0v;
+          a00.y += y0.y * k0v;
+          a01.x += y0.x * k1v;
+          a01.y += y0.y * k1v;
+          a10.x += yb.x * k0v;
+          a10.y += yb.y * k0v;
+          a11.x += yb.x * k1v;
+          a11.y += yb.y * k1v;
+        }
+      }
+      As[ar * kKT + 2 * ac] = a00;
+      As[ar * kKT + 2 * ac + 1] = a01;
+      As[(ar + 16) * kKT + 2 * ac] = a10;
+      As[(ar + 16) * kKT + 2 * ac + 1] = a11;
+      __syncthreads();
+      for (int kk = 0; kk < kt; ++kk) {
+        double2 av[4], yv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) av[i] = As[(br + 8 * i) * kKT + kk];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int m2 = bc + 32 * q;
+          yv[q] = m2 < M ? Y2t[kk * M + m2] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            // a * conj(y)
+            acc[i][q].x += av[i].x * yv[q].x + av[i].y * yv[q].y;
+            acc[i][q].y += av[i].y * yv[q].x - av[i].x * yv[q].y;
           }
-        }
-        As[e] = a;
       }
-      __syncthreads();
-      for (int q2 = 0; q2 < kAccMax; ++q2) {
-        const int e = t + q2 * blockDim.x;
-        if (e >= rows * M) break;
-        const int r = e / M, m2 = e % M;
-        double2 s = acc[q2];
-        for (int kk = 0; kk < kt; ++kk) {
-          const double2 a = As[r * kKT + kk];
-          const double2 y = Ys[m2 * kKT + kk];
-          // a * conj(y)
-          s.x += a.x * y.x + a.y * y.y;
-          s.y += a.y * y.x - a.x * y.y;
-        }
-        acc[q2] = s;
-      }
-      __syncthreads();
     }
   }
   const double sc = scale_eff[blockIdx.y];
   const double norm = 4.0 * pow(kPi, 2.5) * sc * sc * sc;
-  for (int q2 = 0; q2 < kAccMax; ++q2) {
-    const int e = t + q2 * blockDim.x;
-    if (e >= rows * M) break;
-    const int r = e / M, m2i = e % M;
-    const long long id = d.coef_off + kidx(l, m1a + r - l, m2i - l);
-    const double2 v = cscale(acc[q2], norm);
-    I64[id] = v;
-    I32[id] = make_float2((float)v.x, (float)v.y);
-  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = br + 8 * i, m2i = bc + 32 * q;
+      if (r >= rows || m2i >= M) continue;
+      const long long id = d.coef_off + kidx(l, m1a + r - l, m2i - l);
+      const double2 v = cscale(acc[i][q], norm);
+      I64[id] = v;
+      I32[id] = make_float2((float)v.x, (float)v.y);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1224,7 +1261,9 @@ void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, cons
                          const double* scale_eff, double2* I64, float2* I32) {
   int nb = 0;
   for (int l = 0; l <= Lmax; ++l) nb += (2 * l + 1 + kMB - 1) / kMB;
-  const size_t smem = sizeof(double2) * (size_t)(kMB + 2 * Lmax + 1) * kKT;
+  const size_t smem = sizeof(double2) * ((size_t)kMB * kJT + (size_t)kMB * kKT +
+                                         (size_t)kKT * (2 * Lmax + 1)) +
+                      sizeof(double) * (size_t)kJT * kKT;
   hipLaunchKernelGGL(k_sph_contract, dim3(nb, P), dim3(256), smem, s, pd, grp, Y1, Y2, R,
                      scale_eff, I64, I32);
 }
