@@ -940,7 +940,8 @@ __global__ void k_rec_table(int Lt, double* __restrict__ rec, double* __restrict
       double A = 0.0, B = 0.0, C = 0.0;
       if (l < Lt) recurrence_terms(l, m1, m2, A, B, C);
       double* r = rec + 3 * kidx(l, m1, m2);
-      r[0] = A * sqrt((2.0 * l - 1.0) / (2.0 * l + 3.0));
+      // A_0 = 0 (no d^{-1} term); the square root is undefined at l = 0
+      r[0] = l > 0 ? A * sqrt((2.0 * l - 1.0) / (2.0 * l + 3.0)) : 0.0;
       r[1] = B * sqrt((2.0 * l + 1.0) / (2.0 * l + 3.0));
       r[2] = C;
     }
