@@ -34,7 +34,7 @@ void launch_sph_radial(hipStream_t, int, long long, const SphPair*, const SphGro
 void launch_sph_contract(hipStream_t, int, int, const SphPair*, const SphGroup*, const double2*,
                          const double2*, const double*, const double*, double2*, float2*);
 size_t isoft_plane_smem(int, int, int);
-int isoft_plane_chunk(int, int, size_t);
+int isoft_plane_use_fft(int, int, size_t);
 void launch_isoft_plane(hipStream_t, int, int, int, int, long long, const SphPair*, const float*,
                         const float2*, const float2*, const float2*, int, int, PlaneCand*, float*);
 void launch_sph_select(hipStream_t, int, int, long long, int, const SphPair*, const double*,
@@ -323,7 +323,7 @@ struct fo_sph_batch {
   SphLaunch lc{};
   int flags = 0;
   int maxgroup = 1;
-  int fft_chunk = 0;  // rows per FFT pass of the plane kernel (0: direct DFT)
+  int fft_chunk = 0;  // 1: two-pass register transform in the plane kernel, 0: direct DFT
   long long Ntot = 0, Ktot = 0;
   std::vector<SphPair> hp;
   std::vector<SphGroup> hg;
@@ -490,7 +490,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
     return set_err(FO_ERR_UNSUPPORTED, "grid_n and Jmax too large for one beta plane in LDS (" +
                                            std::to_string(isoft_plane_smem(n, Lmax, 0)) + " > " +
                                            std::to_string(c->max_lds) + " bytes)");
-  b->fft_chunk = (d->flags & FO_SPH_DFT) ? 0 : isoft_plane_chunk(n, Lmax, c->max_lds);
+  b->fft_chunk = (d->flags & FO_SPH_DFT) ? 0 : isoft_plane_use_fft(n, Lmax, c->max_lds);
   if (lap_lds_bytes(b->maxgroup) > c->max_lds)
     return set_err(FO_ERR_UNSUPPORTED, "permutation group too large for the LDS assignment solver");
   int rc = get_sph_table(c, n, &b->tab);

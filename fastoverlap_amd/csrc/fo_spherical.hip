@@ -313,22 +313,12 @@ constexpr int kMB = 32;   // m1 rows per workgroup
 constexpr int kKT = 32;   // atoms of structure 2 per tile
 constexpr int kJT = 32;   // atoms of structure 1 per step
 
-__global__ void __launch_bounds__(256) k_sph_contract(
-    const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
-    const double2* __restrict__ Y1, const double2* __restrict__ Y2,
-    const double* __restrict__ R, const double* __restrict__ scale_eff,
-    double2* __restrict__ I64, float2* __restrict__ I32) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const SphPair d = pd[blockIdx.y];
-  const int L = d.L;
-  // decode (l, m1 block) from blockIdx.x
-  int b = blockIdx.x, l = 0;
-  for (; l <= L; ++l) {
-    const int nb = (2 * l + 1 + kMB - 1) / kMB;
-    if (b < nb) break;
-    b -= nb;
-  }
-  if (l > L) return;
+// NQ = number of 32-wide m2 column groups of the second GEMM (M <= 32 NQ).
+template <int NQ>
+FO_DEV void contract_body(const SphPair& d, int l, int b, const SphGroup* __restrict__ grp,
+                          const double2* __restrict__ Y1, const double2* __restrict__ Y2,
+                          const double* __restrict__ R, double sc, double2* __restrict__ I64,
+                          float2* __restrict__ I32, unsigned char* smem) {
   const int M = 2 * l + 1;
   const int m1a = b * kMB;  // row offset within [0, M)
   const int rows = min(kMB, M - m1a);
@@ -339,13 +329,13 @@ __global__ void __launch_bounds__(256) k_sph_contract(
   const int t = threadIdx.x;
   // phase A thread tile: rows {ar, ar+16}, cols {2ac, 2ac+1}
   const int ar = t >> 4, ac = t & 15;
-  // phase B thread tile: rows {br + 8i}, cols {bc + 32q}, i, q < 4
+  // phase B thread tile: rows {br + 8i}, cols {bc + 32q}, i < 4, q < NQ
   const int br = t >> 5, bc = t & 31;
-  double2 acc[4][4];
+  double2 acc[4][NQ];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[i][q] = make_double2(0.0, 0.0);
+    for (int q = 0; q < NQ; ++q) acc[i][q] = make_double2(0.0, 0.0);
   const int Ng = d.ngath;
   const double2* y1 = Y1 + d.ylm_off + (long long)(l * l) * Ng;
   const double2* y2 = Y2 + d.ylm_off + (long long)(l * l) * Ng;
@@ -393,18 +383,18 @@ __global__ void __launch_bounds__(256) k_sph_contract(
       As[(ar + 16) * kKT + 2 * ac + 1] = a11;
       __syncthreads();
       for (int kk = 0; kk < kt; ++kk) {
-        double2 av[4], yv[4];
+        double2 av[4], yv[NQ];
 #pragma unroll
         for (int i = 0; i < 4; ++i) av[i] = As[(br + 8 * i) * kKT + kk];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NQ; ++q) {
           const int m2 = bc + 32 * q;
           yv[q] = m2 < M ? Y2t[kk * M + m2] : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < NQ; ++q) {
             // a * conj(y)
             acc[i][q].x += av[i].x * yv[q].x + av[i].y * yv[q].y;
             acc[i][q].y += av[i].y * yv[q].x - av[i].x * yv[q].y;
@@ -412,12 +402,11 @@ __global__ void __launch_bounds__(256) k_sph_contract(
       }
     }
   }
-  const double sc = scale_eff[blockIdx.y];
   const double norm = 4.0 * pow(kPi, 2.5) * sc * sc * sc;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const int r = br + 8 * i, m2i = bc + 32 * q;
       if (r >= rows || m2i >= M) continue;
       const long long id = d.coef_off + kidx(l, m1a + r - l, m2i - l);
@@ -425,6 +414,32 @@ __global__ void __launch_bounds__(256) k_sph_contract(
       I64[id] = v;
       I32[id] = make_float2((float)v.x, (float)v.y);
     }
+}
+
+__global__ void __launch_bounds__(256) k_sph_contract(
+    const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
+    const double2* __restrict__ Y1, const double2* __restrict__ Y2,
+    const double* __restrict__ R, const double* __restrict__ scale_eff,
+    double2* __restrict__ I64, float2* __restrict__ I32) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const SphPair d = pd[blockIdx.y];
+  const int L = d.L;
+  // decode (l, m1 block) from blockIdx.x
+  int b = blockIdx.x, l = 0;
+  for (; l <= L; ++l) {
+    const int nb = (2 * l + 1 + kMB - 1) / kMB;
+    if (b < nb) break;
+    b -= nb;
+  }
+  if (l > L) return;
+  const double sc = scale_eff[blockIdx.y];
+  const int M = 2 * l + 1;
+  if (M <= 32)
+    contract_body<1>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, smem);
+  else if (M <= 64)
+    contract_body<2>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, smem);
+  else
+    contract_body<4>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -439,52 +454,104 @@ __global__ void __launch_bounds__(256) k_sph_contract(
 // Each plane keeps its kCandPerPlane largest values and an upper bound on
 // the rest, for the FP64 selection kernel.
 // ---------------------------------------------------------------------------
-// Inverse radix-2 Stockham FFTs (e^{+2 pi i jk/n}, unnormalised) of `rows`
-// rows of length n = 2^logn held in a[rows][n]; b is scratch of the same
-// size.  Returns the buffer holding the result (natural order).
-FO_DEV float2* fft_rows_inv(float2* a, float2* b, int rows, int n, int logn,
-                            const float2* __restrict__ tw) {
-  const int half = n >> 1;
-  for (int st = 0; st < logn; ++st) {
-    const int Ns = 1 << st;
-    const int tws = n >> (st + 1);
-    for (int e = threadIdx.x; e < rows * half; e += blockDim.x) {
-      const int r = e >> (logn - 1);
-      const int j = e & (half - 1);
-      const float2* in = a + r * n;
-      float2* out = b + r * n;
-      const int k = j & (Ns - 1);
-      const float2 v0 = in[j];
-      const float2 v1 = cmul(in[j + half], tw[k * tws]);
-      const int o = ((j - k) << 1) + k;
-      out[o] = make_float2(v0.x + v1.x, v0.y + v1.y);
-      out[o + Ns] = make_float2(v0.x - v1.x, v0.y - v1.y);
-    }
-    __syncthreads();
-    float2* tmp = a;
-    a = b;
-    b = tmp;
+// Inverse DFT of R = 2, 4, 8 or 16 values in registers (radix-2
+// decimation in time, natural order in and out), twiddles e^{+2 pi i k/16}.
+template <int R>
+FO_DEV constexpr int bitrev(int i) {
+  int r = 0;
+  for (int b = 1; b < R; b <<= 1) {
+    r = (r << 1) | (i & 1);
+    i >>= 1;
   }
-  return a;
+  return r;
 }
 
-// logn >= 0: n = 2^logn, the gamma and alpha passes are FFTs processed in
-// chunks of `chunk` rows / columns through the workspace WS[2][chunk][n];
-// logn < 0: direct DFT passes (any n).
+template <int R>
+FO_DEV void dft_reg(float2 (&v)[R]) {
+  constexpr float C16[16] = {1.0f,
+                             0.92387953251128674f,
+                             0.70710678118654752f,
+                             0.38268343236508977f,
+                             0.0f,
+                             -0.38268343236508977f,
+                             -0.70710678118654752f,
+                             -0.92387953251128674f,
+                             -1.0f,
+                             -0.92387953251128674f,
+                             -0.70710678118654752f,
+                             -0.38268343236508977f,
+                             0.0f,
+                             0.38268343236508977f,
+                             0.70710678118654752f,
+                             0.92387953251128674f};
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int j = bitrev<R>(i);
+    if (j > i) {
+      const float2 tmp = v[i];
+      v[i] = v[j];
+      v[j] = tmp;
+    }
+  }
+#pragma unroll
+  for (int len = 2; len <= R; len <<= 1) {
+#pragma unroll
+    for (int i = 0; i < R; i += len) {
+#pragma unroll
+      for (int k = 0; k < len / 2; ++k) {
+        const int ti = k * (16 / len);
+        // e^{+2 pi i ti/16}: cos = C16[ti], sin = C16[(ti + 12) % 16]
+        const float2 w = make_float2(C16[ti], C16[(ti + 12) & 15]);
+        const float2 u = v[i + k];
+        const float2 tt = cmul(v[i + k + len / 2], w);
+        v[i + k] = make_float2(u.x + tt.x, u.y + tt.y);
+        v[i + k + len / 2] = make_float2(u.x - tt.x, u.y - tt.y);
+      }
+    }
+  }
+}
+
+// Per-thread record of the largest grid value and a bound on the others.
+struct CandTrack {
+  float bv[2], dm[2];
+  int bi[2];
+  FO_DEV CandTrack() {
+    bv[0] = bv[1] = dm[0] = dm[1] = -INFINITY;
+    bi[0] = bi[1] = 0x7fffffff;
+  }
+  FO_DEV void add(int o, int flat, float val) {
+    if (val > bv[o]) {
+      dm[o] = fmaxf(dm[o], bv[o]);
+      bv[o] = val;
+      bi[o] = flat;
+    } else {
+      dm[o] = fmaxf(dm[o], val);
+    }
+  }
+};
+
+// N1 > 0: n = N1 N2 (powers of two <= 16); each length-n inverse DFT is two
+// passes of register DFTs (N1 of length N2, twiddle e^{2 pi i j1 k2/n},
+// N2 of length N1) with one LDS exchange; groups of TPF = max(N1, N2)
+// threads transform one row / column.
+// N1 == 0: direct DFT passes (any n).
+template <int N1, int N2>
 __global__ void __launch_bounds__(256) k_isoft_plane(
-    const SphPair* __restrict__ pd, int n, int logn, int chunk, long long Kt,
-    const float* __restrict__ tab32, const float2* __restrict__ twg,
-    const float2* __restrict__ IA, const float2* __restrict__ IB, int mode, int norient,
-    PlaneCand* __restrict__ cand, float* __restrict__ grid) {
+    const SphPair* __restrict__ pd, int n, long long Kt, const float* __restrict__ tab32,
+    const float2* __restrict__ twg, const float2* __restrict__ IA,
+    const float2* __restrict__ IB, int mode, int norient, PlaneCand* __restrict__ cand,
+    float* __restrict__ grid) {
   extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int TPF = N1 > N2 ? N1 : N2;
+  constexpr int RR = N1 > 0 ? 256 / TPF : 0;   // rows / columns per round
   const int ib = blockIdx.x, p = blockIdx.y;
   const SphPair d = pd[p];
   const int L = d.L, W = 2 * L + 1, t = threadIdx.x, bd = blockDim.x;
   float2* tw = (float2*)smem;                 // [n]
   float2* S = tw + n;                          // [2][L+1][W]
   float2* T = S + 2 * (L + 1) * W;             // [2][L+1][n]
-  float2* WS = T + 2 * (L + 1) * n;            // [2][chunk][n] (FFT path)
-  float* rv = (float*)(WS + (logn >= 0 ? 2 * chunk * n : 0));  // reduction scratch
+  float2* Yb = T + 2 * (L + 1) * n;            // [RR][n] (two-pass path)
+  float* rv = (float*)(Yb + RR * n);           // reduction scratch
   int* ri = (int*)(rv + 16);
   for (int k = t; k < n; k += bd) tw[k] = twg[k];
   const float* tb = tab32 + (long long)ib * Kt;
@@ -525,69 +592,94 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
     }
   }
   __syncthreads();
-  float bv[2] = {-INFINITY, -INFINITY}, dm[2] = {-INFINITY, -INFINITY};
-  int bi[2] = {0x7fffffff, 0x7fffffff};
+  CandTrack ct;
   auto consider = [&](int o, int a, int g, float val) {
     const int flat = (a * n + ib) * n + g;
     if (grid) grid[((long long)p * norient + o) * n * n * n + flat] = val;
-    if (val > bv[o]) {
-      dm[o] = fmaxf(dm[o], bv[o]);
-      bv[o] = val;
-      bi[o] = flat;
-    } else {
-      dm[o] = fmaxf(dm[o], val);
-    }
+    ct.add(o, flat, val);
   };
-  if (logn >= 0) {
-    float2* wa = WS;
-    float2* wb = WS + chunk * n;
-    // gamma pass: rows (o, m1), input S[o][m1][m2] at index m2 mod n
-    const int nrows = norient * (L + 1);
-    for (int r0 = 0; r0 < nrows; r0 += chunk) {
-      const int rows = min(chunk, nrows - r0);
-      for (int e = t; e < rows * n; e += bd) {
-        const int r = e / n, k = e % n;
-        const int m2 = k <= L ? k : k - n;
-        wa[e] = (m2 >= -L) ? S[(r0 + r) * W + m2 + L] : make_float2(0.f, 0.f);
-      }
-      __syncthreads();
-      const float2* res = fft_rows_inv(wa, wb, rows, n, logn, tw);
-      for (int e = t; e < rows * n; e += bd) T[r0 * n + e] = res[e];
-      __syncthreads();
+  // column g of the alpha pass: X[m1] = T_A[m1][g] + i T_B[m1][g] for m1 >= 0
+  // and the Hermitian extension for m1 < 0; Re / Im of its inverse
+  // transform are the two real grids.
+  const float2* TA = T;
+  const float2* TB = T + (L + 1) * n;
+  auto column_value = [&](int g, int k) {
+    const int m1 = k <= L ? k : k - n;
+    float2 v = make_float2(0.f, 0.f);
+    if (m1 == 0) {
+      // T[0][g] is real for a Hermitian set: keep the real parts only
+      v = make_float2(TA[g].x, norient == 2 ? TB[g].x : 0.f);
+    } else if (m1 > 0) {
+      const float2 A = TA[m1 * n + g];
+      const float2 B = norient == 2 ? TB[m1 * n + g] : make_float2(0.f, 0.f);
+      v = make_float2(A.x - B.y, A.y + B.x);
+    } else if (m1 >= -L) {
+      const float2 A = TA[-m1 * n + g];
+      const float2 B = norient == 2 ? TB[-m1 * n + g] : make_float2(0.f, 0.f);
+      v = make_float2(A.x + B.y, B.x - A.y);
     }
-    // alpha pass: column g holds X[m1] = T_A[m1][g] + i T_B[m1][g] for
-    // m1 >= 0 and its Hermitian extension for m1 < 0; Re / Im of the inverse
-    // transform are the two real grids.
-    const float2* TA = T;
-    const float2* TB = T + (L + 1) * n;
-    for (int g0 = 0; g0 < n; g0 += chunk) {
-      const int cols = min(chunk, n - g0);
-      for (int e = t; e < cols * n; e += bd) {
-        const int c = e / n, k = e % n;
-        const int g = g0 + c;
-        const int m1 = k <= L ? k : k - n;
-        float2 v = make_float2(0.f, 0.f);
-        if (m1 == 0) {
-          // T[0][g] is real for a Hermitian set: keep the real parts only
-          v = make_float2(TA[g].x, norient == 2 ? TB[g].x : 0.f);
-        } else if (m1 > 0) {
-          const float2 A = TA[m1 * n + g];
-          const float2 B = norient == 2 ? TB[m1 * n + g] : make_float2(0.f, 0.f);
-          v = make_float2(A.x - B.y, A.y + B.x);
-        } else if (m1 >= -L) {
-          const float2 A = TA[-m1 * n + g];
-          const float2 B = norient == 2 ? TB[-m1 * n + g] : make_float2(0.f, 0.f);
-          v = make_float2(A.x + B.y, B.x - A.y);
+    return v;
+  };
+  if constexpr (N1 > 0) {
+    const int grp = t / TPF, lane = t % TPF;
+    // gamma pass: row (o, m1) holds S[o][m1][m2] at index m2 mod n
+    const int nrows = norient * (L + 1);
+    for (int r0 = 0; r0 < nrows; r0 += RR) {
+      const int r = r0 + grp;
+      float2* Trow = T + r * n;
+      if (r < nrows && lane < N1) {
+        const int j1 = lane;
+        float2 v[N2];
+#pragma unroll
+        for (int j2 = 0; j2 < N2; ++j2) {
+          const int k = j1 + N1 * j2;
+          const int m2 = k <= L ? k : k - n;
+          v[j2] = (m2 >= -L) ? S[r * W + m2 + L] : make_float2(0.f, 0.f);
         }
-        wa[e] = v;
+        dft_reg<N2>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) Trow[j1 + N1 * k2] = cmul(v[k2], tw[(j1 * k2) % n]);
       }
       __syncthreads();
-      const float2* res = fft_rows_inv(wa, wb, cols, n, logn, tw);
-      for (int e = t; e < cols * n; e += bd) {
-        const int c = e / n, a = e % n;
-        const float2 v = res[e];
-        consider(0, a, g0 + c, v.x);
-        if (norient == 2) consider(1, a, g0 + c, v.y);
+      float2 u[N1];
+      const bool act = r < nrows && lane < N2;
+      if (act) {
+#pragma unroll
+        for (int j1 = 0; j1 < N1; ++j1) u[j1] = Trow[j1 + N1 * lane];
+        dft_reg<N1>(u);
+      }
+      __syncthreads();
+      if (act)
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) Trow[lane + N2 * k1] = u[k1];
+    }
+    __syncthreads();
+    // alpha pass: the second half of each transform goes straight to the
+    // candidate tracker
+    for (int g0 = 0; g0 < n; g0 += RR) {
+      const int g = g0 + grp;
+      float2* Y = Yb + grp * n;
+      if (g < n && lane < N1) {
+        const int j1 = lane;
+        float2 v[N2];
+#pragma unroll
+        for (int j2 = 0; j2 < N2; ++j2) v[j2] = column_value(g, j1 + N1 * j2);
+        dft_reg<N2>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) Y[j1 + N1 * k2] = cmul(v[k2], tw[(j1 * k2) % n]);
+      }
+      __syncthreads();
+      if (g < n && lane < N2) {
+        float2 u[N1];
+#pragma unroll
+        for (int j1 = 0; j1 < N1; ++j1) u[j1] = Y[j1 + N1 * lane];
+        dft_reg<N1>(u);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) {
+          const int a = lane + N2 * k1;
+          consider(0, a, g, u[k1].x);
+          if (norient == 2) consider(1, a, g, u[k1].y);
+        }
       }
       __syncthreads();
     }
@@ -635,8 +727,8 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
     PlaneCand out;
     float fifth = -INFINITY;
     for (int r = 0; r <= kCandPerPlane; ++r) {
-      float v = taken ? -INFINITY : bv[o];
-      int i = taken ? 0x7fffffff : bi[o];
+      float v = taken ? -INFINITY : ct.bv[o];
+      int i = taken ? 0x7fffffff : ct.bi[o];
       block_argmax(v, i, rv, ri);
       if (r < kCandPerPlane) {
         out.val[r] = v;
@@ -644,9 +736,9 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
       } else {
         fifth = v;
       }
-      if (!taken && i == bi[o] && v == bv[o]) taken = true;
+      if (!taken && i == ct.bi[o] && v == ct.bv[o]) taken = true;
     }
-    float dmax = dm[o];
+    float dmax = ct.dm[o];
     int dummy = 0;
     block_argmax(dmax, dummy, rv, ri);
     out.rest = fmaxf(fifth, dmax);
@@ -1046,8 +1138,10 @@ FO_DEV bool chol3_solve(const double H[6], double lam, const double g[3], double
   return true;
 }
 
-// Converged when the gradient or the Newton model decrease -g.dx/2 is at
-// rounding level of f; the line search halves at most 30 times.
+// Converged when the gradient, the Newton model decrease -g.dx/2 or the step
+// reaches the level where rounding noise of f (a sum of ~K(L) terms)
+// dominates: |g| <= 1e-12 |f|, decrease <= 1e-13 |f|, step <= 1e-12 rad.
+// The line search halves at most 8 times.
 __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ pd,
                                                     const double2* __restrict__ IA,
                                                     const double2* __restrict__ IB, int mode,
@@ -1066,11 +1160,11 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
   FGH cur;
   eval_fgh(I, sgn, L, x[0], x[1], x[2], rec, seedu, Lt, cur, red);
   int it = 0;
-  for (; it < 100; ++it) {
+  for (; it < 40; ++it) {
     const double fscale = fabs(cur.v[0]) + 1e-300;
     const double g[3] = {cur.v[1], cur.v[2], cur.v[3]};
     const double gn = fmax(fabs(g[0]), fmax(fabs(g[1]), fabs(g[2])));
-    if (gn <= 1e-13 * fscale) break;
+    if (gn <= 1e-12 * fscale) break;
     // Newton direction with Levenberg damping until positive definite
     const double H[6] = {cur.v[4], cur.v[5], cur.v[6], cur.v[7], cur.v[8], cur.v[9]};
     const double hs = fmax(fabs(H[0]), fmax(fabs(H[3]), fmax(fabs(H[5]), 1e-30)));
@@ -1087,11 +1181,12 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
       dx[2] = -g[2] / hs;
     }
     const double slope = g[0] * dx[0] + g[1] * dx[1] + g[2] * dx[2];
-    if (-0.5 * slope <= 4e-16 * fscale) break;  // predicted decrease below rounding
+    const double dxn = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
+    if (-0.5 * slope <= 1e-13 * fscale || dxn <= 1e-12) break;
     double step = 1.0;
     FGH nxt;
     bool acc = false;
-    for (int ls = 0; ls < 30; ++ls) {
+    for (int ls = 0; ls < 8; ++ls) {
       eval_fgh(I, sgn, L, x[0] + step * dx[0], x[1] + step * dx[1], x[2] + step * dx[2], rec,
                seedu, Lt, nxt, red);
       if (nxt.v[0] <= cur.v[0] + 1e-4 * step * slope) {
@@ -1269,33 +1364,52 @@ void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, cons
                      scale_eff, I64, I32);
 }
 
-static int log2_exact(int n) {
-  int l = 0;
-  while ((1 << l) < n) ++l;
-  return (1 << l) == n ? l : -1;
+// rows / columns per round of the two-pass transform (0: direct DFT)
+static int plane_rounds(int n) {
+  switch (n) {
+    case 16: return 64;
+    case 32: return 32;
+    case 64: return 32;
+    case 128: return 16;
+    case 256: return 16;
+    default: return 0;
+  }
 }
 
-size_t isoft_plane_smem(int n, int Lmax, int chunk) {
+size_t isoft_plane_smem(int n, int Lmax, int use_fft) {
+  const size_t rr = use_fft ? (size_t)plane_rounds(n) : 0;
   return sizeof(float2) * ((size_t)n + 2 * (size_t)(Lmax + 1) * (2 * Lmax + 1) +
-                           2 * (size_t)(Lmax + 1) * n + 2 * (size_t)chunk * n) +
+                           2 * (size_t)(Lmax + 1) * n + rr * n) +
          16 * sizeof(float) + 16 * sizeof(int);
 }
 
-// FFT chunk (rows per pass) that fits max_lds; 0 selects the direct DFT.
-int isoft_plane_chunk(int n, int Lmax, size_t max_lds) {
-  if (log2_exact(n) < 1) return 0;
-  for (int c = 64; c >= 1; c >>= 1)
-    if (isoft_plane_smem(n, Lmax, c) <= max_lds) return c;
-  return 0;
+// 1 if the two-pass transform applies to n and fits max_lds, else 0.
+int isoft_plane_use_fft(int n, int Lmax, size_t max_lds) {
+  return plane_rounds(n) > 0 && isoft_plane_smem(n, Lmax, 1) <= max_lds ? 1 : 0;
 }
 
-void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int chunk, long long Kt,
+void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long long Kt,
                         const SphPair* pd, const float* tab32, const float2* tw32,
                         const float2* IA, const float2* IB, int mode, int norient,
                         PlaneCand* cand, float* grid) {
-  const int logn = chunk > 0 ? log2_exact(n) : -1;
-  hipLaunchKernelGGL(k_isoft_plane, dim3(n, P), dim3(256), isoft_plane_smem(n, Lmax, chunk), s,
-                     pd, n, logn, chunk, Kt, tab32, tw32, IA, IB, mode, norient, cand, grid);
+  const size_t sm = isoft_plane_smem(n, Lmax, use_fft);
+  const dim3 g(n, P), b(256);
+#define FO_PLANE(N1, N2)                                                                        \
+  hipLaunchKernelGGL((k_isoft_plane<N1, N2>), g, b, sm, s, pd, n, Kt, tab32, tw32, IA, IB, mode, \
+                     norient, cand, grid)
+  if (!use_fft) {
+    FO_PLANE(0, 0);
+    return;
+  }
+  switch (n) {
+    case 16: FO_PLANE(4, 4); break;
+    case 32: FO_PLANE(8, 4); break;
+    case 64: FO_PLANE(8, 8); break;
+    case 128: FO_PLANE(16, 8); break;
+    case 256: FO_PLANE(16, 16); break;
+    default: FO_PLANE(0, 0); break;
+  }
+#undef FO_PLANE
 }
 
 void launch_sph_select(hipStream_t s, int P, int n, long long Kt, int norient, const SphPair* pd,
