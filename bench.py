@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 METRIC = "aligned pairs/s batched+served vs plain GPU (same argmax); p50/p99 latency"
 FP32_PEAK = 157.3e12
 HBM_PEAK = 8.0e12
+FP64_PEAK = 78.6e12
 
 
 def log(*a):
@@ -48,6 +49,16 @@ def coeff_flops(groups, L):
     return sum(4 * len(g) ** 2 * (L + 1) ** 2 + 8 * len(g) * kcount(L) for g in groups)
 
 
+def rotopt_eval_flops(L):
+    """FP64 flops of one objective + gradient + Hessian evaluation of the
+    rotation refinement at degree L (array of L): one Wigner-d recurrence
+    step per stored coefficient with m1 >= 0 (sum_l (l+1)(2l+1) of them),
+    32 flops each (d, d', d'' updates and six real accumulations)."""
+    L = np.asarray(L)
+    steps = (L + 1) * (L + 2) * (4 * L + 3) // 6
+    return 32.0 * steps
+
+
 def make_batches(pairs, ctx):
     from fastoverlap_amd.batch import SphericalBatch
     out = []
@@ -67,25 +78,29 @@ def cpu_oracle_pair(c):
     return time.perf_counter() - t
 
 
-def cpu_baseline(pairs, seconds):
+def _noop(_):
+    return 0
+
+
+def cpu_baseline(pairs, per_core):
     """The FP64 oracle (the reference's algorithm restated in numpy/scipy:
     naive einsum coefficients, per-(m1,m2) iSOFT loop, pocketfft, L-BFGS-B,
-    exact LAP) on the host cores, one process per core."""
+    exact LAP) on the host cores, one process per core, over the first
+    per_core * cores pairs of the workload."""
     from concurrent.futures import ProcessPoolExecutor
     os.environ["OMP_NUM_THREADS"] = "1"
     cores = min(16, len(os.sched_getaffinity(0)))
-    # calibrate on one pair, then size the sample to ~`seconds` of wall time
-    t1 = cpu_oracle_pair(pairs[0])
-    nsample = max(cores, min(len(pairs), int(seconds / max(t1, 1e-3)) * cores))
+    nsample = min(len(pairs), per_core * cores)
     sample = pairs[:nsample]
-    t0 = time.perf_counter()
     with ProcessPoolExecutor(cores) as ex:
-        list(ex.map(cpu_oracle_pair, sample))
-    wall = time.perf_counter() - t0
+        list(ex.map(_noop, range(cores)))          # start the workers first
+        t0 = time.perf_counter()
+        list(ex.map(cpu_oracle_pair, sample, chunksize=1))
+        wall = time.perf_counter() - t0
     return {"value": nsample / wall, "unit": "pairs/s", "cores": cores, "kind": "port",
             "sample": f"first {nsample} of the 4096 pairs, FP64 numpy/scipy oracle "
-                      f"(naive einsum, per-(m1,m2) iSOFT, L-BFGS-B, exact LAP), "
-                      f"{cores} processes x 1 thread, {wall:.1f} s"}
+                      f"(naive einsum coefficients, per-(m1,m2) iSOFT, L-BFGS-B, exact LAP), "
+                      f"{cores} processes x 1 thread, {wall:.1f} s wall"}
 
 
 def served(ctx, n_requests, rate_hz, seed=7):
@@ -125,7 +140,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=64)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-per-core", type=int, default=3)
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -163,23 +178,29 @@ def main():
     value = len(pairs) * args.steps / elapsed
     log(f"batched: {ms_per_step:.2f} ms/step, {value:.0f} pairs/s")
 
-    # per-kernel times (events on the batch stream, eager launches)
+    # per-kernel times: HIP events recorded on each batch's stream between the
+    # stages, averaged over `steps` eager passes of the same work
     kt = {}
     for n, sel, b in batches:
         b.enable_timing(True)
-        b.run(graph=False)
-        for k, v in b.last_timing().items():
-            kt[k] = kt.get(k, 0.0) + v
+        for _ in range(args.steps):
+            b.run(graph=False)
+            for k, v in b.last_timing().items():
+                kt[k] = kt.get(k, 0.0) + v / args.steps
         b.enable_timing(False)
     log("kernel ms per step: " + ", ".join(f"{k}={v:.3f}" for k, v in kt.items()))
 
     # batched results kept for the plain-path comparison
     bres = {}
-    its = []
+    its, evs, Ls = [], [], []
     for n, sel, b in batches:
-        its.append(b.opt()["iters"].ravel())
+        o = b.opt()
+        its.append(o["iters"].ravel())
+        evs.append(o["evals"])
+        Ls.append(np.array([c["L"] for c in sel]))
     its = np.concatenate(its)
-    log(f"rotation refinement: Newton iterations mean {its.mean():.2f}, max {its.max()}")
+    log(f"rotation refinement: Newton iterations mean {its.mean():.2f}, max {its.max()}; "
+        f"evaluations mean {np.concatenate([e.ravel() for e in evs]).mean():.2f}")
     for n, sel, b in batches:
         r = b.results()
         pk = b.peaks()
@@ -204,23 +225,31 @@ def main():
     plain = len(sample) / plain_elapsed
     log(f"plain drop-in: {plain:.1f} pairs/s; identical argmax+perm {agree}/{len(sample)}")
 
-    # roofline of the dominant kernel
-    tot_isoft = sum(isoft_flops(c["L"], c["grid_n"]) for c in pairs)
-    tot_coef = sum(coeff_flops(c["groups"], c["L"]) for c in pairs)
+    # roofline of the dominant kernel (algorithmic flops / measured time)
+    work = {
+        "plane": ("k_isoft_plane", sum(isoft_flops(c["L"], c["grid_n"]) for c in pairs), FP32_PEAK),
+        "contract": ("k_sph_contract", sum(coeff_flops(c["groups"], c["L"]) for c in pairs),
+                     FP64_PEAK),
+        "rotopt": ("k_sph_rotopt", sum(float((e * rotopt_eval_flops(L)[:, None]).sum())
+                                       for e, L in zip(evs, Ls)), FP64_PEAK),
+    }
+    per_kernel = {}
+    for k, (name, fl, peak) in work.items():
+        if kt.get(k, 0) > 0:
+            ach = fl / (kt[k] * 1e-3)
+            per_kernel[name] = {"ms": round(kt[k], 4), "achieved_tflops": ach / 1e12,
+                                "peak_tflops": peak / 1e12, "frac": ach / peak}
     dom = max(kt, key=kt.get)
-    if dom == "plane":
-        ach = tot_isoft / (kt["plane"] * 1e-3)
-        roof = {"kernel": "k_isoft_plane", "bound": "mfma", "achieved": ach / 1e12,
-                "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s", "frac": ach / FP32_PEAK,
-                "traffic": None}
-    elif dom == "contract":
-        ach = tot_coef / (kt["contract"] * 1e-3)
-        roof = {"kernel": "k_sph_contract", "bound": "mfma", "achieved": ach / 1e12,
-                "peak": 78.6, "unit": "TFLOP/s", "frac": ach / 78.6e12, "traffic": None}
+    if dom in work:
+        name, fl, peak = work[dom]
+        ach = fl / (kt[dom] * 1e-3)
+        roof = {"kernel": name, "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12,
+                "unit": "TFLOP/s", "frac": ach / peak, "traffic": None}
     else:
         roof = {"kernel": dom, "bound": "mfma", "achieved": None, "peak": None,
                 "unit": "TFLOP/s", "frac": None, "traffic": None}
     roof["kernel_ms"] = {k: round(v, 4) for k, v in kt.items()}
+    roof["per_kernel"] = per_kernel
 
     out = {
         "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1,
@@ -242,7 +271,7 @@ def main():
         out["serve"] = served(ctx, args.serve_requests, rate_hz=2000.0)
         log(f"served: {out['serve']}")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pairs, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(pairs, args.cpu_per_core)
         log(f"cpu baseline: {out['cpu_baseline']}")
     for _, _, b in batches:
         b.close()

@@ -89,7 +89,7 @@ _PROTOS = {
     "fo_sph_batch_sync": (C.c_int, [_vp]),
     "fo_sph_batch_results": (C.c_int, [_vp, _pd, _pd, _pd, _pi, _pd, _pd, _pi]),
     "fo_sph_batch_peaks": (C.c_int, [_vp, _pi, _pd, _pd, _pd, _pd, _pi, _pi]),
-    "fo_sph_batch_opt": (C.c_int, [_vp, _pd, _pd, _pd, _pi]),
+    "fo_sph_batch_opt": (C.c_int, [_vp, _pd, _pd, _pd, _pi, _pi]),
     "fo_sph_batch_scales": (C.c_int, [_vp, _pd]),
     "fo_sph_batch_coeffs": (C.c_int, [_vp, _pd]),
     "fo_sph_batch_grid": (C.c_int, [_vp, C.c_int, _pf]),

@@ -197,10 +197,11 @@ class SphericalBatch:
         R = np.zeros((m, 3))
         fun, gn = np.zeros(m), np.zeros(m)
         it = np.zeros(m, np.int32)
-        call("fo_sph_batch_opt", self.handle, dptr(R), dptr(fun), dptr(gn), iptr(it))
+        ev = np.zeros(m, np.int32)
+        call("fo_sph_batch_opt", self.handle, dptr(R), dptr(fun), dptr(gn), iptr(it), iptr(ev))
         sh = (self.P, self.norient)
         return dict(R=R.reshape(sh + (3,)), fun=fun.reshape(sh), gnorm=gn.reshape(sh),
-                    iters=it.reshape(sh))
+                    iters=it.reshape(sh), evals=ev.reshape(sh))
 
     def scales(self):
         out = np.zeros(self.P)

@@ -780,7 +780,8 @@ int fo_sph_batch_peaks(fo_sph_batch* b, int* ind, double* fun, double* peak_valu
   return FO_OK;
 }
 
-int fo_sph_batch_opt(fo_sph_batch* b, double* R, double* fun, double* gnorm, int* iters) {
+int fo_sph_batch_opt(fo_sph_batch* b, double* R, double* fun, double* gnorm, int* iters,
+                     int* evals) {
   if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
   const size_t m = (size_t)b->lc.P * b->lc.norient;
   std::vector<SphOpt> op(m);
@@ -792,6 +793,7 @@ int fo_sph_batch_opt(fo_sph_batch* b, double* R, double* fun, double* gnorm, int
     if (fun) fun[i] = op[i].f;
     if (gnorm) gnorm[i] = op[i].gnorm;
     if (iters) iters[i] = op[i].iters;
+    if (evals) evals[i] = op[i].evals;
   }
   return FO_OK;
 }

@@ -59,7 +59,7 @@ struct SphOpt {
   double f;            // objective -Re sum conj(I) D(R)
   double gnorm;        // final gradient max-norm
   int iters;
-  int pad;
+  int evals;           // objective evaluations, line search included
 };
 
 struct SphResult {

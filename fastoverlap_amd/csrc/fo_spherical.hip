@@ -1159,6 +1159,7 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
   double x[3] = {pk.R0[0], pk.R0[1], pk.R0[2]};
   FGH cur;
   eval_fgh(I, sgn, L, x[0], x[1], x[2], rec, seedu, Lt, cur, red);
+  int nev = 1;
   int it = 0;
   for (; it < 40; ++it) {
     const double fscale = fabs(cur.v[0]) + 1e-300;
@@ -1189,6 +1190,7 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
     for (int ls = 0; ls < 8; ++ls) {
       eval_fgh(I, sgn, L, x[0] + step * dx[0], x[1] + step * dx[1], x[2] + step * dx[2], rec,
                seedu, Lt, nxt, red);
+      ++nev;
       if (nxt.v[0] <= cur.v[0] + 1e-4 * step * slope) {
         acc = true;
         break;
@@ -1207,7 +1209,7 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
     r.f = cur.v[0];
     r.gnorm = fmax(fabs(cur.v[1]), fmax(fabs(cur.v[2]), fabs(cur.v[3])));
     r.iters = it;
-    r.pad = 0;
+    r.evals = nev;
     opt[(long long)p * norient + o] = r;
   }
 }

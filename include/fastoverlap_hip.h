@@ -121,8 +121,10 @@ int fo_sph_batch_results(fo_sph_batch* b, double* dist, double* X1, double* X2, 
 int fo_sph_batch_peaks(fo_sph_batch* b, int* ind, double* fun, double* peak_value,
                        double* margin, double* delta, int* ncand, int* noverflow);
 /* rotation refinement per (pair, orientation): R[3], objective, final
- * gradient max-norm, Newton iterations */
-int fo_sph_batch_opt(fo_sph_batch* b, double* R, double* fun, double* gnorm, int* iters);
+ * gradient max-norm, Newton iterations, objective evaluations (incl. the
+ * line search) */
+int fo_sph_batch_opt(fo_sph_batch* b, double* R, double* fun, double* gnorm, int* iters,
+                     int* evals);
 /* kernel width used per pair (resolved on the device when <= 0 was given) */
 int fo_sph_batch_scales(fo_sph_batch* b, double* scale);
 int fo_sph_batch_coeffs(fo_sph_batch* b, double* coef);

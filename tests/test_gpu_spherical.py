@@ -179,6 +179,32 @@ def test_c2_batch_parity_and_batch_independence(gpu):
         np.testing.assert_array_equal(r1["X2"][0], r["X2"][-1])
 
 
+def test_c3_large_jmax_against_fixture(gpu):
+    """1000 atoms (600:400), Jmax 40, grid 128, width 0.2: the FP64 Wigner-d
+    table at high l, the FP32 beta contraction over 41 degrees and the FP64
+    re-evaluation, against the oracle's stored answer
+    (tests/golden/make_c3_golden.py)."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "C3_pair0.npz"))
+    sizes = z["group_sizes"]
+    groups = [np.arange(sizes[0]), np.arange(sizes[0], sizes[0] + sizes[1])]
+    L, n, s = int(z["L"]), int(z["grid_n"]), float(z["scale"])
+    say(f"case: C3 N=1000 L={L} n={n} sigma={s}")
+    with SphericalBatch([1000], [L], [s], n, [groups], _lib.SPH_KEEP_GRID) as b:
+        b.set_positions(z["pos1"], z["pos2"])
+        b.run()
+        r = b.results()
+        pk = b.peaks()
+        grids = b.grid(0)
+    for k in range(2):
+        assert tuple(pk["ind"][0][k]) == tuple(z["ind"][k]), (k, pk["ind"][0][k], z["ind"][k])
+        got = grids[k].ravel()[z["top_idx"][k]]
+        assert np.abs(got - z["top_val"][k]).max() <= GRID_TOL * z["gmax"][k], k
+    assert bool(r["inverted"][0]) == bool(z["inverted"])
+    np.testing.assert_array_equal(r["perm"][0], z["perm"])
+    assert abs(r["dist"][0] - float(z["dist"])) <= DIST_TOL, (r["dist"][0], float(z["dist"]))
+
+
 def test_graph_replay_matches_eager(gpu):
     pairs = [W.config2(p) for p in range(100, 108)]
     sel = [c for c in pairs if c["grid_n"] == 64]
