@@ -300,8 +300,11 @@ __global__ void k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __r
 // ---------------------------------------------------------------------------
 // Contraction I^l_{m1m2} = 4 pi^{5/2} s^3 sum_s sum_{j,k in s}
 //   Y1[l,m1,j] K_l[j,k] conj(Y2[l,m2,k])      (sphericalAlignment.py:273,:175)
-// One workgroup (256 threads) per (pair, l, block of kMB rows m1), an LDS
-// tiled FP64 GEMM chain.  Per tile of kKT atoms k of structure 2:
+// Only rows m1 >= 0 are contracted: Y_l^{-m} = (-1)^m conj(Y_l^m) and K_l
+// is real, so I^l_{-m1,-m2} = (-1)^{m1+m2} conj(I^l_{m1m2}) and the m1 < 0
+// rows are stored from the computed ones.
+// One workgroup (256 threads) per (pair, l, block of kMB rows m1 >= 0), an
+// LDS tiled FP64 GEMM chain.  Per tile of kKT atoms k of structure 2:
 //   A[r][k]  = sum_j Y1[m1a+r][j] K_l[j][k]   (j in tiles of kJT; 2x2
 //              complex-by-real register tile per thread, A -> LDS)
 //   acc[r][m2] += sum_k A[r][k] conj(Y2[m2][k])  (4x4 complex register tile;
@@ -320,8 +323,8 @@ FO_DEV void contract_body(const SphPair& d, int l, int b, const SphGroup* __rest
                           const double* __restrict__ R, double sc, double2* __restrict__ I64,
                           float2* __restrict__ I32, unsigned char* smem) {
   const int M = 2 * l + 1;
-  const int m1a = b * kMB;  // row offset within [0, M)
-  const int rows = min(kMB, M - m1a);
+  const int m1a = b * kMB;  // first m1 (>= 0) of this workgroup
+  const int rows = min(kMB, l + 1 - m1a);
   double2* Y1s = (double2*)smem;                 // [kMB][kJT]
   double* Ks = (double*)(Y1s + kMB * kJT);       // [kJT][kKT]
   double2* As = (double2*)(Ks + kJT * kKT);      // [kMB][kKT]
@@ -356,7 +359,7 @@ FO_DEV void contract_body(const SphPair& d, int l, int b, const SphGroup* __rest
         __syncthreads();
         for (int e = t; e < kMB * kJT; e += blockDim.x) {
           const int r = e / kJT, jj = e % kJT;
-          Y1s[e] = (r < rows && jj < jt) ? y1[(long long)(m1a + r) * Ng + G.gath_off + j0 + jj]
+          Y1s[e] = (r < rows && jj < jt) ? y1[(long long)(l + m1a + r) * Ng + G.gath_off + j0 + jj]
                                          : make_double2(0.0, 0.0);
         }
         for (int e = t; e < kJT * kKT; e += blockDim.x) {
@@ -409,10 +412,18 @@ FO_DEV void contract_body(const SphPair& d, int l, int b, const SphGroup* __rest
     for (int q = 0; q < NQ; ++q) {
       const int r = br + 8 * i, m2i = bc + 32 * q;
       if (r >= rows || m2i >= M) continue;
-      const long long id = d.coef_off + kidx(l, m1a + r - l, m2i - l);
+      const int m1 = m1a + r, m2 = m2i - l;
+      const long long id = d.coef_off + kidx(l, m1, m2);
       const double2 v = cscale(acc[i][q], norm);
       I64[id] = v;
       I32[id] = make_float2((float)v.x, (float)v.y);
+      if (m1 > 0) {
+        const long long im = d.coef_off + kidx(l, -m1, -m2);
+        const double sg = ((m1 + m2) & 1) ? -1.0 : 1.0;
+        const double2 w = make_double2(sg * v.x, -sg * v.y);
+        I64[im] = w;
+        I32[im] = make_float2((float)w.x, (float)w.y);
+      }
     }
 }
 
@@ -427,7 +438,7 @@ __global__ void __launch_bounds__(256) k_sph_contract(
   // decode (l, m1 block) from blockIdx.x
   int b = blockIdx.x, l = 0;
   for (; l <= L; ++l) {
-    const int nb = (2 * l + 1 + kMB - 1) / kMB;
+    const int nb = (l + 1 + kMB - 1) / kMB;
     if (b < nb) break;
     b -= nb;
   }
@@ -1358,7 +1369,7 @@ void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, cons
                          const double2* Y1, const double2* Y2, const double* R,
                          const double* scale_eff, double2* I64, float2* I32) {
   int nb = 0;
-  for (int l = 0; l <= Lmax; ++l) nb += (2 * l + 1 + kMB - 1) / kMB;
+  for (int l = 0; l <= Lmax; ++l) nb += (l + 1 + kMB - 1) / kMB;
   const size_t smem = sizeof(double2) * ((size_t)kMB * kJT + (size_t)kMB * kKT +
                                          (size_t)kKT * (2 * Lmax + 1)) +
                       sizeof(double) * (size_t)kJT * kKT;
