@@ -142,6 +142,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--cpu-per-core", type=int, default=3)
     ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--gen-workers", type=int, default=None,
+                    help="processes drawing the synthetic pairs (default: up to 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
@@ -154,7 +156,7 @@ def main():
     from fastoverlap_amd.spherical import SphericalAlign
 
     t = time.perf_counter()
-    pairs = W.generate(W.config2, range(args.pairs))
+    pairs = W.generate(W.config2, range(args.pairs), workers=args.gen_workers)
     log(f"generated {len(pairs)} pairs in {time.perf_counter() - t:.1f}s")
     ctx = _lib.default_context()
     batches = make_batches(pairs, ctx)
