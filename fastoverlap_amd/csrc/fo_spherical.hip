@@ -6,7 +6,9 @@
 //   k_sph_ylm        Y_lm of every grouped atom (FP64)
 //   k_sph_radial     scaled i_l(x) * Gaussian radial kernel (FP64)
 //   k_sph_contract   C^l = Y1 . K_l . Y2^H per degree (FP64)
-//   k_isoft_plane    inverse SOFT of both orientations per beta plane (FP32),
+//   k_isoft_fft      inverse SOFT of both orientations per beta plane (FP32),
+//                    two-pass register transforms (power-of-two n); k_isoft_plane
+//                    is the direct-DFT form for other n;
 //                    top candidates per plane
 //   k_sph_select     FP64 re-evaluation of every candidate within the FP32
 //                    error bound of the maximum -> exact argmax, parabolic
@@ -522,48 +524,369 @@ FO_DEV void dft_reg(float2 (&v)[R]) {
   }
 }
 
-// Per-thread record of the largest grid value and a bound on the others.
-struct CandTrack {
-  float bv[2], dm[2];
-  int bi[2];
-  FO_DEV CandTrack() {
-    bv[0] = bv[1] = dm[0] = dm[1] = -INFINITY;
-    bi[0] = bi[1] = 0x7fffffff;
-  }
-  FO_DEV void add(int o, int flat, float val) {
-    if (val > bv[o]) {
-      dm[o] = fmaxf(dm[o], bv[o]);
-      bv[o] = val;
-      bi[o] = flat;
+// Per-thread record of the largest grid value and a bound on the others,
+// per orientation (named fields: no runtime-indexed private arrays).
+struct CandOne {
+  float bv = -INFINITY, dm = -INFINITY;
+  int bi = 0x7fffffff;
+  FO_DEV void add(int flat, float val) {
+    if (val > bv) {
+      dm = fmaxf(dm, bv);
+      bv = val;
+      bi = flat;
     } else {
-      dm[o] = fmaxf(dm[o], val);
+      dm = fmaxf(dm, val);
     }
   }
 };
+struct CandTrack {
+  CandOne c0, c1;
+  FO_DEV void add(int o, int flat, float val) {
+    if (o == 0)
+      c0.add(flat, val);
+    else
+      c1.add(flat, val);
+  }
+  FO_DEV const CandOne& get(int o) const { return o == 0 ? c0 : c1; }
+};
 
-// N1 > 0: n = N1 N2 (powers of two <= 16); each length-n inverse DFT is two
-// passes of register DFTs (N1 of length N2, twiddle e^{2 pi i j1 k2/n},
-// N2 of length N1) with one LDS exchange; groups of TPF = max(N1, N2)
-// threads transform one row / column.
-// N1 == 0: direct DFT passes (any n).
+// Plane top-kCandPerPlane and a bound on every other value of the plane.
+// Each wavefront lists its kCandPerPlane best per-thread maxima (xor
+// butterflies, no barrier), its next one and the largest per-thread bound;
+// one barrier, then wavefront 0 merges the four lists the same way.
+// `cv` / `ci`: 4 waves x 2 orientations x 6 floats / 4 ints of LDS.
+FO_DEV void plane_candidates(const CandTrack& ct, int norient, float* cv, int* ci,
+                             PlaneCand* out, int ostride) {
+  const int t = threadIdx.x, wl = t & 63, w = t >> 6;
+  for (int o = 0; o < norient; ++o) {
+    const float mbv = o == 0 ? ct.c0.bv : ct.c1.bv;
+    const int mbi = o == 0 ? ct.c0.bi : ct.c1.bi;
+    bool taken = false;
+    for (int r = 0; r <= kCandPerPlane; ++r) {
+      float v = taken ? -INFINITY : mbv;
+      int i = taken ? 0x7fffffff : mbi;
+      wave_argmax(v, i);
+      if (wl == 0) {
+        cv[(w * 2 + o) * 6 + r] = v;
+        if (r < kCandPerPlane) ci[(w * 2 + o) * 4 + r] = i;
+      }
+      if (!taken && i == mbi && v == mbv) taken = true;
+    }
+    const float dmw = wave_max(o == 0 ? ct.c0.dm : ct.c1.dm);
+    if (wl == 0) cv[(w * 2 + o) * 6 + 5] = dmw;
+  }
+  __syncthreads();
+  if (w == 0) {
+    // lane j < 16 holds entry j of the four wave lists
+    const int ww = wl >> 2, r = wl & 3;
+    for (int o = 0; o < norient; ++o) {
+      float v = wl < 16 ? cv[(ww * 2 + o) * 6 + r] : -INFINITY;
+      int i = wl < 16 ? ci[(ww * 2 + o) * 4 + r] : 0x7fffffff;
+      const float rw = wl < 4 ? fmaxf(cv[(wl * 2 + o) * 6 + kCandPerPlane], cv[(wl * 2 + o) * 6 + 5])
+                              : -INFINITY;
+      float rest = wave_max(rw);
+      // lane q < kCandPerPlane keeps the q-th best
+      float keepv = -INFINITY;
+      int keepi = 0x7fffffff;
+      bool taken = false;
+#pragma unroll
+      for (int q = 0; q <= kCandPerPlane; ++q) {
+        float bv = taken ? -INFINITY : v;
+        int bi = taken ? 0x7fffffff : i;
+        wave_argmax(bv, bi);
+        if (q < kCandPerPlane) {
+          if (wl == q) {
+            keepv = bv;
+            keepi = bi;
+          }
+        } else {
+          rest = fmaxf(rest, bv);
+        }
+        if (!taken && bi == i && bv == v) taken = true;
+      }
+      PlaneCand* dst = out + (long long)o * ostride;
+      if (wl < kCandPerPlane) {
+        dst->val[wl] = keepv;
+        dst->idx[wl] = keepi;
+      }
+      if (wl == 0) {
+        dst->rest = rest;
+        dst->pad[0] = dst->pad[1] = dst->pad[2] = 0;
+      }
+    }
+  }
+}
+
+// Ordering of LDS accesses between lanes of one wavefront (no s_barrier).
+FO_DEV void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// DPP row shifts (within rows of 16 lanes): row_shl:S gives lane i the value
+// of lane i + S, row_shr:S that of lane i - S.
+template <int CTRL>
+FO_DEV float dpp_mov(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, true));
+}
+
+// One stage of the lane transpose: lanes i and i ^ S (S < 16, groups
+// aligned within a row of 16 lanes) swap the registers whose index bit S
+// differs from their own lane bit.
+template <int R, int S>
+FO_DEV void transpose_stage(float2 (&v)[R], int lane) {
+  const bool hi = (lane & S) != 0;
+#pragma unroll
+  for (int q = 0; q < R / 2; ++q) {
+    const int r0 = (q / S) * 2 * S + (q % S), r1 = r0 + S;
+    const float2 A = v[r0], B = v[r1];
+    // hi lanes receive B of lane i - S, lo lanes A of lane i + S
+    const float2 fromlo = make_float2(dpp_mov<0x110 + S>(B.x), dpp_mov<0x110 + S>(B.y));
+    const float2 fromhi = make_float2(dpp_mov<0x100 + S>(A.x), dpp_mov<0x100 + S>(A.y));
+    v[r0] = hi ? fromlo : A;
+    v[r1] = hi ? B : fromhi;
+  }
+}
+
+// Transpose of R x R values held by R consecutive lanes (lane i, register r)
+// -> (lane r, register i): log2 R stages.
+template <int R>
+FO_DEV void lane_transpose(float2 (&v)[R], int lane) {
+  static_assert(R == 2 || R == 4 || R == 8 || R == 16, "radix");
+  transpose_stage<R, 1>(v, lane);
+  if constexpr (R >= 4) transpose_stage<R, 2>(v, lane);
+  if constexpr (R >= 8) transpose_stage<R, 4>(v, lane);
+  if constexpr (R >= 16) transpose_stage<R, 8>(v, lane);
+}
+
+// Second half of a two-pass transform of length n = N1 N2: lane j1 < N1
+// holds the twiddled first-pass outputs v[k2] (k2 < N2); afterwards lane
+// k2 < N2 holds the N1 inputs of its length-N1 DFT in u[j1].  Square
+// shapes exchange in registers; others through the group's LDS row xr
+// (index i stored at i + i / N1, so both access patterns are
+// bank-conflict free), ordered within the wavefront.
 template <int N1, int N2>
+FO_DEV void two_pass_exchange(float2 (&v)[N2], float2 (&u)[N1], int lane, float2* xr) {
+  if constexpr (N1 == N2) {
+    lane_transpose<N1>(v, lane);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) u[j] = v[j];
+  } else {
+    if (lane < N1)
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int i = lane + N1 * k2;
+        xr[i + i / N1] = v[k2];
+      }
+    wave_sync();
+    if (lane < N2)
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        const int i = j1 + N1 * lane;
+        u[j1] = xr[i + i / N1];
+      }
+    wave_sync();
+  }
+}
+
+template <int N1, int N2>
+constexpr int fft_tpf() {
+  return N1 > N2 ? N1 : N2;
+}
+
+// ---------------------------------------------------------------------------
+// Power-of-two grids, n = N1 N2 (N1, N2 <= 16, radix-2 register DFTs).  One
+// workgroup (256 threads) per (pair, beta plane), groups of TPF = max(N1,N2)
+// lanes, one block barrier between the passes and one for the candidates.
+//  gamma pass: group <-> row m1 >= 0 (both orientations).  Lane j1 < N1
+//    owns the entries m2 = k (k <= L) or k - n (k >= n - L) of
+//    k = j1 + N1 j2, contracts them over l in an l-outer loop (N2
+//    independent loads per step), then the row transform: DFT of length N2
+//    in registers, twiddle e^{2 pi i j1 k2 / n}, lane exchange, DFT of
+//    length N1; output k2 + N2 k1 is stored at T[o][g][m1] (odd stride LS).
+//  alpha pass: group <-> column g, the same transform of
+//    X[m1] = T_A[m1][g] + i T_B[m1][g] (m1 >= 0) with its Hermitian
+//    extension X[-m1] = conj T_A + i conj T_B; Re / Im of the result are
+//    the grid values of the two orientations.
+// Workgroup w of the 1-D launch serves pair (w % 8) + 8 (w / 8n) and plane
+// (w / 8) % n, so all planes of a pair run on one XCD and share its L2.
+// ---------------------------------------------------------------------------
+template <int N1, int N2, int MODE>
+__global__ void __launch_bounds__(256) k_isoft_fft(
+    const SphPair* __restrict__ pd, int P, long long Kt, const float* __restrict__ tab32,
+    const float2* __restrict__ twg, const float2* __restrict__ IA,
+    const float2* __restrict__ IB, int norient, PlaneCand* __restrict__ cand,
+    float* __restrict__ grid) {
+  constexpr int n = N1 * N2;
+  constexpr int TPF = fft_tpf<N1, N2>();
+  constexpr int NG = 256 / TPF;           // lane groups per workgroup
+  constexpr int XS = n + n / N1;          // padded exchange row (non-square)
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int wg = blockIdx.x;
+  const int p = (wg & 7) + 8 * (wg / (8 * n));
+  const int ib = (wg >> 3) % n;
+  if (p >= P) return;  // whole workgroup: padding of P to a multiple of 8
+  const SphPair d = pd[p];
+  const int L = d.L, t = threadIdx.x;
+  const int LS = (L + 1) | 1;
+  float2* T = (float2*)smem;                              // [2][n][LS]
+  float2* X = T + 2 * n * LS;                             // [NG][XS]
+  float* cv = (float*)(X + (N1 == N2 ? 0 : NG * XS));     // [4][2][6]
+  int* ci = (int*)(cv + 48);                              // [4][2][4]
+  const int grp = t / TPF, lane = t % TPF;
+  float2* xr = X + grp * XS;
+  float2 tw[N2];
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) tw[k2] = twg[(lane * k2) % n];
+  const float* tb = tab32 + (long long)ib * Kt;
+  const float2* ia = IA + d.coef_off;
+  const float2* ibb = IB + d.coef_off;
+  const int no = norient;
+
+  // ---- gamma pass (rows m1 in rounds of NG)
+  for (int m1_0 = 0; m1_0 <= L; m1_0 += NG) {
+    const int m1 = m1_0 + grp;
+    const bool row = m1 <= L;
+    // even / odd-l partial sums (alignment mode) or the A / B sums
+    float2 e[N2], od[N2];
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) e[j2] = od[j2] = make_float2(0.f, 0.f);
+    if (row && lane < N1) {
+      int m2v[N2];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) {
+        const int k = lane + N1 * j2;
+        m2v[j2] = k <= L ? k : (k >= n - L ? k - n : 1 << 20);
+      }
+      int base = m1 * (4 * m1 * m1 - 1) / 3;  // kbase(m1)
+      for (int l = m1; l <= L; ++l) {
+        const int rb = base + (m1 + l) * (2 * l + 1) + l;  // kidx(l, m1, 0)
+        base += (2 * l + 1) * (2 * l + 1);
+        float dv[N2];
+        float2 ca[N2];
+#pragma unroll
+        for (int j2 = 0; j2 < N2; ++j2) {
+          const int m2 = m2v[j2];
+          const bool in = m2 >= -l && m2 <= l;
+          dv[j2] = in ? tb[rb + m2] : 0.f;
+          ca[j2] = in ? ia[rb + m2] : make_float2(0.f, 0.f);
+        }
+        if constexpr (MODE != 0) {
+#pragma unroll
+          for (int j2 = 0; j2 < N2; ++j2) {
+            const int m2 = m2v[j2];
+            const float2 cb = (m2 >= -l && m2 <= l) ? ibb[rb + m2] : make_float2(0.f, 0.f);
+            e[j2].x += dv[j2] * ca[j2].x;
+            e[j2].y += dv[j2] * ca[j2].y;
+            od[j2].x += dv[j2] * cb.x;
+            od[j2].y += dv[j2] * cb.y;
+          }
+        } else {
+          // l parity is uniform over the workgroup
+          if (l & 1) {
+#pragma unroll
+            for (int j2 = 0; j2 < N2; ++j2) {
+              od[j2].x += dv[j2] * ca[j2].x;
+              od[j2].y += dv[j2] * ca[j2].y;
+            }
+          } else {
+#pragma unroll
+            for (int j2 = 0; j2 < N2; ++j2) {
+              e[j2].x += dv[j2] * ca[j2].x;
+              e[j2].y += dv[j2] * ca[j2].y;
+            }
+          }
+        }
+      }
+    }
+    for (int o = 0; o < no; ++o) {
+      float2 v[N2], u[N1];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) {
+        if constexpr (MODE == 0)
+          v[j2] = o == 0 ? make_float2(e[j2].x + od[j2].x, e[j2].y + od[j2].y)
+                         : make_float2(e[j2].x - od[j2].x, e[j2].y - od[j2].y);
+        else
+          v[j2] = o == 0 ? e[j2] : od[j2];
+      }
+      dft_reg<N2>(v);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) v[k2] = cmul(v[k2], tw[k2]);
+      two_pass_exchange<N1, N2>(v, u, lane, xr);
+      dft_reg<N1>(u);
+      if (row && lane < N2)
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) T[(o * n + lane + N2 * k1) * LS + m1] = u[k1];
+    }
+  }
+  __syncthreads();
+
+  // ---- alpha pass (columns g in rounds of NG)
+  CandTrack ct;
+  const float2* TA = T;
+  const float2* TB = T + n * LS;
+  for (int g0 = 0; g0 < n; g0 += NG) {
+    const int g = g0 + grp;
+    const bool col = g < n;
+    float2 v[N2], u[N1];
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) {
+      v[j2] = make_float2(0.f, 0.f);
+      const int k = lane + N1 * j2;
+      const int m1 = k <= L ? k : k - n;
+      if (!col || lane >= N1) continue;
+      if (m1 == 0) {
+        // T[0][g] is real for a Hermitian set: keep the real parts only
+        v[j2] = make_float2(TA[g * LS].x, no == 2 ? TB[g * LS].x : 0.f);
+      } else if (m1 > 0) {
+        const float2 A = TA[g * LS + m1];
+        const float2 B = no == 2 ? TB[g * LS + m1] : make_float2(0.f, 0.f);
+        v[j2] = make_float2(A.x - B.y, A.y + B.x);
+      } else if (m1 >= -L) {
+        const float2 A = TA[g * LS - m1];
+        const float2 B = no == 2 ? TB[g * LS - m1] : make_float2(0.f, 0.f);
+        v[j2] = make_float2(A.x + B.y, B.x - A.y);
+      }
+    }
+    dft_reg<N2>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) v[k2] = cmul(v[k2], tw[k2]);
+    two_pass_exchange<N1, N2>(v, u, lane, xr);
+    dft_reg<N1>(u);
+    if (col && lane < N2) {
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) {
+        const int a = lane + N2 * k1;
+        const int flat = (a * n + ib) * n + g;
+        if (grid) grid[((long long)p * no) * n * n * n + flat] = u[k1].x;
+        ct.add(0, flat, u[k1].x);
+        if (no == 2) {
+          if (grid) grid[((long long)p * no + 1) * n * n * n + flat] = u[k1].y;
+          ct.add(1, flat, u[k1].y);
+        }
+      }
+    }
+  }
+  plane_candidates(ct, no, cv, ci, cand + ((long long)p * no) * n + ib, n);
+}
+
+// Direct-DFT planes (any n): the beta contraction per (m1 >= 0, m2) entry,
+// then the gamma and alpha passes as direct DFTs through LDS.
 __global__ void __launch_bounds__(256) k_isoft_plane(
     const SphPair* __restrict__ pd, int n, long long Kt, const float* __restrict__ tab32,
     const float2* __restrict__ twg, const float2* __restrict__ IA,
     const float2* __restrict__ IB, int mode, int norient, PlaneCand* __restrict__ cand,
     float* __restrict__ grid) {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int TPF = N1 > N2 ? N1 : N2;
-  constexpr int RR = N1 > 0 ? 256 / TPF : 0;   // rows / columns per round
   const int ib = blockIdx.x, p = blockIdx.y;
   const SphPair d = pd[p];
   const int L = d.L, W = 2 * L + 1, t = threadIdx.x, bd = blockDim.x;
   float2* tw = (float2*)smem;                 // [n]
   float2* S = tw + n;                          // [2][L+1][W]
   float2* T = S + 2 * (L + 1) * W;             // [2][L+1][n]
-  float2* Yb = T + 2 * (L + 1) * n;            // [RR][n] (two-pass path)
-  float* rv = (float*)(Yb + RR * n);           // reduction scratch
-  int* ri = (int*)(rv + 16);
+  float* cv = (float*)(T + 2 * (L + 1) * n);   // candidate exchange
+  int* ci = (int*)(cv + 48);
   for (int k = t; k < n; k += bd) tw[k] = twg[k];
   const float* tb = tab32 + (long long)ib * Kt;
   const float2* ia = IA + d.coef_off;
@@ -609,152 +932,43 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
     if (grid) grid[((long long)p * norient + o) * n * n * n + flat] = val;
     ct.add(o, flat, val);
   };
-  // column g of the alpha pass: X[m1] = T_A[m1][g] + i T_B[m1][g] for m1 >= 0
-  // and the Hermitian extension for m1 < 0; Re / Im of its inverse
-  // transform are the two real grids.
-  const float2* TA = T;
-  const float2* TB = T + (L + 1) * n;
-  auto column_value = [&](int g, int k) {
-    const int m1 = k <= L ? k : k - n;
-    float2 v = make_float2(0.f, 0.f);
-    if (m1 == 0) {
-      // T[0][g] is real for a Hermitian set: keep the real parts only
-      v = make_float2(TA[g].x, norient == 2 ? TB[g].x : 0.f);
-    } else if (m1 > 0) {
-      const float2 A = TA[m1 * n + g];
-      const float2 B = norient == 2 ? TB[m1 * n + g] : make_float2(0.f, 0.f);
-      v = make_float2(A.x - B.y, A.y + B.x);
-    } else if (m1 >= -L) {
-      const float2 A = TA[-m1 * n + g];
-      const float2 B = norient == 2 ? TB[-m1 * n + g] : make_float2(0.f, 0.f);
-      v = make_float2(A.x + B.y, B.x - A.y);
+  // gamma pass: T[o][m1][g] = sum_{m2} S[o][m1][m2] e^{i m2 g}
+  for (int e = t; e < norient * (L + 1) * n; e += bd) {
+    const int o = e / ((L + 1) * n);
+    const int rem = e % ((L + 1) * n);
+    const int m1 = rem / n, g = rem % n;
+    const float2* Sr = S + (o * (L + 1) + m1) * W;
+    float2 acc = make_float2(0.f, 0.f);
+    int ph = (n - (L % n) * g % n) % n;  // (-L * g) mod n
+    for (int m2i = 0; m2i < W; ++m2i) {
+      const float2 w = tw[ph];
+      const float2 sv = Sr[m2i];
+      acc.x += sv.x * w.x - sv.y * w.y;
+      acc.y += sv.x * w.y + sv.y * w.x;
+      ph += g;
+      if (ph >= n) ph -= n;
     }
-    return v;
-  };
-  if constexpr (N1 > 0) {
-    const int grp = t / TPF, lane = t % TPF;
-    // gamma pass: row (o, m1) holds S[o][m1][m2] at index m2 mod n
-    const int nrows = norient * (L + 1);
-    for (int r0 = 0; r0 < nrows; r0 += RR) {
-      const int r = r0 + grp;
-      float2* Trow = T + r * n;
-      if (r < nrows && lane < N1) {
-        const int j1 = lane;
-        float2 v[N2];
-#pragma unroll
-        for (int j2 = 0; j2 < N2; ++j2) {
-          const int k = j1 + N1 * j2;
-          const int m2 = k <= L ? k : k - n;
-          v[j2] = (m2 >= -L) ? S[r * W + m2 + L] : make_float2(0.f, 0.f);
-        }
-        dft_reg<N2>(v);
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) Trow[j1 + N1 * k2] = cmul(v[k2], tw[(j1 * k2) % n]);
-      }
-      __syncthreads();
-      float2 u[N1];
-      const bool act = r < nrows && lane < N2;
-      if (act) {
-#pragma unroll
-        for (int j1 = 0; j1 < N1; ++j1) u[j1] = Trow[j1 + N1 * lane];
-        dft_reg<N1>(u);
-      }
-      __syncthreads();
-      if (act)
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) Trow[lane + N2 * k1] = u[k1];
-    }
-    __syncthreads();
-    // alpha pass: the second half of each transform goes straight to the
-    // candidate tracker
-    for (int g0 = 0; g0 < n; g0 += RR) {
-      const int g = g0 + grp;
-      float2* Y = Yb + grp * n;
-      if (g < n && lane < N1) {
-        const int j1 = lane;
-        float2 v[N2];
-#pragma unroll
-        for (int j2 = 0; j2 < N2; ++j2) v[j2] = column_value(g, j1 + N1 * j2);
-        dft_reg<N2>(v);
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) Y[j1 + N1 * k2] = cmul(v[k2], tw[(j1 * k2) % n]);
-      }
-      __syncthreads();
-      if (g < n && lane < N2) {
-        float2 u[N1];
-#pragma unroll
-        for (int j1 = 0; j1 < N1; ++j1) u[j1] = Y[j1 + N1 * lane];
-        dft_reg<N1>(u);
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) {
-          const int a = lane + N2 * k1;
-          consider(0, a, g, u[k1].x);
-          if (norient == 2) consider(1, a, g, u[k1].y);
-        }
-      }
-      __syncthreads();
-    }
-  } else {
-    // gamma pass
-    for (int e = t; e < norient * (L + 1) * n; e += bd) {
-      const int o = e / ((L + 1) * n);
-      const int rem = e % ((L + 1) * n);
-      const int m1 = rem / n, g = rem % n;
-      const float2* Sr = S + (o * (L + 1) + m1) * W;
-      float2 acc = make_float2(0.f, 0.f);
-      int ph = (n - (L % n) * g % n) % n;  // (-L * g) mod n
-      for (int m2i = 0; m2i < W; ++m2i) {
-        const float2 w = tw[ph];
-        const float2 sv = Sr[m2i];
-        acc.x += sv.x * w.x - sv.y * w.y;
-        acc.y += sv.x * w.y + sv.y * w.x;
-        ph += g;
+    T[(o * (L + 1) + m1) * n + g] = acc;
+  }
+  __syncthreads();
+  // alpha pass: out[a][g] = Re T[0][g] + 2 sum_{m1>0} Re(T[m1][g] e^{i m1 a})
+  for (int e = t; e < n * n; e += bd) {
+    const int a = e / n, g = e % n;
+    for (int o = 0; o < norient; ++o) {
+      const float2* To = T + o * (L + 1) * n;
+      float acc = 0.f;
+      int ph = 0;
+      for (int m1 = 1; m1 <= L; ++m1) {
+        ph += a;
         if (ph >= n) ph -= n;
+        const float2 w = tw[ph];
+        const float2 v = To[m1 * n + g];
+        acc += v.x * w.x - v.y * w.y;
       }
-      T[(o * (L + 1) + m1) * n + g] = acc;
-    }
-    __syncthreads();
-    // alpha pass
-    for (int e = t; e < n * n; e += bd) {
-      const int a = e / n, g = e % n;
-      for (int o = 0; o < norient; ++o) {
-        const float2* To = T + o * (L + 1) * n;
-        float acc = 0.f;
-        int ph = 0;
-        for (int m1 = 1; m1 <= L; ++m1) {
-          ph += a;
-          if (ph >= n) ph -= n;
-          const float2 w = tw[ph];
-          const float2 v = To[m1 * n + g];
-          acc += v.x * w.x - v.y * w.y;
-        }
-        consider(o, a, g, To[g].x + 2.f * acc);
-      }
+      consider(o, a, g, To[g].x + 2.f * acc);
     }
   }
-  // plane top-kCandPerPlane + bound on the rest
-  for (int o = 0; o < norient; ++o) {
-    bool taken = false;
-    PlaneCand out;
-    float fifth = -INFINITY;
-    for (int r = 0; r <= kCandPerPlane; ++r) {
-      float v = taken ? -INFINITY : ct.bv[o];
-      int i = taken ? 0x7fffffff : ct.bi[o];
-      block_argmax(v, i, rv, ri);
-      if (r < kCandPerPlane) {
-        out.val[r] = v;
-        out.idx[r] = i;
-      } else {
-        fifth = v;
-      }
-      if (!taken && i == ct.bi[o] && v == ct.bv[o]) taken = true;
-    }
-    float dmax = ct.dm[o];
-    int dummy = 0;
-    block_argmax(dmax, dummy, rv, ri);
-    out.rest = fmaxf(fifth, dmax);
-    if (t == 0) cand[((long long)p * norient + o) * n + ib] = out;
-  }
+  plane_candidates(ct, norient, cv, ci, cand + ((long long)p * norient) * n + ib, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -1377,28 +1591,36 @@ void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, cons
                      scale_eff, I64, I32);
 }
 
-// rows / columns per round of the two-pass transform (0: direct DFT)
-static int plane_rounds(int n) {
+// two-pass factorisation n = N1 N2 of the power-of-two grids (0: direct DFT)
+static bool fft_shape(int n, int& n1, int& n2) {
   switch (n) {
-    case 16: return 64;
-    case 32: return 32;
-    case 64: return 32;
-    case 128: return 16;
-    case 256: return 16;
-    default: return 0;
+    case 16: n1 = 4; n2 = 4; return true;
+    case 32: n1 = 8; n2 = 4; return true;
+    case 64: n1 = 8; n2 = 8; return true;
+    case 128: n1 = 16; n2 = 8; return true;
+    case 256: n1 = 16; n2 = 16; return true;
+    default: n1 = n2 = 0; return false;
   }
 }
 
 size_t isoft_plane_smem(int n, int Lmax, int use_fft) {
-  const size_t rr = use_fft ? (size_t)plane_rounds(n) : 0;
+  const size_t cand = 48 * sizeof(float) + 32 * sizeof(int);
+  int n1, n2;
+  if (use_fft && fft_shape(n, n1, n2)) {
+    const size_t ls = (size_t)((Lmax + 1) | 1);
+    const int tpf = std::max(n1, n2);
+    const size_t x = n1 == n2 ? 0 : (size_t)(256 / tpf) * (n + n / n1);
+    return sizeof(float2) * (2 * (size_t)n * ls + x) + cand;
+  }
   return sizeof(float2) * ((size_t)n + 2 * (size_t)(Lmax + 1) * (2 * Lmax + 1) +
-                           2 * (size_t)(Lmax + 1) * n + rr * n) +
-         16 * sizeof(float) + 16 * sizeof(int);
+                           2 * (size_t)(Lmax + 1) * n) +
+         cand;
 }
 
 // 1 if the two-pass transform applies to n and fits max_lds, else 0.
 int isoft_plane_use_fft(int n, int Lmax, size_t max_lds) {
-  return plane_rounds(n) > 0 && isoft_plane_smem(n, Lmax, 1) <= max_lds ? 1 : 0;
+  int n1, n2;
+  return fft_shape(n, n1, n2) && isoft_plane_smem(n, Lmax, 1) <= max_lds ? 1 : 0;
 }
 
 void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long long Kt,
@@ -1406,23 +1628,28 @@ void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long
                         const float2* IA, const float2* IB, int mode, int norient,
                         PlaneCand* cand, float* grid) {
   const size_t sm = isoft_plane_smem(n, Lmax, use_fft);
-  const dim3 g(n, P), b(256);
-#define FO_PLANE(N1, N2)                                                                        \
-  hipLaunchKernelGGL((k_isoft_plane<N1, N2>), g, b, sm, s, pd, n, Kt, tab32, tw32, IA, IB, mode, \
-                     norient, cand, grid)
-  if (!use_fft) {
-    FO_PLANE(0, 0);
+  int n1, n2;
+  if (!use_fft || !fft_shape(n, n1, n2)) {
+    hipLaunchKernelGGL(k_isoft_plane, dim3(n, P), dim3(256), sm, s, pd, n, Kt, tab32, tw32, IA,
+                       IB, mode, norient, cand, grid);
     return;
   }
+  const dim3 g((unsigned)(8 * ((P + 7) / 8) * n)), b(256);
+#define FO_FFT(N1, N2)                                                                         \
+  if (mode == 0)                                                                                \
+    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 0>), g, b, sm, s, pd, P, Kt, tab32, tw32, IA, IB,   \
+                       norient, cand, grid);                                                    \
+  else                                                                                          \
+    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 1>), g, b, sm, s, pd, P, Kt, tab32, tw32, IA, IB,   \
+                       norient, cand, grid)
   switch (n) {
-    case 16: FO_PLANE(4, 4); break;
-    case 32: FO_PLANE(8, 4); break;
-    case 64: FO_PLANE(8, 8); break;
-    case 128: FO_PLANE(16, 8); break;
-    case 256: FO_PLANE(16, 16); break;
-    default: FO_PLANE(0, 0); break;
+    case 16: FO_FFT(4, 4); break;
+    case 32: FO_FFT(8, 4); break;
+    case 64: FO_FFT(8, 8); break;
+    case 128: FO_FFT(16, 8); break;
+    default: FO_FFT(16, 16); break;
   }
-#undef FO_PLANE
+#undef FO_FFT
 }
 
 void launch_sph_select(hipStream_t s, int P, int n, long long Kt, int norient, const SphPair* pd,
