@@ -429,6 +429,171 @@ FO_DEV void contract_body(const SphPair& d, int l, int b, const SphGroup* __rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// MFMA form of the contraction (v_mfma_f64_16x16x4f64).  One workgroup per
+// (pair, l, block of kMB rows m1 >= 0); per tile of kCT atoms k of
+// structure 2:
+//   A[rr][k] = sum_j Y1'[rr][j] K_l[j][k]   real rows rr = 2r + {Re, Im} of
+//              row m1a + r; wave w computes rows 16w..16w+15, both 16-wide
+//              column tiles; operands stream from L2
+//   A -> LDS (complex [row][k])
+//   C[r][m] += sum_k A[r][k] conj(Y2[m][k]) as four real products
+//              (Cr += Ar Y2r + Ai Y2i, Ci += Ai Y2r - Ar Y2i); complex 16x16
+//              tiles are dealt to the waves round-robin.
+// MFMA lane layout: A operand lane -> (row lane%16, k lane/16), B operand
+// lane -> (k lane/16, col lane%16), result register i of a lane -> (row
+// 4 (lane/16) + i, col lane%16).  The reduction order of every output is
+// fixed, so results do not depend on the batch a pair is in.
+// ---------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+FO_DEV v4d mfma64(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int kCT = 32;  // atoms of structure 2 per tile (MFMA form)
+
+template <int NCT>
+FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* __restrict__ grp,
+                               const double2* __restrict__ Y1, const double2* __restrict__ Y2,
+                               const double* __restrict__ R, double sc,
+                               double2* __restrict__ I64, float2* __restrict__ I32,
+                               double2 (*Ac)[kCT + 1]) {
+  constexpr int QW = (2 * NCT + 3) / 4;  // complex output tiles per wave
+  const int M = 2 * l + 1;
+  const int m1a = b * kMB;
+  const int rows = min(kMB, l + 1 - m1a);
+  const int nrt = (rows + 15) >> 4;
+  const int t = threadIdx.x, w = t >> 6, ln = t & 63, col = ln & 15, kq = ln >> 4;
+  v4d cr[QW], ci[QW];
+#pragma unroll
+  for (int q = 0; q < QW; ++q) {
+    cr[q] = v4d{0.0, 0.0, 0.0, 0.0};
+    ci[q] = v4d{0.0, 0.0, 0.0, 0.0};
+  }
+  const int Ng = d.ngath;
+  const double2* y1 = Y1 + d.ylm_off + (long long)(l * l + l + m1a) * Ng;
+  const double2* y2 = Y2 + d.ylm_off + (long long)(l * l) * Ng;
+  const int rr = 16 * w + col, ar = rr >> 1, apart = rr & 1;
+  const bool awave = 16 * w < 2 * rows;  // wave-uniform
+  const bool arow = ar < rows;
+  for (int g = 0; g < d.ngroups; ++g) {
+    const SphGroup G = grp[d.group_off + g];
+    const int Ns = G.size;
+    const double* Rl = R + d.rad_off + G.rad_off + (long long)l * Ns * Ns;
+    const double2* y1g = y1 + G.gath_off + (long long)ar * Ng;
+    const double2* y2g = y2 + G.gath_off;
+    for (int k0 = 0; k0 < Ns; k0 += kCT) {
+      v4d a0 = v4d{0.0, 0.0, 0.0, 0.0}, a1 = a0;
+      if (awave) {
+        const int c0 = k0 + col, c1 = k0 + 16 + col;
+        for (int j0 = 0; j0 < Ns; j0 += 16) {
+          double av[4], b0[4], b1[4];
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int j = j0 + 4 * s + kq;
+            const bool jok = j < Ns;
+            const double2 y = (arow && jok) ? y1g[j] : make_double2(0.0, 0.0);
+            av[s] = apart ? y.y : y.x;
+            b0[s] = (jok && c0 < Ns) ? Rl[(long long)j * Ns + c0] : 0.0;
+            b1[s] = (jok && c1 < Ns) ? Rl[(long long)j * Ns + c1] : 0.0;
+          }
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            a0 = mfma64(av[s], b0[s], a0);
+            a1 = mfma64(av[s], b1[s], a1);
+          }
+        }
+      }
+      __syncthreads();  // readers of the previous tile are done
+      if (awave) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r2 = 16 * w + 4 * kq + i;
+          double* e0 = (double*)&Ac[r2 >> 1][col];
+          double* e1 = (double*)&Ac[r2 >> 1][16 + col];
+          e0[r2 & 1] = a0[i];
+          e1[r2 & 1] = a1[i];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int qq = 0; qq < QW; ++qq) {
+        const int q = w + 4 * qq;
+        const int rt = q / NCT, ct = q % NCT;
+        if (rt >= nrt) continue;  // wave-uniform
+        const int r = 16 * rt + col, m = 16 * ct + col;
+        const bool rok = r < rows, mok = m < M;
+        const double2* y2m = y2g + (long long)m * Ng;
+#pragma unroll
+        for (int s = 0; s < kCT / 4; ++s) {
+          const int kk = 4 * s + kq, ka = k0 + kk;
+          const double2 a = rok ? Ac[r][kk] : make_double2(0.0, 0.0);
+          const double2 y = (mok && ka < Ns) ? y2m[ka] : make_double2(0.0, 0.0);
+          cr[qq] = mfma64(a.x, y.x, cr[qq]);
+          cr[qq] = mfma64(a.y, y.y, cr[qq]);
+          ci[qq] = mfma64(a.y, y.x, ci[qq]);
+          ci[qq] = mfma64(-a.x, y.y, ci[qq]);
+        }
+      }
+    }
+  }
+  const double norm = 4.0 * pow(kPi, 2.5) * sc * sc * sc;
+#pragma unroll
+  for (int qq = 0; qq < QW; ++qq) {
+    const int q = w + 4 * qq;
+    const int rt = q / NCT, ct = q % NCT;
+    if (rt >= nrt) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 16 * rt + 4 * kq + i, m = 16 * ct + col;
+      if (r >= rows || m >= M) continue;
+      const int m1 = m1a + r, m2 = m - l;
+      const long long id = d.coef_off + kidx(l, m1, m2);
+      const double2 v = make_double2(norm * cr[qq][i], norm * ci[qq][i]);
+      I64[id] = v;
+      I32[id] = make_float2((float)v.x, (float)v.y);
+      if (m1 > 0) {
+        const long long im = d.coef_off + kidx(l, -m1, -m2);
+        const double sg = ((m1 + m2) & 1) ? -1.0 : 1.0;
+        const double2 u = make_double2(sg * v.x, -sg * v.y);
+        I64[im] = u;
+        I32[im] = make_float2((float)u.x, (float)u.y);
+      }
+    }
+  }
+}
+
+// NCTMAX: column tiles of the largest degree in the launch (2l+1 <= 16 NCTMAX).
+template <int NCTMAX>
+__global__ void __launch_bounds__(256) k_sph_contract_mfma(
+    const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
+    const double2* __restrict__ Y1, const double2* __restrict__ Y2,
+    const double* __restrict__ R, const double* __restrict__ scale_eff,
+    double2* __restrict__ I64, float2* __restrict__ I32) {
+  __shared__ double2 Ac[kMB][kCT + 1];
+  const SphPair d = pd[blockIdx.y];
+  const int L = d.L;
+  int b = blockIdx.x, l = 0;
+  for (; l <= L; ++l) {
+    const int nb = (l + 1 + kMB - 1) / kMB;
+    if (b < nb) break;
+    b -= nb;
+  }
+  if (l > L) return;
+  const double sc = scale_eff[blockIdx.y];
+  const int nct = (2 * l + 1 + 15) >> 4;
+#define FO_CB(K) contract_mfma_body<K>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, Ac)
+  if constexpr (NCTMAX <= 2) {
+    if (nct <= 1) FO_CB(1); else FO_CB(2);
+  } else if constexpr (NCTMAX <= 4) {
+    if (nct <= 2) FO_CB(2); else FO_CB(4);
+  } else {
+    if (nct <= 2) FO_CB(2); else if (nct <= 4) FO_CB(4); else FO_CB(8);
+  }
+#undef FO_CB
+}
+
 __global__ void __launch_bounds__(256) k_sph_contract(
     const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
     const double2* __restrict__ Y1, const double2* __restrict__ Y2,
@@ -1584,11 +1749,13 @@ void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, cons
                          const double* scale_eff, double2* I64, float2* I32) {
   int nb = 0;
   for (int l = 0; l <= Lmax; ++l) nb += (l + 1 + kMB - 1) / kMB;
-  const size_t smem = sizeof(double2) * ((size_t)kMB * kJT + (size_t)kMB * kKT +
-                                         (size_t)kKT * (2 * Lmax + 1)) +
-                      sizeof(double) * (size_t)kJT * kKT;
-  hipLaunchKernelGGL(k_sph_contract, dim3(nb, P), dim3(256), smem, s, pd, grp, Y1, Y2, R,
-                     scale_eff, I64, I32);
+  const dim3 g(nb, P), b(256);
+  if (Lmax <= 15)
+    hipLaunchKernelGGL(k_sph_contract_mfma<2>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64, I32);
+  else if (Lmax <= 31)
+    hipLaunchKernelGGL(k_sph_contract_mfma<4>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64, I32);
+  else
+    hipLaunchKernelGGL(k_sph_contract_mfma<8>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64, I32);
 }
 
 // two-pass factorisation n = N1 N2 of the power-of-two grids (0: direct DFT)
