@@ -442,7 +442,7 @@ FO_DEV void contract_body(const SphPair& d, int l, int b, const SphGroup* __rest
 //              tiles are dealt to the waves round-robin.
 // MFMA lane layout: A operand lane -> (row lane%16, k lane/16), B operand
 // lane -> (k lane/16, col lane%16), result register i of a lane -> (row
-// 4 (lane/16) + i, col lane%16).  The reduction order of every output is
+// 4 i + lane/16, col lane%16).  The reduction order of every output is
 // fixed, so results do not depend on the batch a pair is in.
 // ---------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -509,7 +509,7 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
       if (awave) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int r2 = 16 * w + 4 * kq + i;
+          const int r2 = 16 * w + 4 * i + kq;
           double* e0 = (double*)&Ac[r2 >> 1][col];
           double* e1 = (double*)&Ac[r2 >> 1][16 + col];
           e0[r2 & 1] = a0[i];
@@ -546,7 +546,7 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
     if (rt >= nrt) continue;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int r = 16 * rt + 4 * kq + i, m = 16 * ct + col;
+      const int r = 16 * rt + 4 * i + kq, m = 16 * ct + col;
       if (r >= rows || m >= M) continue;
       const int m1 = m1a + r, m2 = m - l;
       const long long id = d.coef_off + kidx(l, m1, m2);
