@@ -1177,6 +1177,72 @@ FO_DEV double grid_eval64(const GridCtx& c, int ia, int ib, int ig, double* red)
   return block_sum(acc, red);
 }
 
+// FP64 grid values at up to kEvalBatch points whose beta indices take at most
+// kEvalBeta distinct values, in one sweep over the coefficients (each
+// coefficient is read once for all points).  Point q: alpha pa[q], gamma
+// pg[q], beta slot pb[q] into betas[].  Each point's value is reduced with
+// the tree of block_sum.
+constexpr int kEvalBatch = 8;
+constexpr int kEvalBeta = 4;
+
+FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int* pg,
+                              const int* pb, int nbeta, const int* betas, double* out,
+                              double* red) {
+  const int L = c.L, W = 2 * L + 1, n = c.n;
+  const double2* I = c.IB ? c.IB : c.I;
+  const double sg = c.IB ? 1.0 : c.sgn;
+  const double* tb[kEvalBeta];
+#pragma unroll
+  for (int b = 0; b < kEvalBeta; ++b) tb[b] = c.tab64 + (long long)betas[b < nbeta ? b : 0] * c.Kt;
+  double acc[kEvalBatch];
+#pragma unroll
+  for (int q = 0; q < kEvalBatch; ++q) acc[q] = 0.0;
+  for (int e = threadIdx.x; e < (L + 1) * W; e += blockDim.x) {
+    const int m1 = e / W, m2 = e % W - L;
+    const int l0 = max(m1, abs(m2));
+    double sx[kEvalBeta], sy[kEvalBeta];
+#pragma unroll
+    for (int b = 0; b < kEvalBeta; ++b) sx[b] = sy[b] = 0.0;
+    double sl = (l0 & 1) ? sg : 1.0;
+    int id = kidx(l0, m1, m2);
+    for (int l = l0; l <= L; ++l) {
+      const double2 Iv = I[id];
+#pragma unroll
+      for (int b = 0; b < kEvalBeta; ++b) {
+        if (b < nbeta) {
+          const double dv = tb[b][id] * sl;
+          sx[b] += dv * Iv.x;
+          sy[b] += dv * Iv.y;
+        }
+      }
+      sl *= sg;
+      id += (2 * l + 1) * (2 * l + 1) + 2 * m1 + 4 * l + 4;  // kidx(l+1, m1, m2)
+    }
+    const double wm = m1 == 0 ? 1.0 : 2.0;
+#pragma unroll
+    for (int q = 0; q < kEvalBatch; ++q) {
+      if (q < np) {
+        const int b = pb[q];
+        double x = sx[0], y = sy[0];
+#pragma unroll
+        for (int bb = 1; bb < kEvalBeta; ++bb)
+          if (b == bb) {
+            x = sx[bb];
+            y = sy[bb];
+          }
+        int ph = (int)(((long long)m1 * pa[q] + (long long)m2 * pg[q]) % n);
+        if (ph < 0) ph += n;
+        const double2 w = c.tw64[ph];
+        acc[q] += wm * (x * w.x - y * w.y);
+      }
+    }
+  }
+  block_sum_n<kEvalBatch>(acc, red);
+#pragma unroll
+  for (int q = 0; q < kEvalBatch; ++q)
+    if (q < np) out[q] = acc[q];
+}
+
 // Full FP64 evaluation of one beta plane; returns its argmax (ties -> lower
 // flat index).  Uses global scratch Sg[(L+1)W], Tg[(L+1)n].
 FO_DEV void plane_eval64(const GridCtx& c, int ib, double2* Sg, double2* Tg, double* red,
@@ -1245,11 +1311,14 @@ __global__ void __launch_bounds__(256) k_sph_select(
     const double2* __restrict__ tw64, const double2* __restrict__ IA,
     const double2* __restrict__ IB, int mode, int norient, const PlaneCand* __restrict__ cand,
     double2* __restrict__ scratch, long long scratch_stride, SphPeak* __restrict__ peaks) {
-  __shared__ double red[16];
+  __shared__ double red[16 * kEvalBatch];
   __shared__ int redi[16];
   __shared__ int s_cnt, s_ovf;
   __shared__ int s_cand[kMaxCand];
   __shared__ int s_ovfl[256];
+  __shared__ int s_pa[kEvalBatch], s_pg[kEvalBatch], s_pb[kEvalBatch], s_beta[kEvalBeta];
+  __shared__ int s_np, s_nb;
+  __shared__ double s_val[kEvalBatch];
   const int p = blockIdx.x, o = blockIdx.y;
   const SphPair d = pd[p];
   const int L = d.L, t = threadIdx.x;
@@ -1272,18 +1341,22 @@ __global__ void __launch_bounds__(256) k_sph_select(
     __shared__ int ri[16];
     block_argmax(mv, mi, rv, ri);
   }
-  // error bound
+  // error bound; Hermitian sets: |I[-m1,-m2]| = |I[m1,m2]|, so the m1 > 0
+  // rows count twice and the m1 < 0 rows are not read
   const double2* Iuse = c.IB ? c.IB : c.I;
   double A = 0.0;
-  const long long K = kcount(L);
-  for (long long e = t; e < K; e += blockDim.x) {
-    int l = 0;
-    while (kbase(l + 1) <= e) ++l;
-    A += sqrt((2.0 * l + 1.0) * 0.5) * hypot(Iuse[e].x, Iuse[e].y);
+  for (int l = 0; l <= L; ++l) {
+    const int W = 2 * l + 1;
+    const double f = sqrt((2.0 * l + 1.0) * 0.5);
+    const double2* row0 = Iuse + kidx(l, 0, -l);
+    for (int e = t; e < (l + 1) * W; e += blockDim.x)
+      A += (e < W ? f : 2.0 * f) * hypot(row0[e].x, row0[e].y);
   }
   A = block_sum(A, red);
   const double delta = (8.0 * (L + 1) + 32.0) * 5.9604644775390625e-08 * A;
-  const double thr = (double)mv - delta;
+  // |g32 - g64| <= delta everywhere, so the FP64 maximum has an FP32 value
+  // of at least (FP32 maximum) - 2 delta
+  const double thr = (double)mv - 2.0 * delta;
   if (t == 0) {
     s_cnt = 0;
     s_ovf = 0;
@@ -1343,17 +1416,47 @@ __global__ void __launch_bounds__(256) k_sph_select(
       }
     }
   } else {
-    for (int q = 0; q < ncand; ++q) {
-      const int f = s_cand[q];
-      const int ia = f / (n * n), ib = (f / n) % n, ig = f % n;
-      const double v = grid_eval64(c, ia, ib, ig, red);
-      if (v > best || (v == best && f < besti)) {
-        second = best;
-        best = v;
-        besti = f;
-      } else if (v > second) {
-        second = v;
+    // listed candidates in passes of <= kEvalBatch points with <= kEvalBeta
+    // distinct beta planes (s_cand is sorted by flat index)
+    for (int q0 = 0; q0 < ncand;) {
+      if (t == 0) {
+        int np = 0, nb = 0, q = q0;
+        for (; q < ncand && np < kEvalBatch; ++q) {
+          const int f = s_cand[q];
+          const int ib = (f / n) % n;
+          int slot = -1;
+          for (int b = 0; b < nb; ++b)
+            if (s_beta[b] == ib) slot = b;
+          if (slot < 0) {
+            if (nb == kEvalBeta) break;
+            slot = nb;
+            s_beta[nb++] = ib;
+          }
+          s_pa[np] = f / (n * n);
+          s_pg[np] = f % n;
+          s_pb[np] = slot;
+          ++np;
+        }
+        s_np = np;
+        s_nb = nb;
       }
+      __syncthreads();
+      const int np = s_np;
+      grid_eval64_batch(c, np, s_pa, s_pg, s_pb, s_nb, s_beta, s_val, red);
+      __syncthreads();
+      for (int q = 0; q < np; ++q) {
+        const int f = s_cand[q0 + q];
+        const double v = s_val[q];
+        if (v > best || (v == best && f < besti)) {
+          second = best;
+          best = v;
+          besti = f;
+        } else if (v > second) {
+          second = v;
+        }
+      }
+      q0 += np;
+      __syncthreads();
     }
     for (int q = 0; q < novf; ++q) {
       double v;
@@ -1368,15 +1471,31 @@ __global__ void __launch_bounds__(256) k_sph_select(
       }
     }
   }
-  // parabolic offsets on |g| with periodic wrap (fastoverlap/utils.py:319-338)
+  // parabolic offsets on |g| with periodic wrap (fastoverlap/utils.py:319-338);
+  // the six neighbours are evaluated in one sweep
   const int ind[3] = {besti / (n * n), (besti / n) % n, besti % n};
+  if (t == 0) {
+    const int bm = (ind[1] + n - 1) % n, bp = (ind[1] + 1) % n;
+    s_beta[0] = ind[1];
+    s_beta[1] = bp;
+    s_beta[2] = bm;
+    s_nb = 3;
+    for (int ax = 0; ax < 3; ++ax)
+      for (int sd = 0; sd < 2; ++sd) {
+        const int q = 2 * ax + sd;  // sd 0: +1, 1: -1
+        const int dlt = sd == 0 ? 1 : n - 1;
+        s_pa[q] = ax == 0 ? (ind[0] + dlt) % n : ind[0];
+        s_pg[q] = ax == 2 ? (ind[2] + dlt) % n : ind[2];
+        s_pb[q] = ax == 1 ? (sd == 0 ? 1 : 2) : 0;
+      }
+  }
+  __syncthreads();
+  grid_eval64_batch(c, 6, s_pa, s_pg, s_pb, s_nb, s_beta, s_val, red);
+  __syncthreads();
   double off[3];
   for (int ax = 0; ax < 3; ++ax) {
-    int ip[3] = {ind[0], ind[1], ind[2]}, im[3] = {ind[0], ind[1], ind[2]};
-    ip[ax] = (ind[ax] + 1) % n;
-    im[ax] = (ind[ax] + n - 1) % n;
-    const double y1 = fabs(grid_eval64(c, ip[0], ip[1], ip[2], red));
-    const double y3 = fabs(grid_eval64(c, im[0], im[1], im[2], red));
+    const double y1 = fabs(s_val[2 * ax]);
+    const double y3 = fabs(s_val[2 * ax + 1]);
     const double y2 = fabs(best);
     off[ax] = (y3 - y1) / (2.0 * (2.0 * y2 - y1 - y3));
   }
