@@ -156,63 +156,78 @@ __global__ void k_sph_prep(const SphPair* __restrict__ pd, const double* __restr
 }
 
 // ---------------------------------------------------------------------------
-// Y_l^m (orthonormal, Condon-Shortley phase: scipy's sph_harm as called at
-// fastoverlap/sphericalAlignment.py:57-65), FP64, via the normalised
-// associated-Legendre recurrence.  Layout: Y[ylm_off + (l*l+l+m)*ngath + g].
-// An atom at the centre takes theta = phi = 0 (only l = 0 survives there).
+// Spherical harmonics of every grouped atom (fastoverlap/utils.py:296-302,
+// fastoverlap/sphericalAlignment.py:57-65), FP64, stored factorised:
+//   Y_l^m = P[q(l,m)][g] E[m][g],  q(l,m) = l(l+1)/2 + m,  m >= 0,
+// with P the normalised associated Legendre function (Condon-Shortley
+// phase) and E = e^{i m phi}; Y_l^{-m} = (-1)^m conj(Y_l^m).  One thread per
+// (atom, m): P_m^m = c_m sin^m(theta), then the three-term recurrence in l
+// with tabulated coefficients (k_leg_table).  An atom at the centre takes
+// theta = phi = 0 (only l = 0 survives there).
 // ---------------------------------------------------------------------------
+FO_HD inline int qlm(int l, int m) { return l * (l + 1) / 2 + m; }
+
 __global__ void k_sph_ylm(const SphPair* __restrict__ pd, const int* __restrict__ gath,
                           const double* __restrict__ X1c, const double* __restrict__ X2c,
-                          double2* __restrict__ Y1, double2* __restrict__ Y2) {
+                          const double2* __restrict__ leg, const double* __restrict__ cm,
+                          double* __restrict__ P1, double* __restrict__ P2,
+                          double2* __restrict__ E1, double2* __restrict__ E2) {
   const SphPair d = pd[blockIdx.x];
   const double* X = blockIdx.y ? X2c : X1c;
-  double2* Y = (blockIdx.y ? Y2 : Y1) + d.ylm_off;
+  double* Pp = (blockIdx.y ? P2 : P1) + d.ylm_off;
+  double2* Ep = (blockIdx.y ? E2 : E1) + d.eph_off;
   const int L = d.L, Ng = d.ngath;
-  const double inv4pi = 1.0 / (4.0 * kPi);
-  for (int g = threadIdx.x; g < Ng; g += blockDim.x) {
+  for (int w = threadIdx.x; w < Ng * (L + 1); w += blockDim.x) {
+    const int m = w / Ng, g = w % Ng;
     const int a = gath[d.gath_off + g];
     const double x = X[3LL * (d.atom_off + a)], y = X[3LL * (d.atom_off + a) + 1],
                  z = X[3LL * (d.atom_off + a) + 2];
     const double rxy = sqrt(x * x + y * y), r = sqrt(x * x + y * y + z * z);
-    double ct = 1.0, st = 0.0, cp = 1.0, sp = 0.0;
+    double ct = 1.0, st = 0.0, phi = 0.0;
     if (r > 0.0) {
       ct = z / r;
       st = rxy / r;
-      const double phi = atan2(y, x);
-      sincos(phi, &sp, &cp);
+      phi = atan2(y, x);
     }
-    double pmm = sqrt(inv4pi);  // P^0_0
-    double2 em = make_double2(1.0, 0.0);  // e^{i m phi}
-    for (int m = 0; m <= L; ++m) {
-      if (m > 0) {
-        pmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m)) * st;
-        em = cmul(em, make_double2(cp, sp));
-      }
-      double p2 = 0.0, p1 = pmm;
-      for (int l = m; l <= L; ++l) {
-        double pl;
-        if (l == m) {
-          pl = pmm;
-        } else if (l == m + 1) {
-          pl = sqrt(2.0 * m + 3.0) * ct * pmm;
-        } else {
-          const double a = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
-          const double b = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) /
-                                (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
-          pl = a * (ct * p1 - b * p2);
-        }
-        if (l > m) {
-          p2 = p1;
-          p1 = pl;
-        }
-        const double2 yp = cscale(em, pl);
-        Y[(long long)(l * l + l + m) * Ng + g] = yp;
-        if (m > 0) {
-          const double sg = (m & 1) ? -1.0 : 1.0;
-          Y[(long long)(l * l + l - m) * Ng + g] = make_double2(sg * yp.x, -sg * yp.y);
-        }
-      }
+    double sp, cp;
+    sincos((double)m * phi, &sp, &cp);
+    Ep[(long long)m * Ng + g] = make_double2(cp, sp);
+    double p2 = 0.0, p1 = cm[m] * ipow(st, m);
+    Pp[(long long)qlm(m, m) * Ng + g] = p1;
+    for (int l = m + 1; l <= L; ++l) {
+      const double2 ab = leg[qlm(l, m)];
+      const double pl = ab.x * (ct * p1 - ab.y * p2);
+      Pp[(long long)qlm(l, m) * Ng + g] = pl;
+      p2 = p1;
+      p1 = pl;
     }
+  }
+}
+
+// Legendre recurrence coefficients P_l^m = a (cos(theta) P_{l-1}^m - b P_{l-2}^m):
+// leg[q(l,m)] = (a, b) with a = sqrt((4l^2-1)/(l^2-m^2)),
+// b = sqrt(((l-1)^2-m^2)/(4(l-1)^2-1)) for l >= m+2, (sqrt(2m+3), 0) for
+// l = m+1; cm[m] = (-1)^m sqrt(1/(4 pi)) prod_{k<=m} sqrt((2k+1)/(2k)).
+__global__ void k_leg_table(int Lt, double2* __restrict__ leg, double* __restrict__ cm) {
+  const int nq = qlm(Lt, Lt) + 1;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nq; e += gridDim.x * blockDim.x) {
+    int l = 0;
+    while (qlm(l + 1, 0) <= e) ++l;
+    const int m = e - qlm(l, 0);
+    double2 v = make_double2(0.0, 0.0);
+    if (l == m + 1) {
+      v.x = sqrt(2.0 * m + 3.0);
+    } else if (l >= m + 2) {
+      const double dl = l, dm = m;
+      v.x = sqrt((4.0 * dl * dl - 1.0) / (dl * dl - dm * dm));
+      v.y = sqrt(((dl - 1.0) * (dl - 1.0) - dm * dm) / (4.0 * (dl - 1.0) * (dl - 1.0) - 1.0));
+    }
+    leg[e] = v;
+  }
+  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m <= Lt; m += gridDim.x * blockDim.x) {
+    double c = sqrt(1.0 / (4.0 * kPi));
+    for (int k = 1; k <= m; ++k) c *= -sqrt((2.0 * k + 1.0) / (2.0 * k));
+    cm[m] = c;
   }
 }
 
@@ -305,129 +320,8 @@ __global__ void k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __r
 // Only rows m1 >= 0 are contracted: Y_l^{-m} = (-1)^m conj(Y_l^m) and K_l
 // is real, so I^l_{-m1,-m2} = (-1)^{m1+m2} conj(I^l_{m1m2}) and the m1 < 0
 // rows are stored from the computed ones.
-// One workgroup (256 threads) per (pair, l, block of kMB rows m1 >= 0), an
-// LDS tiled FP64 GEMM chain.  Per tile of kKT atoms k of structure 2:
-//   A[r][k]  = sum_j Y1[m1a+r][j] K_l[j][k]   (j in tiles of kJT; 2x2
-//              complex-by-real register tile per thread, A -> LDS)
-//   acc[r][m2] += sum_k A[r][k] conj(Y2[m2][k])  (4x4 complex register tile;
-//              Y2 staged transposed so lanes read consecutive m2)
-// Each output is reduced in the same order for any batch, so results do
-// not depend on the batch a pair is in.
 // ---------------------------------------------------------------------------
 constexpr int kMB = 32;   // m1 rows per workgroup
-constexpr int kKT = 32;   // atoms of structure 2 per tile
-constexpr int kJT = 32;   // atoms of structure 1 per step
-
-// NQ = number of 32-wide m2 column groups of the second GEMM (M <= 32 NQ).
-template <int NQ>
-FO_DEV void contract_body(const SphPair& d, int l, int b, const SphGroup* __restrict__ grp,
-                          const double2* __restrict__ Y1, const double2* __restrict__ Y2,
-                          const double* __restrict__ R, double sc, double2* __restrict__ I64,
-                          float2* __restrict__ I32, unsigned char* smem) {
-  const int M = 2 * l + 1;
-  const int m1a = b * kMB;  // first m1 (>= 0) of this workgroup
-  const int rows = min(kMB, l + 1 - m1a);
-  double2* Y1s = (double2*)smem;                 // [kMB][kJT]
-  double* Ks = (double*)(Y1s + kMB * kJT);       // [kJT][kKT]
-  double2* As = (double2*)(Ks + kJT * kKT);      // [kMB][kKT]
-  double2* Y2t = As + kMB * kKT;                 // [kKT][M] (transposed)
-  const int t = threadIdx.x;
-  // phase A thread tile: rows {ar, ar+16}, cols {2ac, 2ac+1}
-  const int ar = t >> 4, ac = t & 15;
-  // phase B thread tile: rows {br + 8i}, cols {bc + 32q}, i < 4, q < NQ
-  const int br = t >> 5, bc = t & 31;
-  double2 acc[4][NQ];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[i][q] = make_double2(0.0, 0.0);
-  const int Ng = d.ngath;
-  const double2* y1 = Y1 + d.ylm_off + (long long)(l * l) * Ng;
-  const double2* y2 = Y2 + d.ylm_off + (long long)(l * l) * Ng;
-  for (int g = 0; g < d.ngroups; ++g) {
-    const SphGroup G = grp[d.group_off + g];
-    const int Ns = G.size;
-    const double* Rl = R + d.rad_off + G.rad_off + (long long)l * Ns * Ns;
-    for (int k0 = 0; k0 < Ns; k0 += kKT) {
-      const int kt = min(kKT, Ns - k0);
-      __syncthreads();  // previous tile's readers of Y2t / As are done
-      for (int e = t; e < kKT * M; e += blockDim.x) {
-        const int kk = e / M, m2 = e % M;
-        Y2t[e] = kk < kt ? y2[(long long)m2 * Ng + G.gath_off + k0 + kk] : make_double2(0.0, 0.0);
-      }
-      double2 a00 = make_double2(0.0, 0.0), a01 = a00, a10 = a00, a11 = a00;
-      for (int j0 = 0; j0 < Ns; j0 += kJT) {
-        const int jt = min(kJT, Ns - j0);
-        __syncthreads();
-        for (int e = t; e < kMB * kJT; e += blockDim.x) {
-          const int r = e / kJT, jj = e % kJT;
-          Y1s[e] = (r < rows && jj < jt) ? y1[(long long)(l + m1a + r) * Ng + G.gath_off + j0 + jj]
-                                         : make_double2(0.0, 0.0);
-        }
-        for (int e = t; e < kJT * kKT; e += blockDim.x) {
-          const int jj = e / kKT, kk = e % kKT;
-          Ks[e] = (jj < jt && kk < kt) ? Rl[(long long)(j0 + jj) * Ns + k0 + kk] : 0.0;
-        }
-        __syncthreads();
-        for (int jj = 0; jj < jt; ++jj) {
-          const double2 y0 = Y1s[ar * kJT + jj], yb = Y1s[(ar + 16) * kJT + jj];
-          const double k0v = Ks[jj * kKT + 2 * ac], k1v = Ks[jj * kKT + 2 * ac + 1];
-          a00.x += y0.x * k0v;
-          a00.y += y0.y * k0v;
-          a01.x += y0.x * k1v;
-          a01.y += y0.y * k1v;
-          a10.x += yb.x * k0v;
-          a10.y += yb.y * k0v;
-          a11.x += yb.x * k1v;
-          a11.y += yb.y * k1v;
-        }
-      }
-      As[ar * kKT + 2 * ac] = a00;
-      As[ar * kKT + 2 * ac + 1] = a01;
-      As[(ar + 16) * kKT + 2 * ac] = a10;
-      As[(ar + 16) * kKT + 2 * ac + 1] = a11;
-      __syncthreads();
-      for (int kk = 0; kk < kt; ++kk) {
-        double2 av[4], yv[NQ];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) av[i] = As[(br + 8 * i) * kKT + kk];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const int m2 = bc + 32 * q;
-          yv[q] = m2 < M ? Y2t[kk * M + m2] : make_double2(0.0, 0.0);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            // a * conj(y)
-            acc[i][q].x += av[i].x * yv[q].x + av[i].y * yv[q].y;
-            acc[i][q].y += av[i].y * yv[q].x - av[i].x * yv[q].y;
-          }
-      }
-    }
-  }
-  const double norm = 4.0 * pow(kPi, 2.5) * sc * sc * sc;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int r = br + 8 * i, m2i = bc + 32 * q;
-      if (r >= rows || m2i >= M) continue;
-      const int m1 = m1a + r, m2 = m2i - l;
-      const long long id = d.coef_off + kidx(l, m1, m2);
-      const double2 v = cscale(acc[i][q], norm);
-      I64[id] = v;
-      I32[id] = make_float2((float)v.x, (float)v.y);
-      if (m1 > 0) {
-        const long long im = d.coef_off + kidx(l, -m1, -m2);
-        const double sg = ((m1 + m2) & 1) ? -1.0 : 1.0;
-        const double2 w = make_double2(sg * v.x, -sg * v.y);
-        I64[im] = w;
-        I32[im] = make_float2((float)w.x, (float)w.y);
-      }
-    }
-}
 
 // ---------------------------------------------------------------------------
 // MFMA form of the contraction (v_mfma_f64_16x16x4f64).  One workgroup per
@@ -455,7 +349,8 @@ constexpr int kCT = 32;  // atoms of structure 2 per tile (MFMA form)
 
 template <int NCT>
 FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* __restrict__ grp,
-                               const double2* __restrict__ Y1, const double2* __restrict__ Y2,
+                               const double* __restrict__ P1, const double* __restrict__ P2,
+                               const double2* __restrict__ E1, const double2* __restrict__ E2,
                                const double* __restrict__ R, double sc,
                                double2* __restrict__ I64, float2* __restrict__ I32,
                                double2 (*Ac)[kCT + 1]) {
@@ -472,17 +367,19 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
     ci[q] = v4d{0.0, 0.0, 0.0, 0.0};
   }
   const int Ng = d.ngath;
-  const double2* y1 = Y1 + d.ylm_off + (long long)(l * l + l + m1a) * Ng;
-  const double2* y2 = Y2 + d.ylm_off + (long long)(l * l) * Ng;
   const int rr = 16 * w + col, ar = rr >> 1, apart = rr & 1;
   const bool awave = 16 * w < 2 * rows;  // wave-uniform
   const bool arow = ar < rows;
+  // Y1 row m1a + ar (m >= 0): P1[q(l, m)] E1[m]
+  const int am1 = arow ? m1a + ar : 0;
+  const double* p1r = P1 + d.ylm_off + (long long)qlm(l, am1) * Ng;
+  const double2* e1r = E1 + d.eph_off + (long long)am1 * Ng;
   for (int g = 0; g < d.ngroups; ++g) {
     const SphGroup G = grp[d.group_off + g];
     const int Ns = G.size;
     const double* Rl = R + d.rad_off + G.rad_off + (long long)l * Ns * Ns;
-    const double2* y1g = y1 + G.gath_off + (long long)ar * Ng;
-    const double2* y2g = y2 + G.gath_off;
+    const double* p1g = p1r + G.gath_off;
+    const double2* e1g = e1r + G.gath_off;
     for (int k0 = 0; k0 < Ns; k0 += kCT) {
       v4d a0 = v4d{0.0, 0.0, 0.0, 0.0}, a1 = a0;
       if (awave) {
@@ -493,8 +390,10 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
           for (int s = 0; s < 4; ++s) {
             const int j = j0 + 4 * s + kq;
             const bool jok = j < Ns;
-            const double2 y = (arow && jok) ? y1g[j] : make_double2(0.0, 0.0);
-            av[s] = apart ? y.y : y.x;
+            const bool ok = arow && jok;
+            const double pv = ok ? p1g[j] : 0.0;
+            const double2 ev = ok ? e1g[j] : make_double2(0.0, 0.0);
+            av[s] = pv * (apart ? ev.y : ev.x);
             b0[s] = (jok && c0 < Ns) ? Rl[(long long)j * Ns + c0] : 0.0;
             b1[s] = (jok && c1 < Ns) ? Rl[(long long)j * Ns + c1] : 0.0;
           }
@@ -524,12 +423,21 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
         if (rt >= nrt) continue;  // wave-uniform
         const int r = 16 * rt + col, m = 16 * ct + col;
         const bool rok = r < rows, mok = m < M;
-        const double2* y2m = y2g + (long long)m * Ng;
+        // Y2 row m2 = m - l: Y_l^{|m2|} = P2 E2, conjugated with sign
+        // (-1)^|m2| for m2 < 0
+        const int m2 = mok ? m - l : 0, am2 = abs(m2);
+        const double* p2m = P2 + d.ylm_off + (long long)qlm(l, am2) * Ng + G.gath_off;
+        const double2* e2m = E2 + d.eph_off + (long long)am2 * Ng + G.gath_off;
+        const double ys = m2 < 0 ? ((am2 & 1) ? -1.0 : 1.0) : 1.0;
+        const double yc = m2 < 0 ? -ys : 1.0;
 #pragma unroll
         for (int s = 0; s < kCT / 4; ++s) {
           const int kk = 4 * s + kq, ka = k0 + kk;
           const double2 a = rok ? Ac[r][kk] : make_double2(0.0, 0.0);
-          const double2 y = (mok && ka < Ns) ? y2m[ka] : make_double2(0.0, 0.0);
+          const bool kok = mok && ka < Ns;
+          const double pv = kok ? p2m[ka] : 0.0;
+          const double2 ev = kok ? e2m[ka] : make_double2(0.0, 0.0);
+          const double2 y = make_double2(ys * pv * ev.x, yc * pv * ev.y);
           cr[qq] = mfma64(a.x, y.x, cr[qq]);
           cr[qq] = mfma64(a.y, y.y, cr[qq]);
           ci[qq] = mfma64(a.y, y.x, ci[qq]);
@@ -568,7 +476,8 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
 template <int NCTMAX>
 __global__ void __launch_bounds__(256) k_sph_contract_mfma(
     const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
-    const double2* __restrict__ Y1, const double2* __restrict__ Y2,
+    const double* __restrict__ P1, const double* __restrict__ P2,
+    const double2* __restrict__ E1, const double2* __restrict__ E2,
     const double* __restrict__ R, const double* __restrict__ scale_eff,
     double2* __restrict__ I64, float2* __restrict__ I32) {
   __shared__ double2 Ac[kMB][kCT + 1];
@@ -583,7 +492,7 @@ __global__ void __launch_bounds__(256) k_sph_contract_mfma(
   if (l > L) return;
   const double sc = scale_eff[blockIdx.y];
   const int nct = (2 * l + 1 + 15) >> 4;
-#define FO_CB(K) contract_mfma_body<K>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, Ac)
+#define FO_CB(K) contract_mfma_body<K>(d, l, b, grp, P1, P2, E1, E2, R, sc, I64, I32, Ac)
   if constexpr (NCTMAX <= 2) {
     if (nct <= 1) FO_CB(1); else FO_CB(2);
   } else if constexpr (NCTMAX <= 4) {
@@ -592,32 +501,6 @@ __global__ void __launch_bounds__(256) k_sph_contract_mfma(
     if (nct <= 2) FO_CB(2); else if (nct <= 4) FO_CB(4); else FO_CB(8);
   }
 #undef FO_CB
-}
-
-__global__ void __launch_bounds__(256) k_sph_contract(
-    const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
-    const double2* __restrict__ Y1, const double2* __restrict__ Y2,
-    const double* __restrict__ R, const double* __restrict__ scale_eff,
-    double2* __restrict__ I64, float2* __restrict__ I32) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const SphPair d = pd[blockIdx.y];
-  const int L = d.L;
-  // decode (l, m1 block) from blockIdx.x
-  int b = blockIdx.x, l = 0;
-  for (; l <= L; ++l) {
-    const int nb = (l + 1 + kMB - 1) / kMB;
-    if (b < nb) break;
-    b -= nb;
-  }
-  if (l > L) return;
-  const double sc = scale_eff[blockIdx.y];
-  const int M = 2 * l + 1;
-  if (M <= 32)
-    contract_body<1>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, smem);
-  else if (M <= 64)
-    contract_body<2>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, smem);
-  else
-    contract_body<4>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -1851,8 +1734,10 @@ void launch_sph_prep(hipStream_t s, int P, const SphPair* pd, const double* pos1
 }
 
 void launch_sph_ylm(hipStream_t s, int P, const SphPair* pd, const int* gath, const double* X1c,
-                    const double* X2c, double2* Y1, double2* Y2) {
-  hipLaunchKernelGGL(k_sph_ylm, dim3(P, 2), dim3(128), 0, s, pd, gath, X1c, X2c, Y1, Y2);
+                    const double* X2c, const double2* leg, const double* cm, double* P1,
+                    double* P2, double2* E1, double2* E2) {
+  hipLaunchKernelGGL(k_sph_ylm, dim3(P, 2), dim3(256), 0, s, pd, gath, X1c, X2c, leg, cm, P1, P2,
+                     E1, E2);
 }
 
 void launch_sph_radial(hipStream_t s, int P, long long maxjk, const SphPair* pd,
@@ -1864,17 +1749,22 @@ void launch_sph_radial(hipStream_t s, int P, long long maxjk, const SphPair* pd,
 }
 
 void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, const SphGroup* grp,
-                         const double2* Y1, const double2* Y2, const double* R,
-                         const double* scale_eff, double2* I64, float2* I32) {
+                         const double* P1, const double* P2, const double2* E1,
+                         const double2* E2, const double* R, const double* scale_eff,
+                         double2* I64, float2* I32) {
   int nb = 0;
   for (int l = 0; l <= Lmax; ++l) nb += (l + 1 + kMB - 1) / kMB;
   const dim3 g(nb, P), b(256);
+#define FO_CL(K)                                                                                \
+  hipLaunchKernelGGL(k_sph_contract_mfma<K>, g, b, 0, s, pd, grp, P1, P2, E1, E2, R, scale_eff, \
+                     I64, I32)
   if (Lmax <= 15)
-    hipLaunchKernelGGL(k_sph_contract_mfma<2>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64, I32);
+    FO_CL(2);
   else if (Lmax <= 31)
-    hipLaunchKernelGGL(k_sph_contract_mfma<4>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64, I32);
+    FO_CL(4);
   else
-    hipLaunchKernelGGL(k_sph_contract_mfma<8>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64, I32);
+    FO_CL(8);
+#undef FO_CL
 }
 
 // two-pass factorisation n = N1 N2 of the power-of-two grids (0: direct DFT)
@@ -2094,9 +1984,11 @@ void launch_sph_energy(hipStream_t s, int L, const double2* Iconj, double a, dou
                      out);
 }
 
-void launch_rec_table(hipStream_t s, int Lt, double* rec, double* seedu) {
+void launch_rec_table(hipStream_t s, int Lt, double* rec, double* seedu, double2* leg,
+                      double* cm) {
   const int W = 2 * Lt + 1;
   hipLaunchKernelGGL(k_rec_table, dim3((W * W + 127) / 128), dim3(128), 0, s, Lt, rec, seedu);
+  hipLaunchKernelGGL(k_leg_table, dim3((qlm(Lt, Lt) + 128) / 128), dim3(128), 0, s, Lt, leg, cm);
 }
 void launch_wigner_D(hipStream_t s, int L, double a, double b, double y, double2* D, double2* G) {
   const int W = 2 * L + 1;
