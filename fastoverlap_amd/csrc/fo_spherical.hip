@@ -325,15 +325,17 @@ constexpr int kMB = 32;   // m1 rows per workgroup
 
 // ---------------------------------------------------------------------------
 // MFMA form of the contraction (v_mfma_f64_16x16x4f64).  One workgroup per
-// (pair, l, block of kMB rows m1 >= 0); per tile of kCT atoms k of
-// structure 2:
-//   A[rr][k] = sum_j Y1'[rr][j] K_l[j][k]   real rows rr = 2r + {Re, Im} of
-//              row m1a + r; wave w computes rows 16w..16w+15, both 16-wide
-//              column tiles; operands stream from L2
-//   A -> LDS (complex [row][k])
+// (pair, l, block of kMB rows m1 >= 0).  Per species group and tile of kCT
+// atoms k of structure 2:
+//   stage Y2[m][k-tile] (built from P2, E2) in LDS
+//   A[rr][k] = sum_j Y1'[rr][j] K_l[j][k]: real rows rr = 2r + {Re, Im} of
+//              row m1a + r; j in chunks of kJC staged in LDS one chunk ahead
+//              (cooperative coalesced loads, double buffer); wave w computes
+//              rows 16w..16w+15 of both 16-wide column tiles
+//   A -> LDS (complex [row][k], aliasing the chunk buffers)
 //   C[r][m] += sum_k A[r][k] conj(Y2[m][k]) as four real products
 //              (Cr += Ar Y2r + Ai Y2i, Ci += Ai Y2r - Ar Y2i); complex 16x16
-//              tiles are dealt to the waves round-robin.
+//              tiles dealt to the waves round-robin.
 // MFMA lane layout: A operand lane -> (row lane%16, k lane/16), B operand
 // lane -> (k lane/16, col lane%16), result register i of a lane -> (row
 // 4 i + lane/16, col lane%16).  The reduction order of every output is
@@ -345,7 +347,21 @@ FO_DEV v4d mfma64(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-constexpr int kCT = 32;  // atoms of structure 2 per tile (MFMA form)
+constexpr int kCT = 32;  // atoms of structure 2 per tile
+constexpr int kJC = 16;  // atoms of structure 1 per staged chunk
+
+// LDS of one contraction workgroup; NCT = 16-wide column tiles of Y2.
+template <int NCT>
+struct ContractLds {
+  union {
+    struct {
+      double y1[2][2 * kMB][kJC + 1];  // [buffer][real row][j]
+      double kc[2][kJC][kCT + 1];      // [buffer][j][k]
+    } a;
+    double2 ac[kMB][kCT + 1];          // A tile [row][k]
+  } u;
+  double2 ys[kCT][16 * NCT + 1];       // Y2 tile [k][m]
+};
 
 template <int NCT>
 FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* __restrict__ grp,
@@ -353,13 +369,15 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
                                const double2* __restrict__ E1, const double2* __restrict__ E2,
                                const double* __restrict__ R, double sc,
                                double2* __restrict__ I64, float2* __restrict__ I32,
-                               double2 (*Ac)[kCT + 1]) {
+                               ContractLds<NCT>& S) {
   constexpr int QW = (2 * NCT + 3) / 4;  // complex output tiles per wave
+  constexpr int MP = 16 * NCT;            // padded columns of Y2
   const int M = 2 * l + 1;
   const int m1a = b * kMB;
   const int rows = min(kMB, l + 1 - m1a);
   const int nrt = (rows + 15) >> 4;
   const int t = threadIdx.x, w = t >> 6, ln = t & 63, col = ln & 15, kq = ln >> 4;
+  const bool awave = 16 * w < 2 * rows;  // wave-uniform
   v4d cr[QW], ci[QW];
 #pragma unroll
   for (int q = 0; q < QW; ++q) {
@@ -367,50 +385,100 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
     ci[q] = v4d{0.0, 0.0, 0.0, 0.0};
   }
   const int Ng = d.ngath;
-  const int rr = 16 * w + col, ar = rr >> 1, apart = rr & 1;
-  const bool awave = 16 * w < 2 * rows;  // wave-uniform
-  const bool arow = ar < rows;
-  // Y1 row m1a + ar (m >= 0): P1[q(l, m)] E1[m]
-  const int am1 = arow ? m1a + ar : 0;
-  const double* p1r = P1 + d.ylm_off + (long long)qlm(l, am1) * Ng;
-  const double2* e1r = E1 + d.eph_off + (long long)am1 * Ng;
+  const double* P1l = P1 + d.ylm_off;
+  const double2* E1l = E1 + d.eph_off;
+  const double* P2l = P2 + d.ylm_off;
+  const double2* E2l = E2 + d.eph_off;
   for (int g = 0; g < d.ngroups; ++g) {
     const SphGroup G = grp[d.group_off + g];
-    const int Ns = G.size;
+    const int Ns = G.size, go = G.gath_off;
     const double* Rl = R + d.rad_off + G.rad_off + (long long)l * Ns * Ns;
-    const double* p1g = p1r + G.gath_off;
-    const double2* e1g = e1r + G.gath_off;
+    // staging roles of this thread (fixed per group):
+    //   Y1' chunk: j = j0 + (t & 15), real rows (t >> 4) + 16 i, i < 4
+    //   K chunk:   k = k0 + (t & 31), j = j0 + (t >> 5) + 8 i, i < 2
+    //   Y2 tile:   k = k0 + (t & 31), m = (t >> 5) + 8 i, i < NY
+    const int yj = t & 15, yr0 = t >> 4;
+    const int kk_t = t & 31, kj0 = t >> 5;
+    const double* P1g = P1l + go;
+    const double2* E1g = E1l + go;
+    const double* P2g = P2l + go;
+    const double2* E2g = E2l + go;
     for (int k0 = 0; k0 < Ns; k0 += kCT) {
+      const int kt = k0 + kk_t;
+      const bool kok = kt < Ns;
+      double sy[4], sk[2];
+#define FO_LOAD_CHUNK(J0)                                                           \
+  {                                                                                 \
+    const int j_ = (J0) + yj;                                                       \
+    _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                 \
+      const int rr = yr0 + 16 * i, r = rr >> 1;                                      \
+      double v = 0.0;                                                               \
+      if (r < rows && j_ < Ns) {                                                    \
+        const int m1 = m1a + r;                                                     \
+        const double2 ev = E1g[m1 * Ng + j_];                                       \
+        v = P1g[qlm(l, m1) * Ng + j_] * ((rr & 1) ? ev.y : ev.x);                   \
+      }                                                                             \
+      sy[i] = v;                                                                    \
+    }                                                                               \
+    _Pragma("unroll") for (int i = 0; i < 2; ++i) {                                 \
+      const int j = (J0) + kj0 + 8 * i;                                             \
+      sk[i] = (j < Ns && kok) ? Rl[j * Ns + kt] : 0.0;                              \
+    }                                                                               \
+  }
+#define FO_STORE_CHUNK(BUF)                                                         \
+  {                                                                                 \
+    _Pragma("unroll") for (int i = 0; i < 4; ++i) S.u.a.y1[BUF][yr0 + 16 * i][yj] = sy[i]; \
+    _Pragma("unroll") for (int i = 0; i < 2; ++i) S.u.a.kc[BUF][kj0 + 8 * i][kk_t] = sk[i]; \
+  }
+      // Y2 tile (row m2 = m - l: Y_l^{m2} = P2 E2 for m2 >= 0,
+      // (-1)^|m2| conj(P2 E2) for m2 < 0)
+      constexpr int NY = kCT * MP / 256;
+      double2 yv[NY];
+#pragma unroll
+      for (int i = 0; i < NY; ++i) {
+        const int m = kj0 + 8 * i;
+        double2 y = make_double2(0.0, 0.0);
+        if (m < M && kok) {
+          const int m2 = m - l, am = abs(m2);
+          const double pv = P2g[qlm(l, am) * Ng + kt];
+          const double2 ev = E2g[am * Ng + kt];
+          const double sgn = (m2 < 0 && (am & 1)) ? -1.0 : 1.0;
+          y = make_double2(sgn * pv * ev.x, (m2 < 0 ? -sgn : 1.0) * pv * ev.y);
+        }
+        yv[i] = y;
+      }
+      FO_LOAD_CHUNK(0);
+      __syncthreads();  // previous tile's readers of ys / ac are done
+#pragma unroll
+      for (int i = 0; i < NY; ++i) S.ys[kk_t][kj0 + 8 * i] = yv[i];
+      FO_STORE_CHUNK(0);
+      __syncthreads();
       v4d a0 = v4d{0.0, 0.0, 0.0, 0.0}, a1 = a0;
-      if (awave) {
-        const int c0 = k0 + col, c1 = k0 + 16 + col;
-        for (int j0 = 0; j0 < Ns; j0 += 16) {
-          double av[4], b0[4], b1[4];
+      const int nch = (Ns + kJC - 1) / kJC;
+      for (int c = 0; c < nch; ++c) {
+        const int buf = c & 1;
+        if (c + 1 < nch) FO_LOAD_CHUNK((c + 1) * kJC);
+        if (awave) {
 #pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            const int j = j0 + 4 * s + kq;
-            const bool jok = j < Ns;
-            const bool ok = arow && jok;
-            const double pv = ok ? p1g[j] : 0.0;
-            const double2 ev = ok ? e1g[j] : make_double2(0.0, 0.0);
-            av[s] = pv * (apart ? ev.y : ev.x);
-            b0[s] = (jok && c0 < Ns) ? Rl[(long long)j * Ns + c0] : 0.0;
-            b1[s] = (jok && c1 < Ns) ? Rl[(long long)j * Ns + c1] : 0.0;
-          }
-#pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            a0 = mfma64(av[s], b0[s], a0);
-            a1 = mfma64(av[s], b1[s], a1);
+          for (int s4 = 0; s4 < kJC / 4; ++s4) {
+            const int jj = 4 * s4 + kq;
+            const double av = S.u.a.y1[buf][16 * w + col][jj];
+            a0 = mfma64(av, S.u.a.kc[buf][jj][col], a0);
+            a1 = mfma64(av, S.u.a.kc[buf][jj][16 + col], a1);
           }
         }
+        if (c + 1 < nch) FO_STORE_CHUNK(buf ^ 1);
+        __syncthreads();
       }
-      __syncthreads();  // readers of the previous tile are done
+#undef FO_LOAD_CHUNK
+#undef FO_STORE_CHUNK
+      // A -> LDS (the chunk buffers are free after the barrier above)
       if (awave) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int r2 = 16 * w + 4 * i + kq;
-          double* e0 = (double*)&Ac[r2 >> 1][col];
-          double* e1 = (double*)&Ac[r2 >> 1][16 + col];
+          double* e0 = (double*)&S.u.ac[r2 >> 1][col];
+          double* e1 = (double*)&S.u.ac[r2 >> 1][16 + col];
           e0[r2 & 1] = a0[i];
           e1[r2 & 1] = a1[i];
         }
@@ -421,23 +489,13 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
         const int q = w + 4 * qq;
         const int rt = q / NCT, ct = q % NCT;
         if (rt >= nrt) continue;  // wave-uniform
-        const int r = 16 * rt + col, m = 16 * ct + col;
-        const bool rok = r < rows, mok = m < M;
-        // Y2 row m2 = m - l: Y_l^{|m2|} = P2 E2, conjugated with sign
-        // (-1)^|m2| for m2 < 0
-        const int m2 = mok ? m - l : 0, am2 = abs(m2);
-        const double* p2m = P2 + d.ylm_off + (long long)qlm(l, am2) * Ng + G.gath_off;
-        const double2* e2m = E2 + d.eph_off + (long long)am2 * Ng + G.gath_off;
-        const double ys = m2 < 0 ? ((am2 & 1) ? -1.0 : 1.0) : 1.0;
-        const double yc = m2 < 0 ? -ys : 1.0;
-#pragma unroll
-        for (int s = 0; s < kCT / 4; ++s) {
-          const int kk = 4 * s + kq, ka = k0 + kk;
-          const double2 a = rok ? Ac[r][kk] : make_double2(0.0, 0.0);
-          const bool kok = mok && ka < Ns;
-          const double pv = kok ? p2m[ka] : 0.0;
-          const double2 ev = kok ? e2m[ka] : make_double2(0.0, 0.0);
-          const double2 y = make_double2(ys * pv * ev.x, yc * pv * ev.y);
+        const int r = 16 * rt + col;
+        const bool rok = r < rows;
+#pragma unroll 2
+        for (int s8 = 0; s8 < kCT / 4; ++s8) {
+          const int kk = 4 * s8 + kq;
+          const double2 a = rok ? S.u.ac[r][kk] : make_double2(0.0, 0.0);
+          const double2 y = S.ys[kk][16 * ct + col];
           cr[qq] = mfma64(a.x, y.x, cr[qq]);
           cr[qq] = mfma64(a.y, y.y, cr[qq]);
           ci[qq] = mfma64(a.y, y.x, ci[qq]);
@@ -474,13 +532,13 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
 
 // NCTMAX: column tiles of the largest degree in the launch (2l+1 <= 16 NCTMAX).
 template <int NCTMAX>
-__global__ void __launch_bounds__(256) k_sph_contract_mfma(
+__global__ void __launch_bounds__(256, 2) k_sph_contract_mfma(
     const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
     const double* __restrict__ P1, const double* __restrict__ P2,
     const double2* __restrict__ E1, const double2* __restrict__ E2,
     const double* __restrict__ R, const double* __restrict__ scale_eff,
     double2* __restrict__ I64, float2* __restrict__ I32) {
-  __shared__ double2 Ac[kMB][kCT + 1];
+  __shared__ ContractLds<NCTMAX> S;
   const SphPair d = pd[blockIdx.y];
   const int L = d.L;
   int b = blockIdx.x, l = 0;
@@ -491,16 +549,7 @@ __global__ void __launch_bounds__(256) k_sph_contract_mfma(
   }
   if (l > L) return;
   const double sc = scale_eff[blockIdx.y];
-  const int nct = (2 * l + 1 + 15) >> 4;
-#define FO_CB(K) contract_mfma_body<K>(d, l, b, grp, P1, P2, E1, E2, R, sc, I64, I32, Ac)
-  if constexpr (NCTMAX <= 2) {
-    if (nct <= 1) FO_CB(1); else FO_CB(2);
-  } else if constexpr (NCTMAX <= 4) {
-    if (nct <= 2) FO_CB(2); else FO_CB(4);
-  } else {
-    if (nct <= 2) FO_CB(2); else if (nct <= 4) FO_CB(4); else FO_CB(8);
-  }
-#undef FO_CB
+  contract_mfma_body<NCTMAX>(d, l, b, grp, P1, P2, E1, E2, R, sc, I64, I32, S);
 }
 
 // ---------------------------------------------------------------------------
