@@ -350,7 +350,9 @@ FO_DEV v4d mfma64(double a, double b, v4d c) {
 constexpr int kCT = 32;  // atoms of structure 2 per tile
 constexpr int kJC = 16;  // atoms of structure 1 per staged chunk
 
-// LDS of one contraction workgroup; NCT = 16-wide column tiles of Y2.
+// LDS of one contraction workgroup; NCT = 16-wide column tiles of Y2.  The
+// chunk buffers of the first product and the A / Y2 tiles of the second are
+// never live together.
 template <int NCT>
 struct ContractLds {
   union {
@@ -358,9 +360,11 @@ struct ContractLds {
       double y1[2][2 * kMB][kJC + 1];  // [buffer][real row][j]
       double kc[2][kJC][kCT + 1];      // [buffer][j][k]
     } a;
-    double2 ac[kMB][kCT + 1];          // A tile [row][k]
+    struct {
+      double2 ac[kMB][kCT + 1];        // A tile [row][k]
+      double2 ys[kCT][16 * NCT + 1];   // Y2 tile [k][m]
+    } b;
   } u;
-  double2 ys[kCT][16 * NCT + 1];       // Y2 tile [k][m]
 };
 
 template <int NCT>
@@ -370,20 +374,22 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
                                const double* __restrict__ R, double sc,
                                double2* __restrict__ I64, float2* __restrict__ I32,
                                ContractLds<NCT>& S) {
-  constexpr int QW = (2 * NCT + 3) / 4;  // complex output tiles per wave
   constexpr int MP = 16 * NCT;            // padded columns of Y2
+  constexpr int NY = kCT * MP / 256;      // staged Y2 entries per thread
   const int M = 2 * l + 1;
   const int m1a = b * kMB;
   const int rows = min(kMB, l + 1 - m1a);
-  const int nrt = (rows + 15) >> 4;
+  const int nrt = (rows + 15) >> 4;       // complex row tiles of C
+  const int nrtA = (2 * rows + 15) >> 4;  // real row tiles of A
+  const int nct = (M + 15) >> 4;          // column tiles of C
   const int t = threadIdx.x, w = t >> 6, ln = t & 63, col = ln & 15, kq = ln >> 4;
-  const bool awave = 16 * w < 2 * rows;  // wave-uniform
-  v4d cr[QW], ci[QW];
+  // first product: wave w owns A tiles (row tile (w >> 1) + 2 qa, column
+  // tile w & 1), qa < 2.  Second product: wave w computes the real (w even)
+  // or imaginary (w odd) part of complex C tiles (w >> 1) + 2 qb, qb < NCT.
+  const int actA = w & 1, part = w & 1;
+  v4d cacc[NCT];
 #pragma unroll
-  for (int q = 0; q < QW; ++q) {
-    cr[q] = v4d{0.0, 0.0, 0.0, 0.0};
-    ci[q] = v4d{0.0, 0.0, 0.0, 0.0};
-  }
+  for (int q = 0; q < NCT; ++q) cacc[q] = v4d{0.0, 0.0, 0.0, 0.0};
   const int Ng = d.ngath;
   const double* P1l = P1 + d.ylm_off;
   const double2* E1l = E1 + d.eph_off;
@@ -393,7 +399,7 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
     const SphGroup G = grp[d.group_off + g];
     const int Ns = G.size, go = G.gath_off;
     const double* Rl = R + d.rad_off + G.rad_off + (long long)l * Ns * Ns;
-    // staging roles of this thread (fixed per group):
+    // staging roles of this thread:
     //   Y1' chunk: j = j0 + (t & 15), real rows (t >> 4) + 16 i, i < 4
     //   K chunk:   k = k0 + (t & 31), j = j0 + (t >> 5) + 8 i, i < 2
     //   Y2 tile:   k = k0 + (t & 31), m = (t >> 5) + 8 i, i < NY
@@ -407,6 +413,7 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
       const int kt = k0 + kk_t;
       const bool kok = kt < Ns;
       double sy[4], sk[2];
+      double2 yv[NY];
 #define FO_LOAD_CHUNK(J0)                                                           \
   {                                                                                 \
     const int j_ = (J0) + yj;                                                       \
@@ -415,8 +422,8 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
       double v = 0.0;                                                               \
       if (r < rows && j_ < Ns) {                                                    \
         const int m1 = m1a + r;                                                     \
-        const double2 ev = E1g[m1 * Ng + j_];                                       \
-        v = P1g[qlm(l, m1) * Ng + j_] * ((rr & 1) ? ev.y : ev.x);                   \
+        const double* ep = (const double*)(E1g + m1 * Ng + j_);                     \
+        v = P1g[qlm(l, m1) * Ng + j_] * ep[rr & 1];                                 \
       }                                                                             \
       sy[i] = v;                                                                    \
     }                                                                               \
@@ -430,27 +437,24 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
     _Pragma("unroll") for (int i = 0; i < 4; ++i) S.u.a.y1[BUF][yr0 + 16 * i][yj] = sy[i]; \
     _Pragma("unroll") for (int i = 0; i < 2; ++i) S.u.a.kc[BUF][kj0 + 8 * i][kk_t] = sk[i]; \
   }
-      // Y2 tile (row m2 = m - l: Y_l^{m2} = P2 E2 for m2 >= 0,
-      // (-1)^|m2| conj(P2 E2) for m2 < 0)
-      constexpr int NY = kCT * MP / 256;
-      double2 yv[NY];
-#pragma unroll
-      for (int i = 0; i < NY; ++i) {
-        const int m = kj0 + 8 * i;
-        double2 y = make_double2(0.0, 0.0);
-        if (m < M && kok) {
-          const int m2 = m - l, am = abs(m2);
-          const double pv = P2g[qlm(l, am) * Ng + kt];
-          const double2 ev = E2g[am * Ng + kt];
-          const double sgn = (m2 < 0 && (am & 1)) ? -1.0 : 1.0;
-          y = make_double2(sgn * pv * ev.x, (m2 < 0 ? -sgn : 1.0) * pv * ev.y);
-        }
-        yv[i] = y;
-      }
+// Y2 row m2 = m - l: Y_l^{m2} = P2 E2 for m2 >= 0, (-1)^|m2| conj(P2 E2) for m2 < 0
+#define FO_LOAD_Y2                                                                  \
+  {                                                                                 \
+    _Pragma("unroll") for (int i = 0; i < NY; ++i) {                                \
+      const int m = kj0 + 8 * i;                                                    \
+      double2 y = make_double2(0.0, 0.0);                                           \
+      if (m < M && kok) {                                                           \
+        const int m2 = m - l, am = abs(m2);                                         \
+        const double pv = P2g[qlm(l, am) * Ng + kt];                                \
+        const double2 ev = E2g[am * Ng + kt];                                       \
+        const double sgn = (m2 < 0 && (am & 1)) ? -1.0 : 1.0;                       \
+        y = make_double2(sgn * pv * ev.x, (m2 < 0 ? -sgn : 1.0) * pv * ev.y);       \
+      }                                                                             \
+      yv[i] = y;                                                                    \
+    }                                                                               \
+  }
       FO_LOAD_CHUNK(0);
-      __syncthreads();  // previous tile's readers of ys / ac are done
-#pragma unroll
-      for (int i = 0; i < NY; ++i) S.ys[kk_t][kj0 + 8 * i] = yv[i];
+      __syncthreads();  // previous tile's readers of the union are done
       FO_STORE_CHUNK(0);
       __syncthreads();
       v4d a0 = v4d{0.0, 0.0, 0.0, 0.0}, a1 = a0;
@@ -458,75 +462,101 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
       for (int c = 0; c < nch; ++c) {
         const int buf = c & 1;
         if (c + 1 < nch) FO_LOAD_CHUNK((c + 1) * kJC);
-        if (awave) {
+        // A tiles of this wave: rows 16 rt, columns 16 actA
 #pragma unroll
-          for (int s4 = 0; s4 < kJC / 4; ++s4) {
-            const int jj = 4 * s4 + kq;
-            const double av = S.u.a.y1[buf][16 * w + col][jj];
-            a0 = mfma64(av, S.u.a.kc[buf][jj][col], a0);
-            a1 = mfma64(av, S.u.a.kc[buf][jj][16 + col], a1);
-          }
+        for (int s4 = 0; s4 < kJC / 4; ++s4) {
+          const int jj = 4 * s4 + kq;
+          const double bv = S.u.a.kc[buf][jj][16 * actA + col];
+          const int rt0 = w >> 1, rt1 = (w >> 1) + 2;
+          if (rt0 < nrtA) a0 = mfma64(S.u.a.y1[buf][16 * rt0 + col][jj], bv, a0);
+          if (rt1 < nrtA) a1 = mfma64(S.u.a.y1[buf][16 * rt1 + col][jj], bv, a1);
         }
         if (c + 1 < nch) FO_STORE_CHUNK(buf ^ 1);
         __syncthreads();
       }
+      FO_LOAD_Y2;
 #undef FO_LOAD_CHUNK
 #undef FO_STORE_CHUNK
-      // A -> LDS (the chunk buffers are free after the barrier above)
-      if (awave) {
+#undef FO_LOAD_Y2
+      // A and Y2 tiles -> LDS (the chunk buffers are free after the barrier)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int r2 = 16 * w + 4 * i + kq;
-          double* e0 = (double*)&S.u.ac[r2 >> 1][col];
-          double* e1 = (double*)&S.u.ac[r2 >> 1][16 + col];
-          e0[r2 & 1] = a0[i];
-          e1[r2 & 1] = a1[i];
+      for (int h = 0; h < 2; ++h) {
+        const int rt = (w >> 1) + 2 * h;
+        if (rt < nrtA) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int r2 = 16 * rt + 4 * i + kq;
+            double* e0 = (double*)&S.u.b.ac[r2 >> 1][16 * actA + col];
+            e0[r2 & 1] = h == 0 ? a0[i] : a1[i];
+          }
         }
       }
-      __syncthreads();
 #pragma unroll
-      for (int qq = 0; qq < QW; ++qq) {
-        const int q = w + 4 * qq;
-        const int rt = q / NCT, ct = q % NCT;
-        if (rt >= nrt) continue;  // wave-uniform
-        const int r = 16 * rt + col;
-        const bool rok = r < rows;
+      for (int i = 0; i < NY; ++i) S.u.b.ys[kk_t][kj0 + 8 * i] = yv[i];
+      __syncthreads();
+      // second product, k-steps outer so consecutive MFMAs update
+      // different accumulators
 #pragma unroll 2
-        for (int s8 = 0; s8 < kCT / 4; ++s8) {
-          const int kk = 4 * s8 + kq;
-          const double2 a = rok ? S.u.ac[r][kk] : make_double2(0.0, 0.0);
-          const double2 y = S.ys[kk][16 * ct + col];
-          cr[qq] = mfma64(a.x, y.x, cr[qq]);
-          cr[qq] = mfma64(a.y, y.y, cr[qq]);
-          ci[qq] = mfma64(a.y, y.x, ci[qq]);
-          ci[qq] = mfma64(-a.x, y.y, ci[qq]);
+      for (int s8 = 0; s8 < kCT / 4; ++s8) {
+        const int kk = 4 * s8 + kq;
+#pragma unroll
+        for (int qb = 0; qb < NCT; ++qb) {
+          const int ctile = (w >> 1) + 2 * qb;
+          const int rt = ctile / nct, ct = ctile % nct;
+          if (rt >= nrt) continue;  // wave-uniform
+          const int r = 16 * rt + col;
+          const double2 a = r < rows ? S.u.b.ac[r][kk] : make_double2(0.0, 0.0);
+          const double2 y = S.u.b.ys[kk][16 * ct + col];
+          if (part == 0) {
+            cacc[qb] = mfma64(a.x, y.x, cacc[qb]);
+            cacc[qb] = mfma64(a.y, y.y, cacc[qb]);
+          } else {
+            cacc[qb] = mfma64(a.y, y.x, cacc[qb]);
+            cacc[qb] = mfma64(-a.x, y.y, cacc[qb]);
+          }
         }
       }
     }
   }
+  // C tiles: real parts in waves 0, 2, imaginary parts in waves 1, 3;
+  // exchange through LDS so one lane writes both parts of an entry
+  __syncthreads();
+  double2 (*cb)[kCT + 1] = S.u.b.ac;  // reuse as [32][33] staging per column tile pass
   const double norm = 4.0 * pow(kPi, 2.5) * sc * sc * sc;
+  for (int ct = 0; ct < nct; ++ct) {
+    // tiles (rt, ct) for rt < nrt: ctile = rt * nct + ct
 #pragma unroll
-  for (int qq = 0; qq < QW; ++qq) {
-    const int q = w + 4 * qq;
-    const int rt = q / NCT, ct = q % NCT;
-    if (rt >= nrt) continue;
+    for (int qb = 0; qb < NCT; ++qb) {
+      const int ctile = (w >> 1) + 2 * qb;
+      const int rt = ctile / nct, cc = ctile % nct;
+      if (rt >= nrt || cc != ct) continue;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = 16 * rt + 4 * i + kq, m = 16 * ct + col;
-      if (r >= rows || m >= M) continue;
-      const int m1 = m1a + r, m2 = m - l;
-      const long long id = d.coef_off + kidx(l, m1, m2);
-      const double2 v = make_double2(norm * cr[qq][i], norm * ci[qq][i]);
-      I64[id] = v;
-      I32[id] = make_float2((float)v.x, (float)v.y);
-      if (m1 > 0) {
-        const long long im = d.coef_off + kidx(l, -m1, -m2);
-        const double sg = ((m1 + m2) & 1) ? -1.0 : 1.0;
-        const double2 u = make_double2(sg * v.x, -sg * v.y);
-        I64[im] = u;
-        I32[im] = make_float2((float)u.x, (float)u.y);
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * rt + 4 * i + kq;
+        double* e = (double*)&cb[r][col];
+        e[part] = cacc[qb][i];
       }
     }
+    __syncthreads();
+    for (int e = t; e < kMB * 16; e += 256) {
+      const int r = e >> 4, mc = e & 15, m = 16 * ct + mc;
+      if (r < rows && m < M) {
+        const int m1 = m1a + r, m2 = m - l;
+        const long long id = d.coef_off + kidx(l, m1, m2);
+        const double2 cv = cb[r][mc];
+        const double2 v = make_double2(norm * cv.x, norm * cv.y);
+        I64[id] = v;
+        I32[id] = make_float2((float)v.x, (float)v.y);
+        if (m1 > 0) {
+          const long long im = d.coef_off + kidx(l, -m1, -m2);
+          const double sg = ((m1 + m2) & 1) ? -1.0 : 1.0;
+          const double2 u = make_double2(sg * v.x, -sg * v.y);
+          I64[im] = u;
+          I32[im] = make_float2((float)u.x, (float)u.y);
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
