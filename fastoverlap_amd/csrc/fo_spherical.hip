@@ -382,7 +382,10 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
   const int nrt = (rows + 15) >> 4;       // complex row tiles of C
   const int nrtA = (2 * rows + 15) >> 4;  // real row tiles of A
   const int nct = (M + 15) >> 4;          // column tiles of C
-  const int t = threadIdx.x, w = t >> 6, ln = t & 63, col = ln & 15, kq = ln >> 4;
+  // the wave index is read into a scalar register so that the tile
+  // conditions below are uniform branches (MFMA needs all lanes active)
+  const int t = threadIdx.x, w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int ln = t & 63, col = ln & 15, kq = ln >> 4;
   // first product: wave w owns A tiles (row tile (w >> 1) + 2 qa, column
   // tile w & 1), qa < 2.  Second product: wave w computes the real (w even)
   // or imaginary (w odd) part of complex C tiles (w >> 1) + 2 qb, qb < NCT.
@@ -507,13 +510,10 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
           const int r = 16 * rt + col;
           const double2 a = r < rows ? S.u.b.ac[r][kk] : make_double2(0.0, 0.0);
           const double2 y = S.u.b.ys[kk][16 * ct + col];
-          if (part == 0) {
-            cacc[qb] = mfma64(a.x, y.x, cacc[qb]);
-            cacc[qb] = mfma64(a.y, y.y, cacc[qb]);
-          } else {
-            cacc[qb] = mfma64(a.y, y.x, cacc[qb]);
-            cacc[qb] = mfma64(-a.x, y.y, cacc[qb]);
-          }
+          // real part: Ar Y2r + Ai Y2i; imaginary part: Ai Y2r - Ar Y2i
+          const double a0 = part ? a.y : a.x, a1 = part ? -a.x : a.y;
+          cacc[qb] = mfma64(a0, y.x, cacc[qb]);
+          cacc[qb] = mfma64(a1, y.y, cacc[qb]);
         }
       }
     }
