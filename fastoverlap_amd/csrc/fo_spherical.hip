@@ -880,11 +880,16 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
 #pragma unroll
     for (int j2 = 0; j2 < N2; ++j2) e[j2] = od[j2] = make_float2(0.f, 0.f);
     if (row && lane < N1) {
-      int m2v[N2];
+      // entry j2: m2 (clamped into [-l, l] for the loads, which are then
+      // always in bounds) and the first degree |m2| at which it exists;
+      // absent entries get a zero table weight
+      int m2v[N2], lmin[N2];
 #pragma unroll
       for (int j2 = 0; j2 < N2; ++j2) {
         const int k = lane + N1 * j2;
-        m2v[j2] = k <= L ? k : (k >= n - L ? k - n : 1 << 20);
+        const bool has = k <= L || k >= n - L;
+        m2v[j2] = k <= L ? k : (k >= n - L ? k - n : 0);
+        lmin[j2] = has ? abs(m2v[j2]) : 1 << 20;
       }
       int base = m1 * (4 * m1 * m1 - 1) / 3;  // kbase(m1)
       for (int l = m1; l <= L; ++l) {
@@ -892,18 +897,18 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
         base += (2 * l + 1) * (2 * l + 1);
         float dv[N2];
         float2 ca[N2];
+        int off[N2];
 #pragma unroll
         for (int j2 = 0; j2 < N2; ++j2) {
-          const int m2 = m2v[j2];
-          const bool in = m2 >= -l && m2 <= l;
-          dv[j2] = in ? tb[rb + m2] : 0.f;
-          ca[j2] = in ? ia[rb + m2] : make_float2(0.f, 0.f);
+          off[j2] = rb + min(max(m2v[j2], -l), l);
+          const float d0 = tb[off[j2]];
+          dv[j2] = l >= lmin[j2] ? d0 : 0.f;
+          ca[j2] = ia[off[j2]];
         }
         if constexpr (MODE != 0) {
 #pragma unroll
           for (int j2 = 0; j2 < N2; ++j2) {
-            const int m2 = m2v[j2];
-            const float2 cb = (m2 >= -l && m2 <= l) ? ibb[rb + m2] : make_float2(0.f, 0.f);
+            const float2 cb = ibb[off[j2]];
             e[j2].x += dv[j2] * ca[j2].x;
             e[j2].y += dv[j2] * ca[j2].y;
             od[j2].x += dv[j2] * cb.x;
@@ -957,24 +962,20 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
     const int g = g0 + grp;
     const bool col = g < n;
     float2 v[N2], u[N1];
+    // X[m1] = T_A + i T_B (m1 > 0), conj T_A + i conj T_B (m1 < 0); T[0][g]
+    // is real for a Hermitian set, so only its real parts are kept
+    const int gr = (col ? g : 0) * LS;
 #pragma unroll
     for (int j2 = 0; j2 < N2; ++j2) {
-      v[j2] = make_float2(0.f, 0.f);
       const int k = lane + N1 * j2;
       const int m1 = k <= L ? k : k - n;
-      if (!col || lane >= N1) continue;
-      if (m1 == 0) {
-        // T[0][g] is real for a Hermitian set: keep the real parts only
-        v[j2] = make_float2(TA[g * LS].x, no == 2 ? TB[g * LS].x : 0.f);
-      } else if (m1 > 0) {
-        const float2 A = TA[g * LS + m1];
-        const float2 B = no == 2 ? TB[g * LS + m1] : make_float2(0.f, 0.f);
-        v[j2] = make_float2(A.x - B.y, A.y + B.x);
-      } else if (m1 >= -L) {
-        const float2 A = TA[g * LS - m1];
-        const float2 B = no == 2 ? TB[g * LS - m1] : make_float2(0.f, 0.f);
-        v[j2] = make_float2(A.x + B.y, B.x - A.y);
-      }
+      const bool ok = col && lane < N1 && (k <= L || k >= n - L);
+      const int am = ok ? abs(m1) : 0;
+      const float s1 = ok ? (float)((m1 > 0) - (m1 < 0)) : 0.f;
+      const float2 A = TA[gr + am];
+      const float2 B = no == 2 ? TB[gr + am] : make_float2(0.f, 0.f);
+      const float x = A.x - s1 * B.y, y = s1 * A.y + B.x;
+      v[j2] = ok ? make_float2(x, y) : make_float2(0.f, 0.f);
     }
     dft_reg<N2>(v);
 #pragma unroll
