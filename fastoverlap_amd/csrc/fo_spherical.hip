@@ -1588,7 +1588,14 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
     acc[9] -= w * (-M2 * M2 * r0);
   }
   block_sum_n<10>(acc, red);
-  for (int q = 0; q < 10; ++q) out.v[q] = acc[q];
+  // the sums are uniform over the workgroup: keep them in scalar registers
+#pragma unroll
+  for (int q = 0; q < 10; ++q) {
+    const long long b = __double_as_longlong(acc[q]);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    out.v[q] = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
 }
 
 FO_DEV bool chol3_solve(const double H[6], double lam, const double g[3], double x[3]) {
@@ -1629,11 +1636,31 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
   const double2* I = ((mode == 1 && o == 1) ? IB : IA) + d.coef_off;
   const SphPeak pk = peaks[(long long)p * norient + o];
   double x[3] = {pk.R0[0], pk.R0[1], pk.R0[2]};
+  // one evaluation site: the first evaluation, then Newton trial points
+  // with backtracking (step halved at most 8 times); <= 40 accepted steps
   FGH cur;
-  eval_fgh(I, sgn, L, x[0], x[1], x[2], rec, seedu, Lt, cur, red);
-  int nev = 1;
-  int it = 0;
-  for (; it < 40; ++it) {
+  double xt[3] = {x[0], x[1], x[2]}, dx[3] = {0.0, 0.0, 0.0};
+  double step = 1.0, slope = 0.0;
+  int nev = 0, it = 0, ls = 0;
+  bool have = false;
+  for (;;) {
+    FGH r;
+    eval_fgh(I, sgn, L, xt[0], xt[1], xt[2], rec, seedu, Lt, r, red);
+    ++nev;
+    if (have) {
+      if (!(r.v[0] <= cur.v[0] + 1e-4 * step * slope)) {
+        // reject: halve the step
+        if (++ls >= 8) break;
+        step *= 0.5;
+        for (int q = 0; q < 3; ++q) xt[q] = x[q] + step * dx[q];
+        continue;
+      }
+      for (int q = 0; q < 3; ++q) x[q] = xt[q];
+      ++it;
+    }
+    cur = r;
+    have = true;
+    if (it >= 40) break;
     const double fscale = fabs(cur.v[0]) + 1e-300;
     const double g[3] = {cur.v[1], cur.v[2], cur.v[3]};
     const double gn = fmax(fabs(g[0]), fmax(fabs(g[1]), fabs(g[2])));
@@ -1641,7 +1668,6 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
     // Newton direction with Levenberg damping until positive definite
     const double H[6] = {cur.v[4], cur.v[5], cur.v[6], cur.v[7], cur.v[8], cur.v[9]};
     const double hs = fmax(fabs(H[0]), fmax(fabs(H[3]), fmax(fabs(H[5]), 1e-30)));
-    double dx[3];
     double lam = 0.0;
     bool ok = chol3_solve(H, lam, g, dx);
     for (int k = 0; !ok && k < 12; ++k) {
@@ -1653,25 +1679,12 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
       dx[1] = -g[1] / hs;
       dx[2] = -g[2] / hs;
     }
-    const double slope = g[0] * dx[0] + g[1] * dx[1] + g[2] * dx[2];
+    slope = g[0] * dx[0] + g[1] * dx[1] + g[2] * dx[2];
     const double dxn = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
     if (-0.5 * slope <= 1e-13 * fscale || dxn <= 1e-12) break;
-    double step = 1.0;
-    FGH nxt;
-    bool acc = false;
-    for (int ls = 0; ls < 8; ++ls) {
-      eval_fgh(I, sgn, L, x[0] + step * dx[0], x[1] + step * dx[1], x[2] + step * dx[2], rec,
-               seedu, Lt, nxt, red);
-      ++nev;
-      if (nxt.v[0] <= cur.v[0] + 1e-4 * step * slope) {
-        acc = true;
-        break;
-      }
-      step *= 0.5;
-    }
-    if (!acc) break;
-    for (int q = 0; q < 3; ++q) x[q] += step * dx[q];
-    cur = nxt;
+    step = 1.0;
+    ls = 0;
+    for (int q = 0; q < 3; ++q) xt[q] = x[q] + dx[q];
   }
   if (threadIdx.x == 0) {
     SphOpt r;
