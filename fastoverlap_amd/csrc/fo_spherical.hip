@@ -1940,7 +1940,9 @@ void launch_sph_finish(hipStream_t s, int P, int lap_nmax, const SphPair* pd, co
                        const int* gath, const double* X1c, const double* X2c, const SphOpt* opt,
                        int norient, double* X2R, int* perm, int* rowsol, double* X2out,
                        SphResult* res) {
-  hipLaunchKernelGGL(k_sph_finish, dim3(P), dim3(256), lap_lds_bytes(lap_nmax), s, pd, grp, gath,
+  // one wavefront per pair: the assignment search is a chain of small
+  // reductions, cheapest without cross-wave barriers
+  hipLaunchKernelGGL(k_sph_finish, dim3(P), dim3(64), lap_lds_bytes(lap_nmax), s, pd, grp, gath,
                      X1c, X2c, opt, norient, lap_nmax, X2R, perm, rowsol, X2out, res);
 }
 
