@@ -167,6 +167,19 @@ __global__ void k_sph_prep(const SphPair* __restrict__ pd, const double* __restr
 // ---------------------------------------------------------------------------
 FO_HD inline int qlm(int l, int m) { return l * (l + 1) / 2 + m; }
 
+// XCD-aware decode of a 1-D launch of 8 ceil(P/8) per workgroups over
+// (pair, item): workgroup w serves pair (w % 8) + 8 (w / (8 per)) and item
+// (w / 8) % per.  Workgroups are dealt to the 8 XCDs round-robin, so all
+// items of a pair run on one XCD and share its L2.  False for the padding
+// workgroups (pair >= P).
+FO_DEV bool xcd_decode(int P, int per, int& p, int& item) {
+  const int w = blockIdx.x;
+  p = (w & 7) + 8 * (w / (8 * per));
+  item = (w >> 3) % per;
+  return p < P;
+}
+static inline unsigned xcd_grid(int P, int per) { return (unsigned)(8 * ((P + 7) / 8) * per); }
+
 __global__ void k_sph_ylm(const SphPair* __restrict__ pd, const int* __restrict__ gath,
                           const double* __restrict__ X1c, const double* __restrict__ X2c,
                           const double2* __restrict__ leg, const double* __restrict__ cm,
@@ -567,18 +580,20 @@ __global__ void __launch_bounds__(256, 2) k_sph_contract_mfma(
     const double* __restrict__ P1, const double* __restrict__ P2,
     const double2* __restrict__ E1, const double2* __restrict__ E2,
     const double* __restrict__ R, const double* __restrict__ scale_eff,
-    double2* __restrict__ I64, float2* __restrict__ I32) {
+    double2* __restrict__ I64, float2* __restrict__ I32, int P, int nbl) {
   __shared__ ContractLds<NCTMAX> S;
-  const SphPair d = pd[blockIdx.y];
+  int pp, b;
+  if (!xcd_decode(P, nbl, pp, b)) return;
+  const SphPair d = pd[pp];
   const int L = d.L;
-  int b = blockIdx.x, l = 0;
+  int l = 0;
   for (; l <= L; ++l) {
     const int nb = (l + 1 + kMB - 1) / kMB;
     if (b < nb) break;
     b -= nb;
   }
   if (l > L) return;
-  const double sc = scale_eff[blockIdx.y];
+  const double sc = scale_eff[pp];
   contract_mfma_body<NCTMAX>(d, l, b, grp, P1, P2, E1, E2, R, sc, I64, I32, S);
 }
 
@@ -1273,7 +1288,7 @@ __global__ void __launch_bounds__(256) k_sph_select(
     const SphPair* __restrict__ pd, int n, long long Kt, const double* __restrict__ tab64,
     const double2* __restrict__ tw64, const double2* __restrict__ IA,
     const double2* __restrict__ IB, int mode, int norient, const PlaneCand* __restrict__ cand,
-    double2* __restrict__ scratch, long long scratch_stride, SphPeak* __restrict__ peaks) {
+    double2* __restrict__ scratch, long long scratch_stride, SphPeak* __restrict__ peaks, int P) {
   __shared__ double red[16 * kEvalBatch];
   __shared__ int redi[16];
   __shared__ int s_cnt, s_ovf;
@@ -1282,7 +1297,8 @@ __global__ void __launch_bounds__(256) k_sph_select(
   __shared__ int s_pa[kEvalBatch], s_pg[kEvalBatch], s_pb[kEvalBatch], s_beta[kEvalBeta];
   __shared__ int s_np, s_nb;
   __shared__ double s_val[kEvalBatch];
-  const int p = blockIdx.x, o = blockIdx.y;
+  int p, o;
+  if (!xcd_decode(P, norient, p, o)) return;
   const SphPair d = pd[p];
   const int L = d.L, t = threadIdx.x;
   GridCtx c;
@@ -1627,9 +1643,10 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
                                                     const SphPeak* __restrict__ peaks, int norient,
                                                     const double* __restrict__ rec,
                                                     const double* __restrict__ seedu, int Lt,
-                                                    SphOpt* __restrict__ opt) {
+                                                    SphOpt* __restrict__ opt, int P) {
   __shared__ double red[16 * 10];
-  const int p = blockIdx.x, o = blockIdx.y;
+  int p, o;
+  if (!xcd_decode(P, norient, p, o)) return;
   const SphPair d = pd[p];
   const int L = d.L;
   const double sgn = (mode == 0 && o == 1) ? -1.0 : 1.0;
@@ -1847,10 +1864,10 @@ void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, cons
                          double2* I64, float2* I32) {
   int nb = 0;
   for (int l = 0; l <= Lmax; ++l) nb += (l + 1 + kMB - 1) / kMB;
-  const dim3 g(nb, P), b(256);
+  const dim3 g(xcd_grid(P, nb)), b(256);
 #define FO_CL(K)                                                                                \
   hipLaunchKernelGGL(k_sph_contract_mfma<K>, g, b, 0, s, pd, grp, P1, P2, E1, E2, R, scale_eff, \
-                     I64, I32)
+                     I64, I32, P, nb)
   if (Lmax <= 15)
     FO_CL(2);
   else if (Lmax <= 31)
@@ -1925,15 +1942,15 @@ void launch_sph_select(hipStream_t s, int P, int n, long long Kt, int norient, c
                        const double* tab64, const double2* tw64, const double2* IA,
                        const double2* IB, int mode, const PlaneCand* cand, double2* scratch,
                        long long scratch_stride, SphPeak* peaks) {
-  hipLaunchKernelGGL(k_sph_select, dim3(P, norient), dim3(256), 0, s, pd, n, Kt, tab64, tw64, IA,
-                     IB, mode, norient, cand, scratch, scratch_stride, peaks);
+  hipLaunchKernelGGL(k_sph_select, dim3(xcd_grid(P, norient)), dim3(256), 0, s, pd, n, Kt, tab64,
+                     tw64, IA, IB, mode, norient, cand, scratch, scratch_stride, peaks, P);
 }
 
 void launch_sph_rotopt(hipStream_t s, int P, int norient, const SphPair* pd, const double2* IA,
                        const double2* IB, int mode, const SphPeak* peaks, const double* rec,
                        const double* seedu, int Lt, SphOpt* opt) {
-  hipLaunchKernelGGL(k_sph_rotopt, dim3(P, norient), dim3(256), 0, s, pd, IA, IB, mode, peaks,
-                     norient, rec, seedu, Lt, opt);
+  hipLaunchKernelGGL(k_sph_rotopt, dim3(xcd_grid(P, norient)), dim3(256), 0, s, pd, IA, IB, mode,
+                     peaks, norient, rec, seedu, Lt, opt, P);
 }
 
 void launch_sph_finish(hipStream_t s, int P, int lap_nmax, const SphPair* pd, const SphGroup* grp,
