@@ -59,6 +59,22 @@ def rotopt_eval_flops(L):
     return 32.0 * steps
 
 
+def measured_traffic(kernel, npairs):
+    """HBM bytes per step (one launch per grid class) of `kernel`, measured
+    with rocprof FETCH_SIZE / WRITE_SIZE passes on this workload and recorded
+    in profiles/roofline_traffic.json; None when not recorded for it."""
+    path = os.path.join(ROOT, "profiles", "roofline_traffic.json")
+    try:
+        with open(path) as f:
+            doc = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = doc.get("kernels", {}).get(kernel)
+    if k is None or doc.get("pairs") != npairs:
+        return None
+    return k["hbm_read_bytes_per_step"] + k["hbm_write_bytes_per_step"]
+
+
 def make_batches(pairs, ctx):
     from fastoverlap_amd.batch import SphericalBatch
     out = []
@@ -263,8 +279,8 @@ def main():
 
     # roofline of the dominant kernel (algorithmic flops / measured time)
     work = {
-        "plane": ("k_isoft_plane", sum(isoft_flops(c["L"], c["grid_n"]) for c in pairs), FP32_PEAK),
-        "contract": ("k_sph_contract", sum(coeff_flops(c["groups"], c["L"]) for c in pairs),
+        "plane": ("k_isoft_fft", sum(isoft_flops(c["L"], c["grid_n"]) for c in pairs), FP32_PEAK),
+        "contract": ("k_sph_contract_mfma", sum(coeff_flops(c["groups"], c["L"]) for c in pairs),
                      FP64_PEAK),
         "rotopt": ("k_sph_rotopt", sum(float((e * rotopt_eval_flops(L)[:, None]).sum())
                                        for e, L in zip(evs, Ls)), FP64_PEAK),
@@ -280,7 +296,8 @@ def main():
         name, fl, peak = work[dom]
         ach = fl / (kt[dom] * 1e-3)
         roof = {"kernel": name, "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": ach / peak, "traffic": None}
+                "unit": "TFLOP/s", "frac": ach / peak,
+                "traffic": measured_traffic(name, len(pairs))}
     else:
         roof = {"kernel": dom, "bound": "mfma", "achieved": None, "peak": None,
                 "unit": "TFLOP/s", "frac": None, "traffic": None}
