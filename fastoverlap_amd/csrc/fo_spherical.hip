@@ -832,6 +832,18 @@ FO_DEV void two_pass_exchange(float2 (&v)[N2], float2 (&u)[N1], int lane, float2
   }
 }
 
+// Raw buffer access (gfx9 descriptor): loads at byte offsets at or beyond
+// `bytes` return zero.
+constexpr int kBufOut = 0x0FFFFFFF;  // element offset beyond any buffer used here
+FO_DEV __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
+}
+FO_DEV float2 load_f2(__amdgpu_buffer_rsrc_t r, int idx) {
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  const u2 w = __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, 0);
+  return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+}
+
 template <int N1, int N2>
 constexpr int fft_tpf() {
   return N1 > N2 ? N1 : N2;
@@ -906,24 +918,29 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
         m2v[j2] = k <= L ? k : (k >= n - L ? k - n : 0);
         lmin[j2] = has ? abs(m2v[j2]) : 1 << 20;
       }
+      const int kl = (int)kcount(L);
+      const __amdgpu_buffer_rsrc_t rtb = buf_rsrc(tb, kl * 4);
+      const __amdgpu_buffer_rsrc_t ria = buf_rsrc(ia, kl * 8);
+      const __amdgpu_buffer_rsrc_t rib = buf_rsrc(MODE != 0 ? (const void*)ibb : (const void*)ia, kl * 8);
       int base = m1 * (4 * m1 * m1 - 1) / 3;  // kbase(m1)
       for (int l = m1; l <= L; ++l) {
         const int rb = base + (m1 + l) * (2 * l + 1) + l;  // kidx(l, m1, 0)
         base += (2 * l + 1) * (2 * l + 1);
         float dv[N2];
         float2 ca[N2];
-        int off[N2];
+        // buffer loads with 32-bit offsets; an absent entry gets an offset
+        // beyond the buffer, which the hardware range check reads as zero
+        int idx[N2];
 #pragma unroll
         for (int j2 = 0; j2 < N2; ++j2) {
-          off[j2] = rb + min(max(m2v[j2], -l), l);
-          const float d0 = tb[off[j2]];
-          dv[j2] = l >= lmin[j2] ? d0 : 0.f;
-          ca[j2] = ia[off[j2]];
+          idx[j2] = l >= lmin[j2] ? rb + m2v[j2] : kBufOut;
+          dv[j2] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rtb, idx[j2] * 4, 0, 0));
+          ca[j2] = load_f2(ria, idx[j2]);
         }
         if constexpr (MODE != 0) {
 #pragma unroll
           for (int j2 = 0; j2 < N2; ++j2) {
-            const float2 cb = ibb[off[j2]];
+            const float2 cb = load_f2(rib, idx[j2]);
             e[j2].x += dv[j2] * ca[j2].x;
             e[j2].y += dv[j2] * ca[j2].y;
             od[j2].x += dv[j2] * cb.x;
