@@ -28,12 +28,11 @@ void launch_twiddles(hipStream_t, int, double, double2*, float2*);
 void launch_sph_prep(hipStream_t, int, const SphPair*, const double*, const double*, double*,
                      double*, double*, int);
 void launch_sph_ylm(hipStream_t, int, const SphPair*, const int*, const double*, const double*,
-                    const double2*, const double*, double*, double*, double2*, double2*);
+                    const double2*, const double*, double2*, double2*);
 void launch_sph_radial(hipStream_t, int, long long, const SphPair*, const SphGroup*, const int*,
                        const double*, const double*, const double*, double*);
-void launch_sph_contract(hipStream_t, int, int, const SphPair*, const SphGroup*, const double*,
-                         const double*, const double2*, const double2*, const double*,
-                         const double*, double2*, float2*);
+void launch_sph_contract(hipStream_t, int, int, const SphPair*, const SphGroup*, const double2*,
+                         const double2*, const double*, const double*, double2*, float2*);
 size_t isoft_plane_smem(int, int, int);
 int isoft_plane_use_fft(int, int, size_t);
 void launch_isoft_plane(hipStream_t, int, int, int, int, long long, const SphPair*, const float*,
@@ -346,8 +345,7 @@ struct fo_sph_batch {
   int* gath = nullptr;
   double *pos1 = nullptr, *pos2 = nullptr, *X1c = nullptr, *X2c = nullptr, *X2R = nullptr,
          *X2out = nullptr, *scale = nullptr, *R = nullptr;
-  double *P1 = nullptr, *P2 = nullptr;          // Legendre factors of Y_lm
-  double2 *E1 = nullptr, *E2 = nullptr;         // e^{i m phi}
+  double2 *Y1 = nullptr, *Y2 = nullptr;         // Y_l^m (m >= 0) of the two structures
   double2 *I64A = nullptr, *I64B = nullptr, *scr = nullptr;
   float2 *I32A = nullptr, *I32B = nullptr;
   PlaneCand* cand = nullptr;
@@ -365,7 +363,7 @@ struct fo_sph_batch {
   bool ev_made = false;
   // layout (byte offsets inside base) and ownership
   struct {
-    size_t pd, grp, gath, pos1, pos2, X1c, X2c, X2R, X2out, scale, R, P1, P2, E1, E2, I64A, I32A, I64B,
+    size_t pd, grp, gath, pos1, pos2, X1c, X2c, X2R, X2out, scale, R, Y1, Y2, I64A, I32A, I64B,
         I32B, cand, peaks, opt, res, perm, rowsol, scr, grid;
   } off{};
   int gsum = 0;
@@ -385,12 +383,11 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
     launch_sph_prep(s, P, b->pd, b->pos1, b->pos2, b->X1c, b->X2c, b->scale, L.center);
   mark(1);
   if (stages & FO_STAGE_COEFF) {
-    launch_sph_ylm(s, P, b->pd, b->gath, b->X1c, b->X2c, b->rect->leg, b->rect->cm, b->P1, b->P2,
-                   b->E1, b->E2);
+    launch_sph_ylm(s, P, b->pd, b->gath, b->X1c, b->X2c, b->rect->leg, b->rect->cm, b->Y1, b->Y2);
     mark(2);
     launch_sph_radial(s, P, L.maxjk, b->pd, b->grp, b->gath, b->X1c, b->X2c, b->scale, b->R);
     mark(3);
-    launch_sph_contract(s, P, L.Lmax, b->pd, b->grp, b->P1, b->P2, b->E1, b->E2, b->R, b->scale, b->I64A,
+    launch_sph_contract(s, P, L.Lmax, b->pd, b->grp, b->Y1, b->Y2, b->R, b->scale, b->I64A,
                         b->I32A);
   } else {
     mark(2);
@@ -433,7 +430,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   b->ctx = c;
   b->flags = d->flags;
   int Lmax = 0, maxatoms = 0, maxgath = 0;
-  long long maxjk = 0, Ntot = 0, Ktot = 0, radtot = 0, ylmtot = 0, ephtot = 0;
+  long long maxjk = 0, Ntot = 0, Ktot = 0, radtot = 0, ylmtot = 0;
   int gsum = 0, gatot = 0;
   for (int p = 0; p < P; ++p) gsum += d->ngroups[p];
   for (int q = 0; q < gsum; ++q) {
@@ -488,11 +485,9 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
     sp.coef_off = Ktot;
     sp.rad_off = radtot;
     sp.ylm_off = ylmtot;
-    sp.eph_off = ephtot;
     Ktot += kcount(Lp);
     radtot += (long long)(Lp + 1) * jk;
     ylmtot += (long long)(Lp + 1) * (Lp + 2) / 2 * ng;
-    ephtot += (long long)(Lp + 1) * ng;
     Lmax = std::max(Lmax, Lp);
     maxatoms = std::max(maxatoms, N);
     maxgath = std::max(maxgath, ng);
@@ -544,10 +539,8 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   o.X2out = lay.add<double>(3 * Ntot);
   o.scale = lay.add<double>(P);
   o.R = lay.add<double>(std::max(radtot, 1LL));
-  o.P1 = lay.add<double>(std::max(ylmtot, 1LL));
-  o.P2 = lay.add<double>(std::max(ylmtot, 1LL));
-  o.E1 = lay.add<double2>(std::max(ephtot, 1LL));
-  o.E2 = lay.add<double2>(std::max(ephtot, 1LL));
+  o.Y1 = lay.add<double2>(std::max(ylmtot, 1LL));
+  o.Y2 = lay.add<double2>(std::max(ylmtot, 1LL));
   o.I64A = lay.add<double2>(Ktot);
   o.I32A = lay.add<float2>(Ktot);
   o.I64B = general ? lay.add<double2>(Ktot) : 0;
@@ -591,10 +584,8 @@ int sph_batch_bind(fo_sph_batch* b, void* mem, hipStream_t stream) {
   b->X2out = (double*)(B + o.X2out);
   b->scale = (double*)(B + o.scale);
   b->R = (double*)(B + o.R);
-  b->P1 = (double*)(B + o.P1);
-  b->P2 = (double*)(B + o.P2);
-  b->E1 = (double2*)(B + o.E1);
-  b->E2 = (double2*)(B + o.E2);
+  b->Y1 = (double2*)(B + o.Y1);
+  b->Y2 = (double2*)(B + o.Y2);
   b->I64A = (double2*)(B + o.I64A);
   b->I32A = (float2*)(B + o.I32A);
   b->I64B = general ? (double2*)(B + o.I64B) : nullptr;

@@ -20,9 +20,8 @@ struct SphPair {
   int pad0;
   long long coef_off;  // first complex entry of the pair's C^l (K(L) entries)
   long long rad_off;   // first double of the pair's radial block
-  long long ylm_off;   // first double of the pair's Legendre block P[q(l,m)][g] (per structure)
+  long long ylm_off;   // first complex entry of the pair's Y block Y[q(l,m)][g], m >= 0 (per structure)
   double scale;        // Gaussian width sigma (<= 0: automatic)
-  long long eph_off;   // first complex entry of the pair's e^{i m phi} block E[m][g]
 };
 
 struct SphGroup {

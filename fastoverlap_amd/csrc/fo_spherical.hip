@@ -157,13 +157,13 @@ __global__ void k_sph_prep(const SphPair* __restrict__ pd, const double* __restr
 
 // ---------------------------------------------------------------------------
 // Spherical harmonics of every grouped atom (fastoverlap/utils.py:296-302,
-// fastoverlap/sphericalAlignment.py:57-65), FP64, stored factorised:
-//   Y_l^m = P[q(l,m)][g] E[m][g],  q(l,m) = l(l+1)/2 + m,  m >= 0,
+// fastoverlap/sphericalAlignment.py:57-65), FP64, for m >= 0 only:
+//   Y[q(l,m)][g] = P_l^m(cos theta) e^{i m phi},  q(l,m) = l(l+1)/2 + m,
 // with P the normalised associated Legendre function (Condon-Shortley
-// phase) and E = e^{i m phi}; Y_l^{-m} = (-1)^m conj(Y_l^m).  One thread per
-// (atom, m): P_m^m = c_m sin^m(theta), then the three-term recurrence in l
-// with tabulated coefficients (k_leg_table).  An atom at the centre takes
-// theta = phi = 0 (only l = 0 survives there).
+// phase); Y_l^{-m} = (-1)^m conj(Y_l^m) is formed where it is read.  One
+// thread per (atom, m): P_m^m = c_m sin^m(theta), then the three-term
+// recurrence in l with tabulated coefficients (k_leg_table).  An atom at the
+// centre takes theta = phi = 0 (only l = 0 survives there).
 // ---------------------------------------------------------------------------
 FO_HD inline int qlm(int l, int m) { return l * (l + 1) / 2 + m; }
 
@@ -183,12 +183,10 @@ static inline unsigned xcd_grid(int P, int per) { return (unsigned)(8 * ((P + 7)
 __global__ void k_sph_ylm(const SphPair* __restrict__ pd, const int* __restrict__ gath,
                           const double* __restrict__ X1c, const double* __restrict__ X2c,
                           const double2* __restrict__ leg, const double* __restrict__ cm,
-                          double* __restrict__ P1, double* __restrict__ P2,
-                          double2* __restrict__ E1, double2* __restrict__ E2) {
+                          double2* __restrict__ Y1, double2* __restrict__ Y2) {
   const SphPair d = pd[blockIdx.x];
   const double* X = blockIdx.y ? X2c : X1c;
-  double* Pp = (blockIdx.y ? P2 : P1) + d.ylm_off;
-  double2* Ep = (blockIdx.y ? E2 : E1) + d.eph_off;
+  double2* Yp = (blockIdx.y ? Y2 : Y1) + d.ylm_off;
   const int L = d.L, Ng = d.ngath;
   for (int w = threadIdx.x; w < Ng * (L + 1); w += blockDim.x) {
     const int m = w / Ng, g = w % Ng;
@@ -204,13 +202,12 @@ __global__ void k_sph_ylm(const SphPair* __restrict__ pd, const int* __restrict_
     }
     double sp, cp;
     sincos((double)m * phi, &sp, &cp);
-    Ep[(long long)m * Ng + g] = make_double2(cp, sp);
     double p2 = 0.0, p1 = cm[m] * ipow(st, m);
-    Pp[(long long)qlm(m, m) * Ng + g] = p1;
+    Yp[(long long)qlm(m, m) * Ng + g] = make_double2(p1 * cp, p1 * sp);
     for (int l = m + 1; l <= L; ++l) {
       const double2 ab = leg[qlm(l, m)];
       const double pl = ab.x * (ct * p1 - ab.y * p2);
-      Pp[(long long)qlm(l, m) * Ng + g] = pl;
+      Yp[(long long)qlm(l, m) * Ng + g] = make_double2(pl * cp, pl * sp);
       p2 = p1;
       p1 = pl;
     }
@@ -382,8 +379,7 @@ struct ContractLds {
 
 template <int NCT>
 FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* __restrict__ grp,
-                               const double* __restrict__ P1, const double* __restrict__ P2,
-                               const double2* __restrict__ E1, const double2* __restrict__ E2,
+                               const double2* __restrict__ Y1, const double2* __restrict__ Y2,
                                const double* __restrict__ R, double sc,
                                double2* __restrict__ I64, float2* __restrict__ I32,
                                ContractLds<NCT>& S) {
@@ -407,41 +403,34 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
 #pragma unroll
   for (int q = 0; q < NCT; ++q) cacc[q] = v4d{0.0, 0.0, 0.0, 0.0};
   const int Ng = d.ngath;
-  const double* P1l = P1 + d.ylm_off;
-  const double2* E1l = E1 + d.eph_off;
-  const double* P2l = P2 + d.ylm_off;
-  const double2* E2l = E2 + d.eph_off;
+  const double2* Y1l = Y1 + d.ylm_off;
+  const double2* Y2l = Y2 + d.ylm_off;
   for (int g = 0; g < d.ngroups; ++g) {
     const SphGroup G = grp[d.group_off + g];
     const int Ns = G.size, go = G.gath_off;
     const double* Rl = R + d.rad_off + G.rad_off + (long long)l * Ns * Ns;
     // staging roles of this thread:
-    //   Y1' chunk: j = j0 + (t & 15), real rows (t >> 4) + 16 i, i < 4
+    //   Y1 chunk:  j = j0 + (t & 15), complex rows (t >> 4) + 16 i, i < 2
+    //              (each gives the real rows 2r and 2r + 1)
     //   K chunk:   k = k0 + (t & 31), j = j0 + (t >> 5) + 8 i, i < 2
     //   Y2 tile:   k = k0 + (t & 31), m = (t >> 5) + 8 i, i < NY
     const int yj = t & 15, yr0 = t >> 4;
     const int kk_t = t & 31, kj0 = t >> 5;
-    const double* P1g = P1l + go;
-    const double2* E1g = E1l + go;
-    const double* P2g = P2l + go;
-    const double2* E2g = E2l + go;
+    const double2* Y1g = Y1l + go;
+    const double2* Y2g = Y2l + go;
     for (int k0 = 0; k0 < Ns; k0 += kCT) {
       const int kt = k0 + kk_t;
       const bool kok = kt < Ns;
-      double sy[4], sk[2];
+      double2 sy[2];
+      double sk[2];
       double2 yv[NY];
 #define FO_LOAD_CHUNK(J0)                                                           \
   {                                                                                 \
     const int j_ = (J0) + yj;                                                       \
-    _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                 \
-      const int rr = yr0 + 16 * i, r = rr >> 1;                                      \
-      double v = 0.0;                                                               \
-      if (r < rows && j_ < Ns) {                                                    \
-        const int m1 = m1a + r;                                                     \
-        const double* ep = (const double*)(E1g + m1 * Ng + j_);                     \
-        v = P1g[qlm(l, m1) * Ng + j_] * ep[rr & 1];                                 \
-      }                                                                             \
-      sy[i] = v;                                                                    \
+    _Pragma("unroll") for (int i = 0; i < 2; ++i) {                                 \
+      const int r = yr0 + 16 * i;                                                   \
+      sy[i] = (r < rows && j_ < Ns) ? Y1g[qlm(l, m1a + r) * Ng + j_]                \
+                                    : make_double2(0.0, 0.0);                       \
     }                                                                               \
     _Pragma("unroll") for (int i = 0; i < 2; ++i) {                                 \
       const int j = (J0) + kj0 + 8 * i;                                             \
@@ -450,7 +439,10 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
   }
 #define FO_STORE_CHUNK(BUF)                                                         \
   {                                                                                 \
-    _Pragma("unroll") for (int i = 0; i < 4; ++i) S.u.a.y1[BUF][yr0 + 16 * i][yj] = sy[i]; \
+    _Pragma("unroll") for (int i = 0; i < 2; ++i) {                                 \
+      S.u.a.y1[BUF][2 * (yr0 + 16 * i)][yj] = sy[i].x;                              \
+      S.u.a.y1[BUF][2 * (yr0 + 16 * i) + 1][yj] = sy[i].y;                          \
+    }                                                                               \
     _Pragma("unroll") for (int i = 0; i < 2; ++i) S.u.a.kc[BUF][kj0 + 8 * i][kk_t] = sk[i]; \
   }
 // Y2 row m2 = m - l: Y_l^{m2} = P2 E2 for m2 >= 0, (-1)^|m2| conj(P2 E2) for m2 < 0
@@ -461,10 +453,9 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
       double2 y = make_double2(0.0, 0.0);                                           \
       if (m < M && kok) {                                                           \
         const int m2 = m - l, am = abs(m2);                                         \
-        const double pv = P2g[qlm(l, am) * Ng + kt];                                \
-        const double2 ev = E2g[am * Ng + kt];                                       \
+        const double2 yp = Y2g[qlm(l, am) * Ng + kt];                               \
         const double sgn = (m2 < 0 && (am & 1)) ? -1.0 : 1.0;                       \
-        y = make_double2(sgn * pv * ev.x, (m2 < 0 ? -sgn : 1.0) * pv * ev.y);       \
+        y = make_double2(sgn * yp.x, (m2 < 0 ? -sgn : 1.0) * yp.y);                 \
       }                                                                             \
       yv[i] = y;                                                                    \
     }                                                                               \
@@ -577,8 +568,7 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
 template <int NCTMAX>
 __global__ void __launch_bounds__(256, 2) k_sph_contract_mfma(
     const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
-    const double* __restrict__ P1, const double* __restrict__ P2,
-    const double2* __restrict__ E1, const double2* __restrict__ E2,
+    const double2* __restrict__ Y1, const double2* __restrict__ Y2,
     const double* __restrict__ R, const double* __restrict__ scale_eff,
     double2* __restrict__ I64, float2* __restrict__ I32, int P, int nbl) {
   __shared__ ContractLds<NCTMAX> S;
@@ -594,7 +584,7 @@ __global__ void __launch_bounds__(256, 2) k_sph_contract_mfma(
   }
   if (l > L) return;
   const double sc = scale_eff[pp];
-  contract_mfma_body<NCTMAX>(d, l, b, grp, P1, P2, E1, E2, R, sc, I64, I32, S);
+  contract_mfma_body<NCTMAX>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -1861,10 +1851,9 @@ void launch_sph_prep(hipStream_t s, int P, const SphPair* pd, const double* pos1
 }
 
 void launch_sph_ylm(hipStream_t s, int P, const SphPair* pd, const int* gath, const double* X1c,
-                    const double* X2c, const double2* leg, const double* cm, double* P1,
-                    double* P2, double2* E1, double2* E2) {
-  hipLaunchKernelGGL(k_sph_ylm, dim3(P, 2), dim3(256), 0, s, pd, gath, X1c, X2c, leg, cm, P1, P2,
-                     E1, E2);
+                    const double* X2c, const double2* leg, const double* cm, double2* Y1,
+                    double2* Y2) {
+  hipLaunchKernelGGL(k_sph_ylm, dim3(P, 2), dim3(256), 0, s, pd, gath, X1c, X2c, leg, cm, Y1, Y2);
 }
 
 void launch_sph_radial(hipStream_t s, int P, long long maxjk, const SphPair* pd,
@@ -1876,15 +1865,14 @@ void launch_sph_radial(hipStream_t s, int P, long long maxjk, const SphPair* pd,
 }
 
 void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, const SphGroup* grp,
-                         const double* P1, const double* P2, const double2* E1,
-                         const double2* E2, const double* R, const double* scale_eff,
-                         double2* I64, float2* I32) {
+                         const double2* Y1, const double2* Y2, const double* R,
+                         const double* scale_eff, double2* I64, float2* I32) {
   int nb = 0;
   for (int l = 0; l <= Lmax; ++l) nb += (l + 1 + kMB - 1) / kMB;
   const dim3 g(xcd_grid(P, nb)), b(256);
 #define FO_CL(K)                                                                                \
-  hipLaunchKernelGGL(k_sph_contract_mfma<K>, g, b, 0, s, pd, grp, P1, P2, E1, E2, R, scale_eff, \
-                     I64, I32, P, nb)
+  hipLaunchKernelGGL(k_sph_contract_mfma<K>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,  \
+                     I32, P, nb)
   if (Lmax <= 15)
     FO_CL(2);
   else if (Lmax <= 31)
