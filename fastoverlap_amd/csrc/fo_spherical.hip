@@ -566,7 +566,7 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
 
 // NCTMAX: column tiles of the largest degree in the launch (2l+1 <= 16 NCTMAX).
 template <int NCTMAX>
-__global__ void __launch_bounds__(256, 2) k_sph_contract_mfma(
+__global__ void __launch_bounds__(256, 3) k_sph_contract_mfma(
     const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
     const double2* __restrict__ Y1, const double2* __restrict__ Y2,
     const double* __restrict__ R, const double* __restrict__ scale_eff,
