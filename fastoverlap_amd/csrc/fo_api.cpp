@@ -31,8 +31,9 @@ void launch_sph_ylm(hipStream_t, int, const SphPair*, const int*, const double*,
                     const double2*, const double*, double2*, double2*);
 void launch_sph_radial(hipStream_t, int, long long, const SphPair*, const SphGroup*, const int*,
                        const double*, const double*, const double*, double*);
-void launch_sph_contract(hipStream_t, int, int, const SphPair*, const SphGroup*, const double2*,
-                         const double2*, const double*, const double*, double2*, float2*);
+void launch_sph_contract(hipStream_t, int, int, int, const SphPair*, const SphGroup*,
+                         const double2*, const double2*, const double*, const double*, double2*,
+                         float2*);
 size_t isoft_plane_smem(int, int, int);
 int isoft_plane_use_fft(int, int, size_t);
 void launch_isoft_plane(hipStream_t, int, int, int, int, long long, const SphPair*, const float*,
@@ -46,7 +47,7 @@ void launch_rec_table(hipStream_t, int, double*, double*, double2*, double*);
 void launch_sph_finish(hipStream_t, int, int, const SphPair*, const SphGroup*, const int*,
                        const double*, const double*, const SphOpt*, int, double*, int*, int*,
                        double*, SphResult*);
-void launch_c64_to_c32(hipStream_t, long long, const double2*, float2*);
+void launch_coef_to_plane(hipStream_t, int, int, int, const SphPair*, const double2*, float2*);
 void launch_sph_energy(hipStream_t, int, const double2*, double, double, double, const double*,
                        const double*, int, double*);
 void launch_wigner_D(hipStream_t, int, double, double, double, double2*, double2*);
@@ -107,7 +108,8 @@ static constexpr int kMaxGroup = 4096;
 // ---------------------------------------------------------------------------
 struct SphTable {
   int n = 0, Lt = 0;
-  long long Kt = 0;
+  long long Kt = 0;         // entries per beta of tab64 (compact layout, K(Lt))
+  long long Kp = 0;         // entries per beta of tab32 (plane layout, rows(Lt) * n)
   double* tab64 = nullptr;
   float* tab32 = nullptr;
   double2* tw64 = nullptr;
@@ -177,7 +179,8 @@ static int get_sph_table(fo_context* c, int n, const SphTable** out) {
   t.n = n;
   t.Lt = std::min(n / 2 - 1, kMaxL);
   t.Kt = kcount(t.Lt);
-  const size_t b64 = sizeof(double) * (size_t)t.Kt * n, b32 = sizeof(float) * (size_t)t.Kt * n;
+  t.Kp = plane_rows(t.Lt) * n;
+  const size_t b64 = sizeof(double) * (size_t)t.Kt * n, b32 = sizeof(float) * (size_t)t.Kp * n;
   const size_t bt = (sizeof(double2) + sizeof(float2)) * (size_t)n;
   const size_t total = b64 + b32 + bt + 1024;
   if (c->budget && c->used + total > c->budget)
@@ -195,6 +198,7 @@ static int get_sph_table(fo_context* c, int n, const SphTable** out) {
   t.tw64 = (double2*)p;
   p += (sizeof(double2) * n + 255) / 256 * 256;
   t.tw32 = (float2*)p;
+  FO_HIP(hipMemsetAsync(t.tab32, 0, b32, c->stream));  // entries |m2| > l stay zero
   launch_wigner_table(c->stream, n, t.Lt, t.tab64, t.tab32);
   launch_twiddles(c->stream, n, 1.0, t.tw64, t.tw32);
   FO_CHECK_LAUNCH();
@@ -387,7 +391,7 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
     mark(2);
     launch_sph_radial(s, P, L.maxjk, b->pd, b->grp, b->gath, b->X1c, b->X2c, b->scale, b->R);
     mark(3);
-    launch_sph_contract(s, P, L.Lmax, b->pd, b->grp, b->Y1, b->Y2, b->R, b->scale, b->I64A,
+    launch_sph_contract(s, P, L.n, L.Lmax, b->pd, b->grp, b->Y1, b->Y2, b->R, b->scale, b->I64A,
                         b->I32A);
   } else {
     mark(2);
@@ -395,7 +399,7 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
   }
   mark(4);
   if (stages & FO_STAGE_GRID) {
-    launch_isoft_plane(s, P, L.n, L.Lmax, b->fft_chunk, b->tab->Kt, b->pd, b->tab->tab32,
+    launch_isoft_plane(s, P, L.n, L.Lmax, b->fft_chunk, b->tab->Kp, b->pd, b->tab->tab32,
                        b->tab->tw32, b->I32A, b->I32B, L.mode, L.norient, b->cand, b->grid);
     mark(5);
     launch_sph_select(s, P, L.n, b->tab->Kt, L.norient, b->pd, b->tab->tab64, b->tab->tw64,
@@ -430,7 +434,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   b->ctx = c;
   b->flags = d->flags;
   int Lmax = 0, maxatoms = 0, maxgath = 0;
-  long long maxjk = 0, Ntot = 0, Ktot = 0, radtot = 0, ylmtot = 0;
+  long long maxjk = 0, Ntot = 0, Ktot = 0, C32tot = 0, radtot = 0, ylmtot = 0;
   int gsum = 0, gatot = 0;
   for (int p = 0; p < P; ++p) gsum += d->ngroups[p];
   for (int q = 0; q < gsum; ++q) {
@@ -483,9 +487,11 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
     }
     sp.ngath = ng;
     sp.coef_off = Ktot;
+    sp.c32_off = C32tot;
     sp.rad_off = radtot;
     sp.ylm_off = ylmtot;
     Ktot += kcount(Lp);
+    C32tot += plane_rows(Lp) * n;
     radtot += (long long)(Lp + 1) * jk;
     ylmtot += (long long)(Lp + 1) * (Lp + 2) / 2 * ng;
     Lmax = std::max(Lmax, Lp);
@@ -542,9 +548,9 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   o.Y1 = lay.add<double2>(std::max(ylmtot, 1LL));
   o.Y2 = lay.add<double2>(std::max(ylmtot, 1LL));
   o.I64A = lay.add<double2>(Ktot);
-  o.I32A = lay.add<float2>(Ktot);
+  o.I32A = lay.add<float2>(C32tot);
   o.I64B = general ? lay.add<double2>(Ktot) : 0;
-  o.I32B = general ? lay.add<float2>(Ktot) : 0;
+  o.I32B = general ? lay.add<float2>(C32tot) : 0;
   o.cand = lay.add<PlaneCand>((size_t)P * no * n);
   o.peaks = lay.add<SphPeak>((size_t)P * no);
   o.opt = lay.add<SphOpt>((size_t)P * no);
@@ -676,11 +682,11 @@ int fo_sph_batch_set_positions_device(fo_sph_batch* b, const void* p1, const voi
 int fo_sph_batch_set_coeffs(fo_sph_batch* b, const double* A, const double* Bc) {
   if (!b || !A) return set_err(FO_ERR_INVALID, "NULL argument");
   FO_HIP(hipMemcpyAsync(b->I64A, A, sizeof(double2) * b->Ktot, hipMemcpyHostToDevice, b->stream));
-  launch_c64_to_c32(b->stream, b->Ktot, b->I64A, b->I32A);
+  launch_coef_to_plane(b->stream, b->lc.P, b->lc.n, b->lc.Lmax, b->pd, b->I64A, b->I32A);
   if (b->lc.mode == 1) {
     if (!Bc) return set_err(FO_ERR_INVALID, "general mode needs the second coefficient set");
     FO_HIP(hipMemcpyAsync(b->I64B, Bc, sizeof(double2) * b->Ktot, hipMemcpyHostToDevice, b->stream));
-    launch_c64_to_c32(b->stream, b->Ktot, b->I64B, b->I32B);
+    launch_coef_to_plane(b->stream, b->lc.P, b->lc.n, b->lc.Lmax, b->pd, b->I64B, b->I32B);
   }
   FO_CHECK_LAUNCH();
   FO_HIP(hipStreamSynchronize(b->stream));

@@ -29,6 +29,31 @@ FO_HD long long kidx(int l, int m1, int m2) {
   return kbase(l) + (long long)(m1 + l) * (2 * l + 1) + (m2 + l);
 }
 
+// triangular index of (l, m), 0 <= m <= l
+FO_HD int qlm(int l, int m) { return l * (l + 1) / 2 + m; }
+
+// ---- plane layout (FP32 operands of the iSOFT plane kernels) --------------
+// Rows (l, m1) with 0 <= m1 <= l at row l(l+1)/2 + m1 (l-major, so the rows
+// of degrees <= L are a prefix), n entries per row indexed by the gamma
+// transform input k (m2 = k for k < n/2, k - n above) and zero where |m2| > l.
+// Inside a row, entry k sits at plane_pos(k, n): lane-major for the grid
+// sides of the two-pass transform (n = N1*N2, lane k1 owns k = k1 + N1*k2 and
+// its N2 entries are contiguous), in k order for other n.
+FO_HD int plane_n1(int n) {  // N1 of the two-pass transform, 0 if none
+  switch (n) {
+    case 16: return 4;
+    case 32: case 64: return 8;
+    case 128: case 256: return 16;
+    default: return 0;
+  }
+}
+FO_HD long long plane_rows(int L) { return (long long)(L + 1) * (L + 2) / 2; }
+FO_HD int plane_k(int m2, int n) { return m2 < 0 ? m2 + n : m2; }
+FO_HD int plane_pos(int k, int n) {
+  const int n1 = plane_n1(n);
+  return n1 ? (k % n1) * (n / n1) + k / n1 : k;
+}
+
 // ---- complex helpers (double2 / float2 = re, im) -------------------------
 FO_HD double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);

@@ -21,6 +21,7 @@ struct SphPair {
   long long coef_off;  // first complex entry of the pair's C^l (K(L) entries)
   long long rad_off;   // first double of the pair's radial block
   long long ylm_off;   // first complex entry of the pair's Y block Y[q(l,m)][g], m >= 0 (per structure)
+  long long c32_off;   // first complex entry of the pair's FP32 C^l in the plane layout
   double scale;        // Gaussian width sigma (<= 0: automatic)
 };
 

@@ -55,11 +55,14 @@ FO_DEV double seed_xi(int m1, int m2) {
   return (m2 < m1 && ((m1 - m2) & 1)) ? -1.0 : 1.0;
 }
 
+// tab64: compact layout, [k][kidx(l,m1,m2)]; tab32: plane layout (rows
+// m1 >= 0 only, zero entries pre-cleared by the caller), [k][row][pos].
 __global__ void k_wigner_table(int n, int Lt, double* __restrict__ tab64,
                                float* __restrict__ tab32) {
   const int W = 2 * Lt + 1;
   const long long total = (long long)n * W * W;
   const long long Kt = kcount(Lt);
+  const long long Kp = plane_rows(Lt) * n;
   for (long long w = blockIdx.x * (long long)blockDim.x + threadIdx.x; w < total;
        w += (long long)gridDim.x * blockDim.x) {
     const int k = (int)(w / (W * W));
@@ -73,19 +76,17 @@ __global__ void k_wigner_table(int n, int Lt, double* __restrict__ tab64,
                         exp(seed_log_factor(mu, nu) + mu * log(s2) + nu * log(c2));
     double prev = 0.0, cur = seed;
     double* row64 = tab64 + (long long)k * Kt;
-    float* row32 = tab32 + (long long)k * Kt;
-    long long id = kidx(l0, m1, m2);
-    row64[id] = cur;
-    row32[id] = (float)cur;
+    float* row32 = tab32 + (long long)k * Kp + plane_pos(plane_k(m2, n), n);
+    row64[kidx(l0, m1, m2)] = cur;
+    if (m1 >= 0) row32[(long long)qlm(l0, m1) * n] = (float)cur;
     for (int l = l0; l < Lt; ++l) {
       double A, B, C;
       recurrence_terms(l, m1, m2, A, B, C);
       const double nxt = B * (cb - C) * cur - A * prev;
       prev = cur;
       cur = nxt;
-      id = kidx(l + 1, m1, m2);
-      row64[id] = cur;
-      row32[id] = (float)cur;
+      row64[kidx(l + 1, m1, m2)] = cur;
+      if (m1 >= 0) row32[(long long)qlm(l + 1, m1) * n] = (float)cur;
     }
   }
 }
@@ -165,8 +166,6 @@ __global__ void k_sph_prep(const SphPair* __restrict__ pd, const double* __restr
 // recurrence in l with tabulated coefficients (k_leg_table).  An atom at the
 // centre takes theta = phi = 0 (only l = 0 survives there).
 // ---------------------------------------------------------------------------
-FO_HD inline int qlm(int l, int m) { return l * (l + 1) / 2 + m; }
-
 // XCD-aware decode of a 1-D launch of 8 ceil(P/8) per workgroups over
 // (pair, item): workgroup w serves pair (w % 8) + 8 (w / (8 per)) and item
 // (w / 8) % per.  Workgroups are dealt to the 8 XCDs round-robin, so all
@@ -381,7 +380,7 @@ template <int NCT>
 FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* __restrict__ grp,
                                const double2* __restrict__ Y1, const double2* __restrict__ Y2,
                                const double* __restrict__ R, double sc,
-                               double2* __restrict__ I64, float2* __restrict__ I32,
+                               double2* __restrict__ I64, float2* __restrict__ I32, int n,
                                ContractLds<NCT>& S) {
   constexpr int MP = 16 * NCT;            // padded columns of Y2
   constexpr int NY = kCT * MP / 256;      // staged Y2 entries per thread
@@ -550,17 +549,23 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
         const double2 cv = cb[r][mc];
         const double2 v = make_double2(norm * cv.x, norm * cv.y);
         I64[id] = v;
-        I32[id] = make_float2((float)v.x, (float)v.y);
+        I32[d.c32_off + (long long)qlm(l, m1) * n + plane_pos(plane_k(m2, n), n)] =
+            make_float2((float)v.x, (float)v.y);
         if (m1 > 0) {
           const long long im = d.coef_off + kidx(l, -m1, -m2);
           const double sg = ((m1 + m2) & 1) ? -1.0 : 1.0;
-          const double2 u = make_double2(sg * v.x, -sg * v.y);
-          I64[im] = u;
-          I32[im] = make_float2((float)u.x, (float)u.y);
+          I64[im] = make_double2(sg * v.x, -sg * v.y);
         }
       }
     }
     __syncthreads();
+  }
+  // plane-layout entries |m2| > l of the block's rows are zero
+  for (int e = t; e < rows * n; e += 256) {
+    const int r = e / n, k = e - r * n;
+    const int m2 = k < n / 2 ? k : k - n;
+    if (abs(m2) > l)
+      I32[d.c32_off + (long long)qlm(l, m1a + r) * n + plane_pos(k, n)] = make_float2(0.f, 0.f);
   }
 }
 
@@ -570,7 +575,7 @@ __global__ void __launch_bounds__(256, 3) k_sph_contract_mfma(
     const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
     const double2* __restrict__ Y1, const double2* __restrict__ Y2,
     const double* __restrict__ R, const double* __restrict__ scale_eff,
-    double2* __restrict__ I64, float2* __restrict__ I32, int P, int nbl) {
+    double2* __restrict__ I64, float2* __restrict__ I32, int n, int P, int nbl) {
   __shared__ ContractLds<NCTMAX> S;
   int pp, b;
   if (!xcd_decode(P, nbl, pp, b)) return;
@@ -584,7 +589,7 @@ __global__ void __launch_bounds__(256, 3) k_sph_contract_mfma(
   }
   if (l > L) return;
   const double sc = scale_eff[pp];
-  contract_mfma_body<NCTMAX>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, S);
+  contract_mfma_body<NCTMAX>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, n, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -822,16 +827,17 @@ FO_DEV void two_pass_exchange(float2 (&v)[N2], float2 (&u)[N1], int lane, float2
   }
 }
 
-// Raw buffer access (gfx9 descriptor): loads at byte offsets at or beyond
-// `bytes` return zero.
-constexpr int kBufOut = 0x0FFFFFFF;  // element offset beyond any buffer used here
+// Raw buffer access (gfx9 descriptor): a per-lane byte offset plus a
+// wavefront-uniform one (scalar register), 16 bytes per lane; offsets at or
+// beyond `bytes` read zero.
 FO_DEV __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, int bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
 }
-FO_DEV float2 load_f2(__amdgpu_buffer_rsrc_t r, int idx) {
-  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
-  const u2 w = __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, 0);
-  return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+FO_DEV float4 load_f4(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const u4 w = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z),
+                     __uint_as_float(w.w));
 }
 
 template <int N1, int N2>
@@ -858,13 +864,15 @@ constexpr int fft_tpf() {
 // ---------------------------------------------------------------------------
 template <int N1, int N2, int MODE>
 __global__ void __launch_bounds__(256) k_isoft_fft(
-    const SphPair* __restrict__ pd, int P, long long Kt, const float* __restrict__ tab32,
+    const SphPair* __restrict__ pd, int P, long long Kp, const float* __restrict__ tab32,
     const float2* __restrict__ twg, const float2* __restrict__ IA,
     const float2* __restrict__ IB, int norient, PlaneCand* __restrict__ cand,
     float* __restrict__ grid) {
   constexpr int n = N1 * N2;
   constexpr int TPF = fft_tpf<N1, N2>();
+  static_assert(TPF == N1 && N2 % 4 == 0, "lanes own N2 contiguous plane-layout entries");
   constexpr int NG = 256 / TPF;           // lane groups per workgroup
+  constexpr int GPW = 64 / TPF;           // lane groups (rows) per wavefront
   constexpr int XS = n + n / N1;          // padded exchange row (non-square)
   extern __shared__ __align__(16) unsigned char smem[];
   const int wg = blockIdx.x;
@@ -879,80 +887,102 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
   float* cv = (float*)(X + (N1 == N2 ? 0 : NG * XS));     // [4][2][6]
   int* ci = (int*)(cv + 48);                              // [4][2][4]
   const int grp = t / TPF, lane = t % TPF;
+  const int wrow = __builtin_amdgcn_readfirstlane(t >> 6) * GPW;  // first row of the wavefront
   float2* xr = X + grp * XS;
   float2 tw[N2];
 #pragma unroll
   for (int k2 = 0; k2 < N2; ++k2) tw[k2] = twg[(lane * k2) % n];
-  const float* tb = tab32 + (long long)ib * Kt;
-  const float2* ia = IA + d.coef_off;
-  const float2* ibb = IB + d.coef_off;
   const int no = norient;
+  // plane-layout operands of this pair and beta: rows q(l, m1) of n entries,
+  // lane k1's entries k1 + N1*k2 contiguous at N2*k1
+  const int rbytes = (int)plane_rows(L) * n;
+  const __amdgpu_buffer_rsrc_t rtb = buf_rsrc(tab32 + (long long)ib * Kp, rbytes * 4);
+  const __amdgpu_buffer_rsrc_t ria = buf_rsrc(IA + d.c32_off, rbytes * 8);
+  const __amdgpu_buffer_rsrc_t rib =
+      buf_rsrc(MODE != 0 ? (const void*)(IB + d.c32_off) : (const void*)(IA + d.c32_off), rbytes * 8);
 
   // ---- gamma pass (rows m1 in rounds of NG)
   for (int m1_0 = 0; m1_0 <= L; m1_0 += NG) {
     const int m1 = m1_0 + grp;
     const bool row = m1 <= L;
-    // even / odd-l partial sums (alignment mode) or the A / B sums
-    float2 e[N2], od[N2];
+    const int m1w = m1_0 + wrow;  // wavefront-uniform
+    // sums over degrees l = m1w, m1w + 2, ... (A) and m1w + 1, m1w + 3, ... (B):
+    // the even / odd-l partial sums (alignment mode) in some order, or, in
+    // general mode, the two coefficient sets (A, B) over all degrees
+    float2 a[N2], bq[N2];
 #pragma unroll
-    for (int j2 = 0; j2 < N2; ++j2) e[j2] = od[j2] = make_float2(0.f, 0.f);
-    if (row && lane < N1) {
-      // entry j2: m2 (clamped into [-l, l] for the loads, which are then
-      // always in bounds) and the first degree |m2| at which it exists;
-      // absent entries get a zero table weight
-      int m2v[N2], lmin[N2];
+    for (int j2 = 0; j2 < N2; ++j2) a[j2] = bq[j2] = make_float2(0.f, 0.f);
+    const int vo = (m1 * n + lane * N2) * 4;  // lane's entries in a degree block, table bytes
+    // one degree: s += d^l * I^l over the lane's N2 entries
+    auto step = [&](int l, float2(&sa)[N2], float2(&sb)[N2]) {
+      const int so = qlm(l, 0) * n * 4;  // degree block, uniform
+      float dv[N2];
+      float2 ca[N2];
 #pragma unroll
-      for (int j2 = 0; j2 < N2; ++j2) {
-        const int k = lane + N1 * j2;
-        const bool has = k <= L || k >= n - L;
-        m2v[j2] = k <= L ? k : (k >= n - L ? k - n : 0);
-        lmin[j2] = has ? abs(m2v[j2]) : 1 << 20;
+      for (int i = 0; i < N2 / 4; ++i) {
+        const float4 w = load_f4(rtb, vo + 16 * i, so);
+        dv[4 * i] = w.x;
+        dv[4 * i + 1] = w.y;
+        dv[4 * i + 2] = w.z;
+        dv[4 * i + 3] = w.w;
       }
-      const int kl = (int)kcount(L);
-      const __amdgpu_buffer_rsrc_t rtb = buf_rsrc(tb, kl * 4);
-      const __amdgpu_buffer_rsrc_t ria = buf_rsrc(ia, kl * 8);
-      const __amdgpu_buffer_rsrc_t rib = buf_rsrc(MODE != 0 ? (const void*)ibb : (const void*)ia, kl * 8);
-      int base = m1 * (4 * m1 * m1 - 1) / 3;  // kbase(m1)
-      for (int l = m1; l <= L; ++l) {
-        const int rb = base + (m1 + l) * (2 * l + 1) + l;  // kidx(l, m1, 0)
-        base += (2 * l + 1) * (2 * l + 1);
-        float dv[N2];
-        float2 ca[N2];
-        // buffer loads with 32-bit offsets; an absent entry gets an offset
-        // beyond the buffer, which the hardware range check reads as zero
-        int idx[N2];
+#pragma unroll
+      for (int i = 0; i < N2 / 2; ++i) {
+        const float4 w = load_f4(ria, 2 * vo + 16 * i, 2 * so);
+        ca[2 * i] = make_float2(w.x, w.y);
+        ca[2 * i + 1] = make_float2(w.z, w.w);
+      }
+      if constexpr (MODE != 0) {
+        float2 cb[N2];
+#pragma unroll
+        for (int i = 0; i < N2 / 2; ++i) {
+          const float4 w = load_f4(rib, 2 * vo + 16 * i, 2 * so);
+          cb[2 * i] = make_float2(w.x, w.y);
+          cb[2 * i + 1] = make_float2(w.z, w.w);
+        }
 #pragma unroll
         for (int j2 = 0; j2 < N2; ++j2) {
-          idx[j2] = l >= lmin[j2] ? rb + m2v[j2] : kBufOut;
-          dv[j2] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rtb, idx[j2] * 4, 0, 0));
-          ca[j2] = load_f2(ria, idx[j2]);
+          sa[j2].x += dv[j2] * ca[j2].x;
+          sa[j2].y += dv[j2] * ca[j2].y;
+          sb[j2].x += dv[j2] * cb[j2].x;
+          sb[j2].y += dv[j2] * cb[j2].y;
         }
-        if constexpr (MODE != 0) {
+      } else {
 #pragma unroll
-          for (int j2 = 0; j2 < N2; ++j2) {
-            const float2 cb = load_f2(rib, idx[j2]);
-            e[j2].x += dv[j2] * ca[j2].x;
-            e[j2].y += dv[j2] * ca[j2].y;
-            od[j2].x += dv[j2] * cb.x;
-            od[j2].y += dv[j2] * cb.y;
-          }
-        } else {
-          // l parity is uniform over the workgroup
-          if (l & 1) {
-#pragma unroll
-            for (int j2 = 0; j2 < N2; ++j2) {
-              od[j2].x += dv[j2] * ca[j2].x;
-              od[j2].y += dv[j2] * ca[j2].y;
-            }
-          } else {
-#pragma unroll
-            for (int j2 = 0; j2 < N2; ++j2) {
-              e[j2].x += dv[j2] * ca[j2].x;
-              e[j2].y += dv[j2] * ca[j2].y;
-            }
-          }
+        for (int j2 = 0; j2 < N2; ++j2) {
+          sa[j2].x += dv[j2] * ca[j2].x;
+          sa[j2].y += dv[j2] * ca[j2].y;
         }
       }
+    };
+    // the wavefront's rows start at degree m1w .. m1w + GPW - 1: row m1
+    // takes part from l = m1 on; from l = m1w + GPW every row does.  Rows
+    // beyond L load nothing, so every access stays inside rows(L) * n.
+    if (row) {
+      const int lp = min(m1w + GPW, L + 1);
+      int l = m1w;
+      if constexpr (MODE == 0) {
+        for (; l < lp; l += 2) {
+          if (m1 <= l) step(l, a, a);
+          if (l + 1 <= L && m1 <= l + 1) step(l + 1, bq, bq);
+        }
+        for (; l <= L; l += 2) {
+          step(l, a, a);
+          if (l + 1 <= L) step(l + 1, bq, bq);
+        }
+      } else {
+        for (; l < lp; ++l)
+          if (m1 <= l) step(l, a, bq);
+        for (; l <= L; ++l) step(l, a, bq);
+      }
+    }
+    // alignment mode: e = even-l sums, od = odd-l sums
+    float2 e[N2], od[N2];
+    const bool swap = MODE == 0 && (m1w & 1);
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) {
+      e[j2] = swap ? bq[j2] : a[j2];
+      od[j2] = swap ? a[j2] : bq[j2];
     }
     for (int o = 0; o < no; ++o) {
       float2 v[N2], u[N1];
@@ -1024,7 +1054,7 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
 // Direct-DFT planes (any n): the beta contraction per (m1 >= 0, m2) entry,
 // then the gamma and alpha passes as direct DFTs through LDS.
 __global__ void __launch_bounds__(256) k_isoft_plane(
-    const SphPair* __restrict__ pd, int n, long long Kt, const float* __restrict__ tab32,
+    const SphPair* __restrict__ pd, int n, long long Kp, const float* __restrict__ tab32,
     const float2* __restrict__ twg, const float2* __restrict__ IA,
     const float2* __restrict__ IB, int mode, int norient, PlaneCand* __restrict__ cand,
     float* __restrict__ grid) {
@@ -1038,17 +1068,19 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
   float* cv = (float*)(T + 2 * (L + 1) * n);   // candidate exchange
   int* ci = (int*)(cv + 48);
   for (int k = t; k < n; k += bd) tw[k] = twg[k];
-  const float* tb = tab32 + (long long)ib * Kt;
-  const float2* ia = IA + d.coef_off;
-  const float2* ibb = IB + d.coef_off;
+  // plane-layout operands: entry (l, m1, m2) at row q(l, m1), position pos
+  const float* tb = tab32 + (long long)ib * Kp;
+  const float2* ia = IA + d.c32_off;
+  const float2* ibb = IB + d.c32_off;
   // beta contraction
   for (int e = t; e < (L + 1) * W; e += bd) {
     const int m1 = e / W, m2 = e % W - L;
     const int l0 = max(m1, abs(m2));
+    const int pos = plane_pos(plane_k(m2, n), n);
     float2 s0 = make_float2(0.f, 0.f), s1 = make_float2(0.f, 0.f);
     if (mode == 0) {
       for (int l = l0; l <= L; ++l) {
-        const long long id = kidx(l, m1, m2);
+        const long long id = (long long)qlm(l, m1) * n + pos;
         const float dv = tb[id];
         const float2 c = ia[id];
         if (l & 1) {
@@ -1063,7 +1095,7 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
       S[(L + 1) * W + e] = make_float2(s0.x - s1.x, s0.y - s1.y);
     } else {
       for (int l = l0; l <= L; ++l) {
-        const long long id = kidx(l, m1, m2);
+        const long long id = (long long)qlm(l, m1) * n + pos;
         const float dv = tb[id];
         const float2 ca = ia[id], cb = ibb[id];
         s0.x += dv * ca.x;
@@ -1864,15 +1896,15 @@ void launch_sph_radial(hipStream_t s, int P, long long maxjk, const SphPair* pd,
                      scale_eff, R);
 }
 
-void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, const SphGroup* grp,
-                         const double2* Y1, const double2* Y2, const double* R,
-                         const double* scale_eff, double2* I64, float2* I32) {
+void launch_sph_contract(hipStream_t s, int P, int n, int Lmax, const SphPair* pd,
+                         const SphGroup* grp, const double2* Y1, const double2* Y2,
+                         const double* R, const double* scale_eff, double2* I64, float2* I32) {
   int nb = 0;
   for (int l = 0; l <= Lmax; ++l) nb += (l + 1 + kMB - 1) / kMB;
   const dim3 g(xcd_grid(P, nb)), b(256);
 #define FO_CL(K)                                                                                \
   hipLaunchKernelGGL(k_sph_contract_mfma<K>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,  \
-                     I32, P, nb)
+                     I32, n, P, nb)
   if (Lmax <= 15)
     FO_CL(2);
   else if (Lmax <= 31)
@@ -1884,14 +1916,9 @@ void launch_sph_contract(hipStream_t s, int P, int Lmax, const SphPair* pd, cons
 
 // two-pass factorisation n = N1 N2 of the power-of-two grids (0: direct DFT)
 static bool fft_shape(int n, int& n1, int& n2) {
-  switch (n) {
-    case 16: n1 = 4; n2 = 4; return true;
-    case 32: n1 = 8; n2 = 4; return true;
-    case 64: n1 = 8; n2 = 8; return true;
-    case 128: n1 = 16; n2 = 8; return true;
-    case 256: n1 = 16; n2 = 16; return true;
-    default: n1 = n2 = 0; return false;
-  }
+  n1 = plane_n1(n);
+  n2 = n1 ? n / n1 : 0;
+  return n1 != 0;
 }
 
 size_t isoft_plane_smem(int n, int Lmax, int use_fft) {
@@ -1914,24 +1941,24 @@ int isoft_plane_use_fft(int n, int Lmax, size_t max_lds) {
   return fft_shape(n, n1, n2) && isoft_plane_smem(n, Lmax, 1) <= max_lds ? 1 : 0;
 }
 
-void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long long Kt,
+void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long long Kp,
                         const SphPair* pd, const float* tab32, const float2* tw32,
                         const float2* IA, const float2* IB, int mode, int norient,
                         PlaneCand* cand, float* grid) {
   const size_t sm = isoft_plane_smem(n, Lmax, use_fft);
   int n1, n2;
   if (!use_fft || !fft_shape(n, n1, n2)) {
-    hipLaunchKernelGGL(k_isoft_plane, dim3(n, P), dim3(256), sm, s, pd, n, Kt, tab32, tw32, IA,
+    hipLaunchKernelGGL(k_isoft_plane, dim3(n, P), dim3(256), sm, s, pd, n, Kp, tab32, tw32, IA,
                        IB, mode, norient, cand, grid);
     return;
   }
   const dim3 g((unsigned)(8 * ((P + 7) / 8) * n)), b(256);
 #define FO_FFT(N1, N2)                                                                         \
   if (mode == 0)                                                                                \
-    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 0>), g, b, sm, s, pd, P, Kt, tab32, tw32, IA, IB,   \
+    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 0>), g, b, sm, s, pd, P, Kp, tab32, tw32, IA, IB,   \
                        norient, cand, grid);                                                    \
   else                                                                                          \
-    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 1>), g, b, sm, s, pd, P, Kt, tab32, tw32, IA, IB,   \
+    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 1>), g, b, sm, s, pd, P, Kp, tab32, tw32, IA, IB,   \
                        norient, cand, grid)
   switch (n) {
     case 16: FO_FFT(4, 4); break;
@@ -1975,11 +2002,25 @@ void launch_sph_finish(hipStream_t s, int P, int lap_nmax, const SphPair* pd, co
 // ---------------------------------------------------------------------------
 namespace fo {
 
-// FP64 -> FP32 copy of compact coefficients
-__global__ void k_c64_to_c32(long long n, const double2* __restrict__ a, float2* __restrict__ b) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    b[i] = make_float2((float)a[i].x, (float)a[i].y);
+// FP32 plane-layout copy of compact FP64 coefficients, one thread per
+// (row q(l, m1), k) entry of pair blockIdx.y (zero where |m2| > l)
+__global__ void k_coef_to_plane(const SphPair* __restrict__ pd, int n,
+                                const double2* __restrict__ a, float2* __restrict__ b) {
+  const SphPair d = pd[blockIdx.y];
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= plane_rows(d.L) * n) return;
+  const int row = (int)(e / n), k = (int)(e % n);
+  int l = (int)((sqrt(8.0 * row + 1.0) - 1.0) * 0.5);
+  while (qlm(l + 1, 0) <= row) ++l;
+  while (qlm(l, 0) > row) --l;
+  const int m1 = row - qlm(l, 0);
+  const int m2 = k < n / 2 ? k : k - n;
+  float2 v = make_float2(0.f, 0.f);
+  if (abs(m2) <= l) {
+    const double2 c = a[d.coef_off + kidx(l, m1, m2)];
+    v = make_float2((float)c.x, (float)c.y);
+  }
+  b[d.c32_off + (long long)row * n + plane_pos(k, n)] = v;
 }
 
 // getEnergyGradient (fastoverlap/sphericalAlignment.py:98-100) for one
@@ -2091,9 +2132,11 @@ __global__ void k_ylm_angles(int L, int npts, const double* __restrict__ theta,
   }
 }
 
-void launch_c64_to_c32(hipStream_t s, long long n, const double2* a, float2* b) {
-  const int blocks = (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 4096));
-  hipLaunchKernelGGL(k_c64_to_c32, dim3(blocks), dim3(256), 0, s, n, a, b);
+void launch_coef_to_plane(hipStream_t s, int P, int n, int Lmax, const SphPair* pd,
+                          const double2* a, float2* b) {
+  const long long e = plane_rows(Lmax) * n;
+  hipLaunchKernelGGL(k_coef_to_plane, dim3((unsigned)((e + 255) / 256), P), dim3(256), 0, s, pd,
+                     n, a, b);
 }
 void launch_sph_energy(hipStream_t s, int L, const double2* Iconj, double a, double b, double y,
                        const double* rec, const double* seedu, int Lt, double* out) {
