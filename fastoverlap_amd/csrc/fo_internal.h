@@ -32,15 +32,17 @@ struct SphGroup {
   long long rad_off;   // (L+1) * jk_off : offset inside the pair's radial block
 };
 
-// Per (pair, orientation, beta-plane) candidate record written by the
-// iSOFT plane kernel.
+// Per (pair, orientation, plane) candidate record written by the plane
+// kernels: the plane's values are split into kCandPerPlane parts (the
+// spherical kernels: one per wavefront) and entry r lists the largest value
+// of part r with a bound on the part's other values.  Entries are unordered;
+// parts with no values hold -inf.
 constexpr int kCandPerPlane = 4;
 constexpr int kMaxCand = 64;     // FP64-re-evaluated candidates listed per grid
 struct PlaneCand {
-  float val[kCandPerPlane];  // largest grid values of the plane (descending)
-  int idx[kCandPerPlane];    // their flat grid indices (ia*n+ib)*n+ig
-  float rest;                // upper bound on every other value of the plane
-  int pad[3];
+  float val[kCandPerPlane];   // largest value of each part
+  int idx[kCandPerPlane];     // its flat grid index (ia*n+ib)*n+ig
+  float rest[kCandPerPlane];  // upper bound on every other value of the part
 };
 
 // Per (pair, orientation) peak record written by the selection kernel.

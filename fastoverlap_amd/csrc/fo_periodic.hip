@@ -216,7 +216,9 @@ __global__ void __launch_bounds__(256) k_per_plane(
   }
   int dummy = 0;
   block_argmax(dm, dummy, rv, ri);
-  out.rest = fmaxf(fifth, dm);
+  // one part: the block's top kCandPerPlane, the bound in rest[0]
+  out.rest[0] = fmaxf(fifth, dm);
+  for (int r = 1; r < kCandPerPlane; ++r) out.rest[r] = -INFINITY;
   if (t == 0) cand[(long long)p * F + x] = out;
 }
 
@@ -303,7 +305,8 @@ __global__ void __launch_bounds__(256) k_per_select(
   const PlaneCand* pc = cand + (long long)p * F;
   float mv = -INFINITY;
   int mi = 0x7fffffff;
-  for (int b = t; b < F; b += blockDim.x) argmax_merge(mv, mi, pc[b].val[0], pc[b].idx[0]);
+  for (int b = t; b < F; b += blockDim.x)
+    for (int r = 0; r < kCandPerPlane; ++r) argmax_merge(mv, mi, pc[b].val[r], pc[b].idx[r]);
   block_argmax(mv, mi, rv, ri);
   double A = 0.0;
   for (int e = t; e < NB * NB * NB; e += blockDim.x) A += hypot(c.C[e].x, c.C[e].y);
@@ -316,7 +319,9 @@ __global__ void __launch_bounds__(256) k_per_select(
   }
   __syncthreads();
   for (int b = t; b < F; b += blockDim.x) {
-    if ((double)pc[b].rest >= thr) {
+    float rest = pc[b].rest[0];
+    for (int r = 1; r < kCandPerPlane; ++r) rest = fmaxf(rest, pc[b].rest[r]);
+    if ((double)rest >= thr) {
       int q = atomicAdd(&s_ovf, 1);
       s_ovfl[q] = b;
     } else {

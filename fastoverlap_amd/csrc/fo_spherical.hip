@@ -687,73 +687,6 @@ struct CandTrack {
   FO_DEV const CandOne& get(int o) const { return o == 0 ? c0 : c1; }
 };
 
-// Plane top-kCandPerPlane and a bound on every other value of the plane.
-// Each wavefront lists its kCandPerPlane best per-thread maxima (xor
-// butterflies, no barrier), its next one and the largest per-thread bound;
-// one barrier, then wavefront 0 merges the four lists the same way.
-// `cv` / `ci`: 4 waves x 2 orientations x 6 floats / 4 ints of LDS.
-FO_DEV void plane_candidates(const CandTrack& ct, int norient, float* cv, int* ci,
-                             PlaneCand* out, int ostride) {
-  const int t = threadIdx.x, wl = t & 63, w = t >> 6;
-  for (int o = 0; o < norient; ++o) {
-    const float mbv = o == 0 ? ct.c0.bv : ct.c1.bv;
-    const int mbi = o == 0 ? ct.c0.bi : ct.c1.bi;
-    bool taken = false;
-    for (int r = 0; r <= kCandPerPlane; ++r) {
-      float v = taken ? -INFINITY : mbv;
-      int i = taken ? 0x7fffffff : mbi;
-      wave_argmax(v, i);
-      if (wl == 0) {
-        cv[(w * 2 + o) * 6 + r] = v;
-        if (r < kCandPerPlane) ci[(w * 2 + o) * 4 + r] = i;
-      }
-      if (!taken && i == mbi && v == mbv) taken = true;
-    }
-    const float dmw = wave_max(o == 0 ? ct.c0.dm : ct.c1.dm);
-    if (wl == 0) cv[(w * 2 + o) * 6 + 5] = dmw;
-  }
-  __syncthreads();
-  if (w == 0) {
-    // lane j < 16 holds entry j of the four wave lists
-    const int ww = wl >> 2, r = wl & 3;
-    for (int o = 0; o < norient; ++o) {
-      float v = wl < 16 ? cv[(ww * 2 + o) * 6 + r] : -INFINITY;
-      int i = wl < 16 ? ci[(ww * 2 + o) * 4 + r] : 0x7fffffff;
-      const float rw = wl < 4 ? fmaxf(cv[(wl * 2 + o) * 6 + kCandPerPlane], cv[(wl * 2 + o) * 6 + 5])
-                              : -INFINITY;
-      float rest = wave_max(rw);
-      // lane q < kCandPerPlane keeps the q-th best
-      float keepv = -INFINITY;
-      int keepi = 0x7fffffff;
-      bool taken = false;
-#pragma unroll
-      for (int q = 0; q <= kCandPerPlane; ++q) {
-        float bv = taken ? -INFINITY : v;
-        int bi = taken ? 0x7fffffff : i;
-        wave_argmax(bv, bi);
-        if (q < kCandPerPlane) {
-          if (wl == q) {
-            keepv = bv;
-            keepi = bi;
-          }
-        } else {
-          rest = fmaxf(rest, bv);
-        }
-        if (!taken && bi == i && bv == v) taken = true;
-      }
-      PlaneCand* dst = out + (long long)o * ostride;
-      if (wl < kCandPerPlane) {
-        dst->val[wl] = keepv;
-        dst->idx[wl] = keepi;
-      }
-      if (wl == 0) {
-        dst->rest = rest;
-        dst->pad[0] = dst->pad[1] = dst->pad[2] = 0;
-      }
-    }
-  }
-}
-
 // Ordering of LDS accesses between lanes of one wavefront (no s_barrier).
 FO_DEV void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -766,6 +699,65 @@ FO_DEV void wave_sync() {
 template <int CTRL>
 FO_DEV float dpp_mov(float x) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, true));
+}
+
+// Wavefront all-reduce of (value, flat index) pairs, larger value first and
+// ties to the smaller index: butterflies inside each row of 16 lanes by DPP
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror),
+// then the four row results through readlane.  Every lane must be active.
+template <int CTRL>
+FO_DEV void argmax_dpp_step(float& v, int& i) {
+  const float ov = dpp_mov<CTRL>(v);
+  const int oi = __builtin_amdgcn_update_dpp(0, i, CTRL, 0xf, 0xf, true);
+  argmax_merge(v, i, ov, oi);
+}
+FO_DEV void wave_argmax_dpp(float& v, int& i) {
+  argmax_dpp_step<0xB1>(v, i);
+  argmax_dpp_step<0x4E>(v, i);
+  argmax_dpp_step<0x141>(v, i);
+  argmax_dpp_step<0x140>(v, i);
+  float rv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+  int ri = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+  for (int r = 16; r < 64; r += 16)
+    argmax_merge(rv, ri, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), r)),
+                 __builtin_amdgcn_readlane(i, r));
+  v = rv;
+  i = ri;
+}
+FO_DEV float wave_max_dpp(float v) {
+  v = fmaxf(v, dpp_mov<0xB1>(v));
+  v = fmaxf(v, dpp_mov<0x4E>(v));
+  v = fmaxf(v, dpp_mov<0x141>(v));
+  v = fmaxf(v, dpp_mov<0x140>(v));
+  float r = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+#pragma unroll
+  for (int q = 16; q < 64; q += 16)
+    r = fmaxf(r, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q)));
+  return r;
+}
+
+// Plane candidates of a 256-thread workgroup: wavefront w fills entry w of
+// the plane's record with its largest value and a bound on its other values
+// (no LDS, no barrier).  Every thread must call it.
+FO_DEV void plane_candidates(const CandTrack& ct, int norient, PlaneCand* out, int ostride) {
+  const int wl = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int o = 0; o < norient; ++o) {
+    const float bv = o == 0 ? ct.c0.bv : ct.c1.bv;
+    const int bi = o == 0 ? ct.c0.bi : ct.c1.bi;
+    const float dm = o == 0 ? ct.c0.dm : ct.c1.dm;
+    float v = bv;
+    int i = bi;
+    wave_argmax_dpp(v, i);
+    // the lane that holds the winner bounds the rest by its second value
+    const float rest = wave_max_dpp(i == bi && v == bv ? dm : fmaxf(bv, dm));
+    if (wl == 0) {
+      PlaneCand* dst = out + (long long)o * ostride;
+      dst->val[w] = v;
+      dst->idx[w] = i;
+      dst->rest[w] = rest;
+    }
+  }
 }
 
 // One stage of the lane transpose: lanes i and i ^ S (S < 16, groups
@@ -864,7 +856,7 @@ constexpr int fft_tpf() {
 // ---------------------------------------------------------------------------
 template <int N1, int N2, int MODE>
 __global__ void __launch_bounds__(256) k_isoft_fft(
-    const SphPair* __restrict__ pd, int P, long long Kp, const float* __restrict__ tab32,
+    const SphPair* __restrict__ pd, int P, int C, long long Kp, const float* __restrict__ tab32,
     const float2* __restrict__ twg, const float2* __restrict__ IA,
     const float2* __restrict__ IB, int norient, PlaneCand* __restrict__ cand,
     float* __restrict__ grid) {
@@ -875,17 +867,18 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
   constexpr int GPW = 64 / TPF;           // lane groups (rows) per wavefront
   constexpr int XS = n + n / N1;          // padded exchange row (non-square)
   extern __shared__ __align__(16) unsigned char smem[];
+  // workgroups are dealt to the 8 XCDs round-robin; each XCD walks its
+  // pairs in chunks of C, beta-major inside a chunk, so the coefficient
+  // blocks of C pairs and a few beta slices of the table share its L2
   const int wg = blockIdx.x;
-  const int p = (wg & 7) + 8 * (wg / (8 * n));
-  const int ib = (wg >> 3) % n;
-  if (p >= P) return;  // whole workgroup: padding of P to a multiple of 8
+  const int sx = wg >> 3, chunk = sx / (C * n), rr = sx - chunk * C * n;
+  const int ib = rr / C, p = (wg & 7) + 8 * (chunk * C + rr % C);
+  if (p >= P) return;  // whole workgroup: padding of P to a multiple of 8 C
   const SphPair d = pd[p];
   const int L = d.L, t = threadIdx.x;
   const int LS = (L + 1) | 1;
   float2* T = (float2*)smem;                              // [2][n][LS]
-  float2* X = T + 2 * n * LS;                             // [NG][XS]
-  float* cv = (float*)(X + (N1 == N2 ? 0 : NG * XS));     // [4][2][6]
-  int* ci = (int*)(cv + 48);                              // [4][2][4]
+  float2* X = T + 2 * n * LS;                             // [NG][XS] (non-square)
   const int grp = t / TPF, lane = t % TPF;
   const int wrow = __builtin_amdgcn_readfirstlane(t >> 6) * GPW;  // first row of the wavefront
   float2* xr = X + grp * XS;
@@ -1048,7 +1041,7 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
       }
     }
   }
-  plane_candidates(ct, no, cv, ci, cand + ((long long)p * no) * n + ib, n);
+  plane_candidates(ct, no, cand + ((long long)p * no) * n + ib, n);
 }
 
 // Direct-DFT planes (any n): the beta contraction per (m1 >= 0, m2) entry,
@@ -1065,8 +1058,6 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
   float2* tw = (float2*)smem;                 // [n]
   float2* S = tw + n;                          // [2][L+1][W]
   float2* T = S + 2 * (L + 1) * W;             // [2][L+1][n]
-  float* cv = (float*)(T + 2 * (L + 1) * n);   // candidate exchange
-  int* ci = (int*)(cv + 48);
   for (int k = t; k < n; k += bd) tw[k] = twg[k];
   // plane-layout operands: entry (l, m1, m2) at row q(l, m1), position pos
   const float* tb = tab32 + (long long)ib * Kp;
@@ -1150,7 +1141,7 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
       consider(o, a, g, To[g].x + 2.f * acc);
     }
   }
-  plane_candidates(ct, norient, cv, ci, cand + ((long long)p * norient) * n + ib, n);
+  plane_candidates(ct, norient, cand + ((long long)p * norient) * n + ib, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -1353,7 +1344,8 @@ __global__ void __launch_bounds__(256) k_sph_select(
   // FP32 maximum over planes
   float mv = -INFINITY;
   int mi = 0x7fffffff;
-  for (int b = t; b < n; b += blockDim.x) argmax_merge(mv, mi, pc[b].val[0], pc[b].idx[0]);
+  for (int b = t; b < n; b += blockDim.x)
+    for (int r = 0; r < kCandPerPlane; ++r) argmax_merge(mv, mi, pc[b].val[r], pc[b].idx[r]);
   {
     __shared__ float rv[16];
     __shared__ int ri[16];
@@ -1381,7 +1373,9 @@ __global__ void __launch_bounds__(256) k_sph_select(
   }
   __syncthreads();
   for (int b = t; b < n; b += blockDim.x) {
-    if ((double)pc[b].rest >= thr) {
+    float rest = pc[b].rest[0];
+    for (int r = 1; r < kCandPerPlane; ++r) rest = fmaxf(rest, pc[b].rest[r]);
+    if ((double)rest >= thr) {
       int q = atomicAdd(&s_ovf, 1);
       s_ovfl[q] = b;
     } else {
@@ -1922,17 +1916,15 @@ static bool fft_shape(int n, int& n1, int& n2) {
 }
 
 size_t isoft_plane_smem(int n, int Lmax, int use_fft) {
-  const size_t cand = 48 * sizeof(float) + 32 * sizeof(int);
   int n1, n2;
   if (use_fft && fft_shape(n, n1, n2)) {
     const size_t ls = (size_t)((Lmax + 1) | 1);
     const int tpf = std::max(n1, n2);
     const size_t x = n1 == n2 ? 0 : (size_t)(256 / tpf) * (n + n / n1);
-    return sizeof(float2) * (2 * (size_t)n * ls + x) + cand;
+    return sizeof(float2) * (2 * (size_t)n * ls + x);
   }
   return sizeof(float2) * ((size_t)n + 2 * (size_t)(Lmax + 1) * (2 * Lmax + 1) +
-                           2 * (size_t)(Lmax + 1) * n) +
-         cand;
+                           2 * (size_t)(Lmax + 1) * n);
 }
 
 // 1 if the two-pass transform applies to n and fits max_lds, else 0.
@@ -1952,14 +1944,16 @@ void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long
                        IB, mode, norient, cand, grid);
     return;
   }
-  const dim3 g((unsigned)(8 * ((P + 7) / 8) * n)), b(256);
+  // pair chunk per XCD (16 measured best on the 4096-pair n=64 class)
+  const int C = std::max(1, std::min(16, (P + 7) / 8));
+  const dim3 g((unsigned)(8 * ((P + 8 * C - 1) / (8 * C)) * C * n)), b(256);
 #define FO_FFT(N1, N2)                                                                         \
   if (mode == 0)                                                                                \
-    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 0>), g, b, sm, s, pd, P, Kp, tab32, tw32, IA, IB,   \
-                       norient, cand, grid);                                                    \
+    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 0>), g, b, sm, s, pd, P, C, Kp, tab32, tw32, IA,    \
+                       IB, norient, cand, grid);                                                \
   else                                                                                          \
-    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 1>), g, b, sm, s, pd, P, Kp, tab32, tw32, IA, IB,   \
-                       norient, cand, grid)
+    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 1>), g, b, sm, s, pd, P, C, Kp, tab32, tw32, IA,    \
+                       IB, norient, cand, grid)
   switch (n) {
     case 16: FO_FFT(4, 4); break;
     case 32: FO_FFT(8, 4); break;
