@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -29,7 +30,7 @@ void launch_sph_prep(hipStream_t, int, const SphPair*, const double*, const doub
                      double*, double*, int);
 void launch_sph_ylm(hipStream_t, int, const SphPair*, const int*, const double*, const double*,
                     const double2*, const double*, double2*, double2*);
-void launch_sph_radial(hipStream_t, int, long long, const SphPair*, const SphGroup*, const int*,
+void launch_sph_radial(hipStream_t, int, int, long long, const SphPair*, const SphGroup*, const int*,
                        const double*, const double*, const double*, double*);
 void launch_sph_contract(hipStream_t, int, int, int, const SphPair*, const SphGroup*,
                          const double2*, const double2*, const double*, const double*, double2*,
@@ -328,6 +329,16 @@ int fo_context_memory(fo_context* c, unsigned long long* used, unsigned long lon
 // ---------------------------------------------------------------------------
 // spherical batch
 // ---------------------------------------------------------------------------
+// Pairs whose harmonics and radial kernels are built and contracted
+// together: the coefficient stage runs chunk by chunk through one reused
+// Y / K buffer sized for the largest chunk, so those intermediates stay in
+// the device's last-level cache instead of streaming through HBM.
+struct CoefChunk {
+  int p0 = 0, p1 = 0;   // pairs [p0, p1)
+  int Lmax = 0;         // max L over the chunk
+  long long maxjk = 0;  // max sum of group size^2 over the chunk
+};
+
 struct fo_sph_batch {
   fo_context* ctx = nullptr;
   hipStream_t stream = nullptr;
@@ -361,10 +372,13 @@ struct fo_sph_batch {
   long long scr_stride = 0;
   // graphs per stage mask
   std::map<int, hipGraphExec_t> graphs;
+  std::vector<CoefChunk> chunks;
   // timing
   bool timing = false;
   hipEvent_t ev[9] = {};
+  std::vector<hipEvent_t> cev;  // 3 per chunk + 1: ylm | radial | contraction boundaries
   bool ev_made = false;
+  bool cev_valid = false;       // the last timed run recorded cev
   // layout (byte offsets inside base) and ownership
   struct {
     size_t pd, grp, gath, pos1, pos2, X1c, X2c, X2R, X2out, scale, R, Y1, Y2, I64A, I32A, I64B,
@@ -387,16 +401,27 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
     launch_sph_prep(s, P, b->pd, b->pos1, b->pos2, b->X1c, b->X2c, b->scale, L.center);
   mark(1);
   if (stages & FO_STAGE_COEFF) {
-    launch_sph_ylm(s, P, b->pd, b->gath, b->X1c, b->X2c, b->rect->leg, b->rect->cm, b->Y1, b->Y2);
-    mark(2);
-    launch_sph_radial(s, P, L.maxjk, b->pd, b->grp, b->gath, b->X1c, b->X2c, b->scale, b->R);
-    mark(3);
-    launch_sph_contract(s, P, L.n, L.Lmax, b->pd, b->grp, b->Y1, b->Y2, b->R, b->scale, b->I64A,
-                        b->I32A);
-  } else {
-    mark(2);
-    mark(3);
+    auto cmark = [&](size_t i) {
+      if (tm) (void)hipEventRecord(b->cev[i], s);
+    };
+    for (size_t c = 0; c < b->chunks.size(); ++c) {
+      const CoefChunk& C = b->chunks[c];
+      const int Pc = C.p1 - C.p0;
+      const SphPair* pdc = b->pd + C.p0;
+      const double* scc = b->scale + C.p0;
+      cmark(3 * c);
+      launch_sph_ylm(s, Pc, pdc, b->gath, b->X1c, b->X2c, b->rect->leg, b->rect->cm, b->Y1, b->Y2);
+      cmark(3 * c + 1);
+      launch_sph_radial(s, Pc, C.Lmax, C.maxjk, pdc, b->grp, b->gath, b->X1c, b->X2c, scc, b->R);
+      cmark(3 * c + 2);
+      launch_sph_contract(s, Pc, L.n, C.Lmax, pdc, b->grp, b->Y1, b->Y2, b->R, scc, b->I64A,
+                          b->I32A);
+    }
+    cmark(3 * b->chunks.size());
   }
+  if (tm) b->cev_valid = (stages & FO_STAGE_COEFF) != 0;
+  mark(2);
+  mark(3);
   mark(4);
   if (stages & FO_STAGE_GRID) {
     launch_isoft_plane(s, P, L.n, L.Lmax, b->fft_chunk, b->tab->Kp, b->pd, b->tab->tab32,
@@ -422,6 +447,18 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
 
 namespace fo {
 
+// Byte budget of one coefficient chunk (harmonics of both structures plus
+// the radial kernels).  Unlimited by default (one chunk: the kernels need
+// many pairs per launch to fill the device); FO_COEF_CHUNK_MB caps the
+// batch footprint of these intermediates at the cost of throughput.
+static size_t coef_chunk_bytes() {
+  if (const char* e = getenv("FO_COEF_CHUNK_MB")) {
+    const long v = atol(e);
+    if (v > 0) return (size_t)v << 20;
+  }
+  return SIZE_MAX;
+}
+
 int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out) {
   if (!c || !d || !out) return set_err(FO_ERR_INVALID, "NULL argument");
   if (d->npairs < 1) return set_err(FO_ERR_INVALID, "npairs must be >= 1");
@@ -435,6 +472,8 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   b->flags = d->flags;
   int Lmax = 0, maxatoms = 0, maxgath = 0;
   long long maxjk = 0, Ntot = 0, Ktot = 0, C32tot = 0, radtot = 0, ylmtot = 0;
+  long long radmax = 0, ylmmax = 0;  // largest chunk
+  const size_t chunk_budget = coef_chunk_bytes();
   int gsum = 0, gatot = 0;
   for (int p = 0; p < P; ++p) gsum += d->ngroups[p];
   for (int q = 0; q < gsum; ++q) {
@@ -488,12 +527,25 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
     sp.ngath = ng;
     sp.coef_off = Ktot;
     sp.c32_off = C32tot;
-    sp.rad_off = radtot;
-    sp.ylm_off = ylmtot;
     Ktot += kcount(Lp);
     C32tot += plane_rows(Lp) * n;
-    radtot += (long long)(Lp + 1) * jk;
-    ylmtot += (long long)(Lp + 1) * (Lp + 2) / 2 * ng;
+    // coefficient chunks: K and Y offsets are relative to the pair's chunk
+    const long long rs = (long long)(Lp + 1) * jk, ys = (long long)(Lp + 1) * (Lp + 2) / 2 * ng;
+    const size_t cb = sizeof(double) * (size_t)(radtot + rs) + 2 * sizeof(double2) * (size_t)(ylmtot + ys);
+    if (b->chunks.empty() || (cb > chunk_budget && radtot + ylmtot > 0)) {
+      b->chunks.push_back(CoefChunk{p, p, 0, 0});
+      radtot = ylmtot = 0;
+    }
+    CoefChunk& ch = b->chunks.back();
+    ch.p1 = p + 1;
+    ch.Lmax = std::max(ch.Lmax, Lp);
+    ch.maxjk = std::max(ch.maxjk, jk);
+    sp.rad_off = radtot;
+    sp.ylm_off = ylmtot;
+    radtot += rs;
+    ylmtot += ys;
+    radmax = std::max(radmax, radtot);
+    ylmmax = std::max(ylmmax, ylmtot);
     Lmax = std::max(Lmax, Lp);
     maxatoms = std::max(maxatoms, N);
     maxgath = std::max(maxgath, ng);
@@ -544,9 +596,9 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   o.X2R = lay.add<double>(3 * Ntot);
   o.X2out = lay.add<double>(3 * Ntot);
   o.scale = lay.add<double>(P);
-  o.R = lay.add<double>(std::max(radtot, 1LL));
-  o.Y1 = lay.add<double2>(std::max(ylmtot, 1LL));
-  o.Y2 = lay.add<double2>(std::max(ylmtot, 1LL));
+  o.R = lay.add<double>(std::max(radmax, 1LL));
+  o.Y1 = lay.add<double2>(std::max(ylmmax, 1LL));
+  o.Y2 = lay.add<double2>(std::max(ylmmax, 1LL));
   o.I64A = lay.add<double2>(Ktot);
   o.I32A = lay.add<float2>(C32tot);
   o.I64B = general ? lay.add<double2>(Ktot) : 0;
@@ -646,8 +698,10 @@ int fo_sph_batch_destroy(fo_sph_batch* b) {
   if (!b) return FO_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-  if (b->ev_made)
+  if (b->ev_made) {
     for (auto& e : b->ev) (void)hipEventDestroy(e);
+    for (auto& e : b->cev) (void)hipEventDestroy(e);
+  }
   if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
   if (b->own_mem) ctx_free(b->ctx, b->base, b->bytes);
   delete b;
@@ -843,6 +897,8 @@ int fo_sph_batch_timing(fo_sph_batch* b, int enable, float* ms8) {
   if (enable >= 0) {
     if (enable && !b->ev_made) {
       for (auto& e : b->ev) FO_HIP(hipEventCreate(&e));
+      b->cev.resize(3 * b->chunks.size() + 1);
+      for (auto& e : b->cev) FO_HIP(hipEventCreate(&e));
       b->ev_made = true;
     }
     b->timing = enable != 0;
@@ -850,6 +906,16 @@ int fo_sph_batch_timing(fo_sph_batch* b, int enable, float* ms8) {
   if (ms8 && b->ev_made) {
     FO_HIP(hipStreamSynchronize(b->stream));
     for (int i = 0; i < 8; ++i) FO_HIP(hipEventElapsedTime(&ms8[i], b->ev[i], b->ev[i + 1]));
+    // coefficient stage (between events 1 and 4): harmonics / radial /
+    // contraction, each summed over the chunks
+    float t3[3] = {0.f, 0.f, 0.f};
+    if (b->cev_valid)
+      for (size_t i = 0; i + 1 < b->cev.size(); ++i) {
+        float ms = 0.f;
+        FO_HIP(hipEventElapsedTime(&ms, b->cev[i], b->cev[i + 1]));
+        t3[i % 3] += ms;
+      }
+    for (int k = 0; k < 3; ++k) ms8[1 + k] = t3[k];
   }
   return FO_OK;
 }

@@ -247,8 +247,11 @@ __global__ void k_leg_table(int Lt, double2* __restrict__ leg, double* __restric
 // for x < 1; otherwise Miller's backward recurrence started at
 // l_s = ceil(sqrt(L^2 + 40x)) + 10 (where the minimal-solution
 // contamination is below 1e-17) and normalised by i_0(x) e^{-x} =
-// -expm1(-2x)/(2x).  Layout: R[rad_off + grp.rad_off + (l*Ns + j)*Ns + k].
+// -expm1(-2x)/(2x).  One pass: the values f_l, l <= L, are kept in
+// registers (LB >= L, unrolled) until the normalisation is known at l = 0.
+// Layout: R[rad_off + grp.rad_off + (l*Ns + j)*Ns + k].
 // ---------------------------------------------------------------------------
+template <int LB>
 FO_DEV void radial_store(int L, double x, double gauss, double* out, long long stride) {
   if (x == 0.0) {
     out[0] = gauss;
@@ -272,28 +275,33 @@ FO_DEV void radial_store(int L, double x, double gauss, double* out, long long s
   }
   const int ls = (int)ceil(sqrt((double)L * L + 40.0 * x)) + 10;
   const double ix = 1.0 / x;
-  // pass 1: normalisation
   double fp1 = 0.0, f = 1.0;
-  for (int l = ls; l > 0; --l) {
+  for (int l = ls; l > L; --l) {
     const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
     fp1 = f;
     f = fm;
+  }
+  double fv[LB + 1];  // f_l, l <= L (L is uniform over the workgroup)
+#pragma unroll
+  for (int l = LB; l >= 0; --l) {
+    if (l <= L) {
+      fv[l] = f;
+      if (l > 0) {
+        const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
+        fp1 = f;
+        f = fm;
+      }
+    }
   }
   const double i0 = -expm1(-2.0 * x) / (2.0 * x);
   const double scal = i0 / f * gauss;
-  // pass 2: store
-  fp1 = 0.0;
-  f = 1.0;
-  for (int l = ls; l >= 0; --l) {
-    if (l <= L) out[l * stride] = f * scal;
-    if (l == 0) break;
-    const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
-    fp1 = f;
-    f = fm;
-  }
+#pragma unroll
+  for (int l = 0; l <= LB; ++l)
+    if (l <= L) out[l * stride] = fv[l] * scal;
 }
 
-__global__ void k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
+template <int LB>
+__global__ void __launch_bounds__(256) k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
                              const int* __restrict__ gath, const double* __restrict__ X1c,
                              const double* __restrict__ X2c, const double* __restrict__ scale_eff,
                              double* __restrict__ R) {
@@ -318,7 +326,7 @@ __global__ void k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __r
     const double r2 = sqrt(x2[0] * x2[0] + x2[1] * x2[1] + x2[2] * x2[2]);
     const double x = r1 * r2 * i2s2;
     const double gauss = exp(-(r1 - r2) * (r1 - r2) * i4s2);
-    radial_store(d.L, x, gauss, R + d.rad_off + g[q].rad_off + (long long)j * Ns + k,
+    radial_store<LB>(d.L, x, gauss, R + d.rad_off + g[q].rad_off + (long long)j * Ns + k,
                  (long long)Ns * Ns);
   }
 }
@@ -1882,12 +1890,20 @@ void launch_sph_ylm(hipStream_t s, int P, const SphPair* pd, const int* gath, co
   hipLaunchKernelGGL(k_sph_ylm, dim3(P, 2), dim3(256), 0, s, pd, gath, X1c, X2c, leg, cm, Y1, Y2);
 }
 
-void launch_sph_radial(hipStream_t s, int P, long long maxjk, const SphPair* pd,
+void launch_sph_radial(hipStream_t s, int P, int Lmax, long long maxjk, const SphPair* pd,
                        const SphGroup* grp, const int* gath, const double* X1c,
                        const double* X2c, const double* scale_eff, double* R) {
   const int bx = std::max(1, std::min(cdiv(maxjk, 256), 1024));
-  hipLaunchKernelGGL(k_sph_radial, dim3(bx, P), dim3(256), 0, s, pd, grp, gath, X1c, X2c,
-                     scale_eff, R);
+#define FO_RAD(LB)                                                                              \
+  hipLaunchKernelGGL(k_sph_radial<LB>, dim3(bx, P), dim3(256), 0, s, pd, grp, gath, X1c, X2c,   \
+                     scale_eff, R)
+  if (Lmax <= 15)
+    FO_RAD(15);
+  else if (Lmax <= 31)
+    FO_RAD(31);
+  else
+    FO_RAD(63);
+#undef FO_RAD
 }
 
 void launch_sph_contract(hipStream_t s, int P, int n, int Lmax, const SphPair* pd,
