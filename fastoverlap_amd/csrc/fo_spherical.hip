@@ -1573,15 +1573,46 @@ struct FGH {
   double v[10];  // f, fa, fb, fy, faa, fab, fay, fbb, fby, fyy
 };
 
+// `ph`: LDS for 3L + 2 phase factors.
+constexpr int kEvalUnroll = 2;  // degrees per batch of operand loads
 FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a, double b,
                      double y, const double* __restrict__ rec, const double* __restrict__ seedu,
-                     int Lt, FGH& out, double* red) {
+                     int Lt, FGH& out, double* red, double2* ph) {
   const int W = 2 * L + 1, Wt = 2 * Lt + 1;
   const double cb = cos(b), sb = sin(b), c2 = cos(0.5 * b), s2 = sin(0.5 * b);
+  // phase factors e^{-i m1 a} (m1 = 0..L) and e^{-i m2 y} (m2 = -L..L)
+  for (int q = threadIdx.x; q < 3 * L + 2; q += blockDim.x) {
+    double sp, cp;
+    if (q <= L)
+      sincos(-(q * a), &sp, &cp);
+    else
+      sincos(-((q - L - 1 - L) * y), &sp, &cp);
+    ph[q] = make_double2(cp, sp);
+  }
+  __syncthreads();
   double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int e = threadIdx.x; e < (L + 1) * W; e += blockDim.x) {
-    const int m1 = e / W, m2 = e % W - L;
-    const int l0 = max(m1, abs(m2));
+  // entries (m1 >= 0, m2) in shell order: shell l0 = max(m1, |m2|) holds
+  // 4 l0 + 1 entries from e = l0 (2 l0 - 1) on (m1 = l0, m2 = -l0..l0, then
+  // m1 < l0 with m2 = +-l0), so the lanes of a wavefront start their degree
+  // recurrences at nearly the same l and the loop over l is wavefront-uniform
+  const int ne = (L + 1) * W;
+  for (int e0 = 0; e0 < ne; e0 += blockDim.x) {
+    const int e = e0 + threadIdx.x;
+    const bool act = e < ne;
+    int l0 = L + 1, m1 = 0, m2 = 0;
+    if (act) {
+      l0 = (int)((1.0 + sqrt(1.0 + 8.0 * e)) * 0.25);
+      while (l0 * (2 * l0 - 1) > e) --l0;
+      while ((l0 + 1) * (2 * l0 + 1) <= e) ++l0;
+      const int r = e - l0 * (2 * l0 - 1);
+      if (r <= 2 * l0) {
+        m1 = l0;
+        m2 = r - l0;
+      } else {
+        m1 = (r - 2 * l0 - 1) >> 1;
+        m2 = ((r - 2 * l0 - 1) & 1) ? -l0 : l0;
+      }
+    }
     const int mu = abs(m1 - m2), nu = abs(m1 + m2);
     // edge element g = s2^mu c2^nu and its b-derivatives
     const double fct = seedu[(m1 + Lt) * Wt + (m2 + Lt)];
@@ -1599,9 +1630,7 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
     double d0 = fct * g0, d1 = fct * g1, dd2 = fct * g2;
     double p0 = 0, p1 = 0, p2 = 0;
     double sl = (l0 & 1) ? sgn : 1.0;
-    long long id = kidx(l0, m1, m2);
-    for (int l = l0; l <= L; ++l) {
-      const double2 Iv = I[id];
+    auto step = [&](int l, const double2& Iv, double Au, double Bu, double C) {
       // conj(sgn^l I) * d
       const double ix = sl * Iv.x, iy = -sl * Iv.y;
       c0x += d0 * ix;
@@ -1611,8 +1640,7 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
       c2x += dd2 * ix;
       c2y += dd2 * iy;
       sl *= sgn;
-      if (l == L) break;
-      const double Au = rec[3 * id], Bu = rec[3 * id + 1], C = rec[3 * id + 2];
+      if (l == L) return;
       const double bc = Bu * (cb - C);
       const double n0 = bc * d0 - Au * p0;
       const double n1 = bc * d1 - Bu * sb * d0 - Au * p1;
@@ -1623,10 +1651,39 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
       d0 = n0;
       d1 = n1;
       dd2 = n2;
-      id = kidx(l + 1, m1, m2);
+    };
+    // operands (coefficient, recurrence terms to l + 1) of kEvalUnroll
+    // degrees at a time, all loads issued before the steps.  The loaded degree is
+    // max(l, l0) clamped to L (the table covers every l <= Lt; terms of
+    // degree L are not used).
+    // compact index of degree l + 1 from that of l (same m1, m2)
+    auto next_id = [&](int id, int l) {
+      return (l >= l0 && l < L) ? id + (2 * l + 1) * (2 * l + 1) + 2 * m1 + 4 * l + 4 : id;
+    };
+    const int lw = __builtin_amdgcn_readfirstlane(l0);  // smallest l0 of the wavefront
+    int id = act ? (int)kidx(l0, m1, m2) : 0;
+    for (int l = lw; l <= L; l += kEvalUnroll) {  // wavefront-uniform
+      int ids[kEvalUnroll];
+      double2 Iv[kEvalUnroll];
+      double Au[kEvalUnroll], Bu[kEvalUnroll], C[kEvalUnroll];
+      ids[0] = id;
+#pragma unroll
+      for (int q = 1; q < kEvalUnroll; ++q) ids[q] = next_id(ids[q - 1], l + q - 1);
+#pragma unroll
+      for (int q = 0; q < kEvalUnroll; ++q) {
+        Iv[q] = I[ids[q]];
+        Au[q] = rec[3 * ids[q]];
+        Bu[q] = rec[3 * ids[q] + 1];
+        C[q] = rec[3 * ids[q] + 2];
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads in degree order
+      }
+#pragma unroll
+      for (int q = 0; q < kEvalUnroll; ++q)
+        if (l + q <= L && l + q >= l0) step(l + q, Iv[q], Au[q], Bu[q], C[q]);
+      id = next_id(ids[kEvalUnroll - 1], l + kEvalUnroll - 1);
     }
-    double sp, cp;
-    sincos(-(m1 * a + m2 * y), &sp, &cp);
+    const double2 ea = ph[m1], ey = ph[2 * L + 1 + m2];
+    const double cp = ea.x * ey.x - ea.y * ey.y, sp = ea.x * ey.y + ea.y * ey.x;
     const double w = (m1 == 0) ? 1.0 : 2.0;
     // Re(c E) and Re(-i c E) = Im(c E)
     const double r0 = c0x * cp - c0y * sp, i0 = c0x * sp + c0y * cp;
@@ -1678,7 +1735,7 @@ FO_DEV bool chol3_solve(const double H[6], double lam, const double g[3], double
 // reaches the level where rounding noise of f (a sum of ~K(L) terms)
 // dominates: |g| <= 1e-12 |f|, decrease <= 1e-13 |f|, step <= 1e-12 rad.
 // The line search halves at most 8 times.
-__global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ pd,
+__global__ void __launch_bounds__(256, 3) k_sph_rotopt(const SphPair* __restrict__ pd,
                                                     const double2* __restrict__ IA,
                                                     const double2* __restrict__ IB, int mode,
                                                     const SphPeak* __restrict__ peaks, int norient,
@@ -1686,6 +1743,7 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
                                                     const double* __restrict__ seedu, int Lt,
                                                     SphOpt* __restrict__ opt, int P) {
   __shared__ double red[16 * 10];
+  __shared__ double2 ph[3 * 64 + 2];
   int p, o;
   if (!xcd_decode(P, norient, p, o)) return;
   const SphPair d = pd[p];
@@ -1703,7 +1761,7 @@ __global__ void __launch_bounds__(256) k_sph_rotopt(const SphPair* __restrict__ 
   bool have = false;
   for (;;) {
     FGH r;
-    eval_fgh(I, sgn, L, xt[0], xt[1], xt[2], rec, seedu, Lt, r, red);
+    eval_fgh(I, sgn, L, xt[0], xt[1], xt[2], rec, seedu, Lt, r, red, ph);
     ++nev;
     if (have) {
       if (!(r.v[0] <= cur.v[0] + 1e-4 * step * slope)) {
@@ -2041,8 +2099,9 @@ __global__ void __launch_bounds__(256) k_sph_energy(int L, const double2* __rest
                                                     const double* __restrict__ seedu, int Lt,
                                                     double* __restrict__ out) {
   __shared__ double red[16 * 10];
+  __shared__ double2 ph[3 * 64 + 2];
   FGH r;
-  eval_fgh(Iconj, 1.0, L, a, b, y, rec, seedu, Lt, r, red);
+  eval_fgh(Iconj, 1.0, L, a, b, y, rec, seedu, Lt, r, red, ph);
   if (threadIdx.x == 0)
     for (int q = 0; q < 10; ++q) out[q] = r.v[q];
 }
