@@ -38,10 +38,11 @@ void launch_sph_contract(hipStream_t, int, int, int, const SphPair*, const SphGr
 size_t isoft_plane_smem(int, int, int);
 int isoft_plane_use_fft(int, int, size_t);
 void launch_isoft_plane(hipStream_t, int, int, int, int, long long, const SphPair*, const float*,
-                        const float2*, const float2*, const float2*, int, int, PlaneCand*, float*);
+                        const float2*, const float2*, const float2*, int, int, PlaneCand*, float*,
+                        double*);
 void launch_sph_select(hipStream_t, int, int, long long, int, const SphPair*, const double*,
                        const double2*, const double2*, const double2*, int, const PlaneCand*,
-                       double2*, long long, SphPeak*);
+                       double2*, long long, const double*, SphPeak*);
 void launch_sph_rotopt(hipStream_t, int, int, const SphPair*, const double2*, const double2*, int,
                        const SphPeak*, const double*, const double*, int, SphOpt*);
 void launch_rec_table(hipStream_t, int, double*, double*, double2*, double*);
@@ -364,6 +365,7 @@ struct fo_sph_batch {
   double2 *I64A = nullptr, *I64B = nullptr, *scr = nullptr;
   float2 *I32A = nullptr, *I32B = nullptr;
   PlaneCand* cand = nullptr;
+  double* abound = nullptr;  // per (pair, orientation): sum of weighted |C^l| (error bound)
   SphPeak* peaks = nullptr;
   SphOpt* opt = nullptr;
   SphResult* res = nullptr;
@@ -382,7 +384,7 @@ struct fo_sph_batch {
   // layout (byte offsets inside base) and ownership
   struct {
     size_t pd, grp, gath, pos1, pos2, X1c, X2c, X2R, X2out, scale, R, Y1, Y2, I64A, I32A, I64B,
-        I32B, cand, peaks, opt, res, perm, rowsol, scr, grid;
+        I32B, cand, abound, peaks, opt, res, perm, rowsol, scr, grid;
   } off{};
   int gsum = 0;
   bool own_mem = false, own_stream = false;
@@ -425,10 +427,12 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
   mark(4);
   if (stages & FO_STAGE_GRID) {
     launch_isoft_plane(s, P, L.n, L.Lmax, b->fft_chunk, b->tab->Kp, b->pd, b->tab->tab32,
-                       b->tab->tw32, b->I32A, b->I32B, L.mode, L.norient, b->cand, b->grid);
+                       b->tab->tw32, b->I32A, b->I32B, L.mode, L.norient, b->cand, b->grid,
+                       b->abound);
     mark(5);
     launch_sph_select(s, P, L.n, b->tab->Kt, L.norient, b->pd, b->tab->tab64, b->tab->tw64,
-                      b->I64A, b->I64B, L.mode, b->cand, b->scr, b->scr_stride, b->peaks);
+                      b->I64A, b->I64B, L.mode, b->cand, b->scr, b->scr_stride, b->abound,
+                      b->peaks);
   } else {
     mark(5);
   }
@@ -604,6 +608,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   o.I64B = general ? lay.add<double2>(Ktot) : 0;
   o.I32B = general ? lay.add<float2>(C32tot) : 0;
   o.cand = lay.add<PlaneCand>((size_t)P * no * n);
+  o.abound = lay.add<double>((size_t)P * no);
   o.peaks = lay.add<SphPeak>((size_t)P * no);
   o.opt = lay.add<SphOpt>((size_t)P * no);
   o.res = lay.add<SphResult>(P);
@@ -649,6 +654,7 @@ int sph_batch_bind(fo_sph_batch* b, void* mem, hipStream_t stream) {
   b->I64B = general ? (double2*)(B + o.I64B) : nullptr;
   b->I32B = general ? (float2*)(B + o.I32B) : nullptr;
   b->cand = (PlaneCand*)(B + o.cand);
+  b->abound = (double*)(B + o.abound);
   b->peaks = (SphPeak*)(B + o.peaks);
   b->opt = (SphOpt*)(B + o.opt);
   b->res = (SphResult*)(B + o.res);

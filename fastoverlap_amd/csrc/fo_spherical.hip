@@ -862,12 +862,26 @@ constexpr int fft_tpf() {
 // Workgroup w of the 1-D launch serves pair (w % 8) + 8 (w / 8n) and plane
 // (w / 8) % n, so all planes of a pair run on one XCD and share its L2.
 // ---------------------------------------------------------------------------
+// A = sum_l sqrt((2l+1)/2) sum_{m1,m2} |I^l_{m1 m2}| for the selection's
+// error bound, from the per-thread partial sums (FP32 magnitudes of the FP32
+// coefficients, summed in FP64).  The margin 1 + 2^-16 covers the FP32
+// rounding of the coefficients and of the magnitudes, so A stays an upper
+// bound of the FP64 sum.  Alignment mode: one set serves both orientations.
+// Every thread of the workgroup must call it.
+FO_DEV void plane_abound(const double (&asum)[2], bool two_sets, int norient, double* out) {
+  __shared__ double sc[8];
+  double v[2] = {asum[0], asum[1]};
+  block_sum_n<2>(v, sc);
+  if (threadIdx.x == 0)
+    for (int o = 0; o < norient; ++o) out[o] = (two_sets && o == 1 ? v[1] : v[0]) * (1.0 + 1.0 / 65536.0);
+}
+
 template <int N1, int N2, int MODE>
 __global__ void __launch_bounds__(256) k_isoft_fft(
     const SphPair* __restrict__ pd, int P, int C, long long Kp, const float* __restrict__ tab32,
     const float2* __restrict__ twg, const float2* __restrict__ IA,
     const float2* __restrict__ IB, int norient, PlaneCand* __restrict__ cand,
-    float* __restrict__ grid) {
+    float* __restrict__ grid, double* __restrict__ abound) {
   constexpr int n = N1 * N2;
   constexpr int TPF = fft_tpf<N1, N2>();
   static_assert(TPF == N1 && N2 % 4 == 0, "lanes own N2 contiguous plane-layout entries");
@@ -901,6 +915,10 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
   const __amdgpu_buffer_rsrc_t ria = buf_rsrc(IA + d.c32_off, rbytes * 8);
   const __amdgpu_buffer_rsrc_t rib =
       buf_rsrc(MODE != 0 ? (const void*)(IB + d.c32_off) : (const void*)(IA + d.c32_off), rbytes * 8);
+  // the beta = 0 workgroup also sums the coefficient magnitudes for the
+  // selection's error bound (each entry is loaded once per workgroup)
+  const bool wantA = ib == 0;
+  double asum[2] = {0.0, 0.0};
 
   // ---- gamma pass (rows m1 in rounds of NG)
   for (int m1_0 = 0; m1_0 <= L; m1_0 += NG) {
@@ -933,6 +951,11 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
         ca[2 * i] = make_float2(w.x, w.y);
         ca[2 * i + 1] = make_float2(w.z, w.w);
       }
+      // weight sqrt((2l+1)/2), doubled for m1 > 0 (the m1 < 0 rows)
+      const float wl = wantA ? sqrtf(l + 0.5f) * (m1 == 0 ? 1.f : 2.f) : 0.f;
+      if (wantA)
+#pragma unroll
+        for (int j2 = 0; j2 < N2; ++j2) asum[0] += wl * sqrtf(ca[j2].x * ca[j2].x + ca[j2].y * ca[j2].y);
       if constexpr (MODE != 0) {
         float2 cb[N2];
 #pragma unroll
@@ -941,6 +964,9 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
           cb[2 * i] = make_float2(w.x, w.y);
           cb[2 * i + 1] = make_float2(w.z, w.w);
         }
+        if (wantA)
+#pragma unroll
+          for (int j2 = 0; j2 < N2; ++j2) asum[1] += wl * sqrtf(cb[j2].x * cb[j2].x + cb[j2].y * cb[j2].y);
 #pragma unroll
         for (int j2 = 0; j2 < N2; ++j2) {
           sa[j2].x += dv[j2] * ca[j2].x;
@@ -1006,6 +1032,7 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
     }
   }
   __syncthreads();
+  if (wantA) plane_abound(asum, MODE != 0, no, abound + (long long)p * no);
 
   // ---- alpha pass (columns g in rounds of NG)
   CandTrack ct;
@@ -1058,7 +1085,7 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
     const SphPair* __restrict__ pd, int n, long long Kp, const float* __restrict__ tab32,
     const float2* __restrict__ twg, const float2* __restrict__ IA,
     const float2* __restrict__ IB, int mode, int norient, PlaneCand* __restrict__ cand,
-    float* __restrict__ grid) {
+    float* __restrict__ grid, double* __restrict__ abound) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int ib = blockIdx.x, p = blockIdx.y;
   const SphPair d = pd[p];
@@ -1071,17 +1098,22 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
   const float* tb = tab32 + (long long)ib * Kp;
   const float2* ia = IA + d.c32_off;
   const float2* ibb = IB + d.c32_off;
-  // beta contraction
+  // beta contraction (the beta = 0 workgroup also sums the coefficient
+  // magnitudes for the selection's error bound)
+  const bool wantA = ib == 0;
+  double asum[2] = {0.0, 0.0};
   for (int e = t; e < (L + 1) * W; e += bd) {
     const int m1 = e / W, m2 = e % W - L;
     const int l0 = max(m1, abs(m2));
     const int pos = plane_pos(plane_k(m2, n), n);
+    const float wm = m1 == 0 ? 1.f : 2.f;
     float2 s0 = make_float2(0.f, 0.f), s1 = make_float2(0.f, 0.f);
     if (mode == 0) {
       for (int l = l0; l <= L; ++l) {
         const long long id = (long long)qlm(l, m1) * n + pos;
         const float dv = tb[id];
         const float2 c = ia[id];
+        if (wantA) asum[0] += wm * sqrtf(l + 0.5f) * sqrtf(c.x * c.x + c.y * c.y);
         if (l & 1) {
           s1.x += dv * c.x;
           s1.y += dv * c.y;
@@ -1097,6 +1129,10 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
         const long long id = (long long)qlm(l, m1) * n + pos;
         const float dv = tb[id];
         const float2 ca = ia[id], cb = ibb[id];
+        if (wantA) {
+          asum[0] += wm * sqrtf(l + 0.5f) * sqrtf(ca.x * ca.x + ca.y * ca.y);
+          asum[1] += wm * sqrtf(l + 0.5f) * sqrtf(cb.x * cb.x + cb.y * cb.y);
+        }
         s0.x += dv * ca.x;
         s0.y += dv * ca.y;
         s1.x += dv * cb.x;
@@ -1107,6 +1143,7 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
     }
   }
   __syncthreads();
+  if (wantA) plane_abound(asum, mode != 0, norient, abound + (long long)p * norient);
   CandTrack ct;
   auto consider = [&](int o, int a, int g, float val) {
     const int flat = (a * n + ib) * n + g;
@@ -1326,7 +1363,8 @@ __global__ void __launch_bounds__(256) k_sph_select(
     const SphPair* __restrict__ pd, int n, long long Kt, const double* __restrict__ tab64,
     const double2* __restrict__ tw64, const double2* __restrict__ IA,
     const double2* __restrict__ IB, int mode, int norient, const PlaneCand* __restrict__ cand,
-    double2* __restrict__ scratch, long long scratch_stride, SphPeak* __restrict__ peaks, int P) {
+    double2* __restrict__ scratch, long long scratch_stride, const double* __restrict__ abound,
+    SphPeak* __restrict__ peaks, int P) {
   __shared__ double red[16 * kEvalBatch];
   __shared__ int redi[16];
   __shared__ int s_cnt, s_ovf;
@@ -1359,18 +1397,9 @@ __global__ void __launch_bounds__(256) k_sph_select(
     __shared__ int ri[16];
     block_argmax(mv, mi, rv, ri);
   }
-  // error bound; Hermitian sets: |I[-m1,-m2]| = |I[m1,m2]|, so the m1 > 0
-  // rows count twice and the m1 < 0 rows are not read
-  const double2* Iuse = c.IB ? c.IB : c.I;
-  double A = 0.0;
-  for (int l = 0; l <= L; ++l) {
-    const int W = 2 * l + 1;
-    const double f = sqrt((2.0 * l + 1.0) * 0.5);
-    const double2* row0 = Iuse + kidx(l, 0, -l);
-    for (int e = t; e < (l + 1) * W; e += blockDim.x)
-      A += (e < W ? f : 2.0 * f) * hypot(row0[e].x, row0[e].y);
-  }
-  A = block_sum(A, red);
+  // error bound: A = sum_l sqrt((2l+1)/2) sum |I^l| >= every |grid value|,
+  // from the plane kernel (Hermitian sets: |I[-m1,-m2]| = |I[m1,m2]|)
+  const double A = abound[(long long)p * norient + o];
   const double delta = (8.0 * (L + 1) + 32.0) * 5.9604644775390625e-08 * A;
   // |g32 - g64| <= delta everywhere, so the FP64 maximum has an FP32 value
   // of at least (FP32 maximum) - 2 delta
@@ -1421,7 +1450,12 @@ __global__ void __launch_bounds__(256) k_sph_select(
   int besti = 0x7fffffff;
   double2* Sg = scratch + ((long long)p * norient + o) * scratch_stride;
   double2* Tg = Sg + (L + 1) * (2 * L + 1);
-  if (cand_overflow) {
+  // one listed candidate (the common case): it is the argmax, and it is
+  // evaluated in the same sweep as its six neighbours below
+  const bool merged = !cand_overflow && novf == 0 && ncand == 1;
+  if (merged) {
+    besti = s_cand[0];
+  } else if (cand_overflow) {
     // too many near-maximal points to list: evaluate every plane in FP64
     for (int b = 0; b < n; ++b) {
       double v;
@@ -1492,17 +1526,23 @@ __global__ void __launch_bounds__(256) k_sph_select(
     }
   }
   // parabolic offsets on |g| with periodic wrap (fastoverlap/utils.py:319-338);
-  // the six neighbours are evaluated in one sweep
+  // the six neighbours (after the argmax itself when merged) in one sweep
   const int ind[3] = {besti / (n * n), (besti / n) % n, besti % n};
+  const int q0 = merged ? 1 : 0;
   if (t == 0) {
     const int bm = (ind[1] + n - 1) % n, bp = (ind[1] + 1) % n;
     s_beta[0] = ind[1];
     s_beta[1] = bp;
     s_beta[2] = bm;
     s_nb = 3;
+    if (merged) {
+      s_pa[0] = ind[0];
+      s_pg[0] = ind[2];
+      s_pb[0] = 0;
+    }
     for (int ax = 0; ax < 3; ++ax)
       for (int sd = 0; sd < 2; ++sd) {
-        const int q = 2 * ax + sd;  // sd 0: +1, 1: -1
+        const int q = q0 + 2 * ax + sd;  // sd 0: +1, 1: -1
         const int dlt = sd == 0 ? 1 : n - 1;
         s_pa[q] = ax == 0 ? (ind[0] + dlt) % n : ind[0];
         s_pg[q] = ax == 2 ? (ind[2] + dlt) % n : ind[2];
@@ -1510,12 +1550,13 @@ __global__ void __launch_bounds__(256) k_sph_select(
       }
   }
   __syncthreads();
-  grid_eval64_batch(c, 6, s_pa, s_pg, s_pb, s_nb, s_beta, s_val, red);
+  grid_eval64_batch(c, q0 + 6, s_pa, s_pg, s_pb, s_nb, s_beta, s_val, red);
   __syncthreads();
+  if (merged) best = s_val[0];
   double off[3];
   for (int ax = 0; ax < 3; ++ax) {
-    const double y1 = fabs(s_val[2 * ax]);
-    const double y3 = fabs(s_val[2 * ax + 1]);
+    const double y1 = fabs(s_val[q0 + 2 * ax]);
+    const double y3 = fabs(s_val[q0 + 2 * ax + 1]);
     const double y2 = fabs(best);
     off[ax] = (y3 - y1) / (2.0 * (2.0 * y2 - y1 - y3));
   }
@@ -2010,12 +2051,12 @@ int isoft_plane_use_fft(int n, int Lmax, size_t max_lds) {
 void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long long Kp,
                         const SphPair* pd, const float* tab32, const float2* tw32,
                         const float2* IA, const float2* IB, int mode, int norient,
-                        PlaneCand* cand, float* grid) {
+                        PlaneCand* cand, float* grid, double* abound) {
   const size_t sm = isoft_plane_smem(n, Lmax, use_fft);
   int n1, n2;
   if (!use_fft || !fft_shape(n, n1, n2)) {
     hipLaunchKernelGGL(k_isoft_plane, dim3(n, P), dim3(256), sm, s, pd, n, Kp, tab32, tw32, IA,
-                       IB, mode, norient, cand, grid);
+                       IB, mode, norient, cand, grid, abound);
     return;
   }
   // pair chunk per XCD (16 measured best on the 4096-pair n=64 class)
@@ -2024,10 +2065,10 @@ void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long
 #define FO_FFT(N1, N2)                                                                         \
   if (mode == 0)                                                                                \
     hipLaunchKernelGGL((k_isoft_fft<N1, N2, 0>), g, b, sm, s, pd, P, C, Kp, tab32, tw32, IA,    \
-                       IB, norient, cand, grid);                                                \
+                       IB, norient, cand, grid, abound);                                        \
   else                                                                                          \
     hipLaunchKernelGGL((k_isoft_fft<N1, N2, 1>), g, b, sm, s, pd, P, C, Kp, tab32, tw32, IA,    \
-                       IB, norient, cand, grid)
+                       IB, norient, cand, grid, abound)
   switch (n) {
     case 16: FO_FFT(4, 4); break;
     case 32: FO_FFT(8, 4); break;
@@ -2041,9 +2082,9 @@ void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long
 void launch_sph_select(hipStream_t s, int P, int n, long long Kt, int norient, const SphPair* pd,
                        const double* tab64, const double2* tw64, const double2* IA,
                        const double2* IB, int mode, const PlaneCand* cand, double2* scratch,
-                       long long scratch_stride, SphPeak* peaks) {
+                       long long scratch_stride, const double* abound, SphPeak* peaks) {
   hipLaunchKernelGGL(k_sph_select, dim3(xcd_grid(P, norient)), dim3(256), 0, s, pd, n, Kt, tab64,
-                     tw64, IA, IB, mode, norient, cand, scratch, scratch_stride, peaks, P);
+                     tw64, IA, IB, mode, norient, cand, scratch, scratch_stride, abound, peaks, P);
 }
 
 void launch_sph_rotopt(hipStream_t s, int P, int norient, const SphPair* pd, const double2* IA,
