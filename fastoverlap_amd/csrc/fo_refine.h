@@ -35,10 +35,43 @@ FO_DEV LapShared lap_carve(unsigned char* base, int nmax) {
   return s;
 }
 
+// Wavefront argmin of (value, index) pairs, ties to the smaller index:
+// butterflies inside rows of 16 lanes by DPP (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror), then the four row
+// results through readlane.  Every lane must be active.
+template <int CTRL>
+FO_DEV void argmin_dpp_step(double& v, int& i) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
+  const double ov = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  const int oi = __builtin_amdgcn_update_dpp(0, i, CTRL, 0xf, 0xf, true);
+  argmin_merge(v, i, ov, oi);
+}
+FO_DEV double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+FO_DEV void wave_argmin_dpp(double& v, int& i) {
+  argmin_dpp_step<0xB1>(v, i);
+  argmin_dpp_step<0x4E>(v, i);
+  argmin_dpp_step<0x141>(v, i);
+  argmin_dpp_step<0x140>(v, i);
+  double rv = readlane_d(v, 0);
+  int ri = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+  for (int r = 16; r < 64; r += 16) argmin_merge(rv, ri, readlane_d(v, r), __builtin_amdgcn_readlane(i, r));
+  v = rv;
+  i = ri;
+}
+
 // Minimum-cost assignment of an n x n cost matrix given by cost(i, j)
 // (0-based).  rowsol[i] = column assigned to row i.  Ties in the column
 // search are broken toward the smaller column index, so the result does not
-// depend on the launch width.  Every thread of the block must call this.
+// depend on the launch width.  Every thread of the block must call this;
+// a one-wavefront block reduces without LDS or barrier.
 // Returns false (rowsol = identity) if a non-finite cost stops the search.
 template <class CostF>
 FO_DEV bool lap_solve(int n, CostF cost, LapShared s, int* rowsol, double* red_v,
@@ -81,7 +114,10 @@ FO_DEV bool lap_solve(int n, CostF cost, LapShared s, int* rowsol, double* red_v
           }
         }
       }
-      block_argmin(best, bj, red_v, red_i);
+      if (bd == 64)
+        wave_argmin_dpp(best, bj);
+      else
+        block_argmin(best, bj, red_v, red_i);
       if (bj == 0x7fffffff || !(best < kLapInf)) {
         // no finite reduced cost left: the costs are not finite numbers
         for (int k = t; k < n; k += bd) rowsol[k] = k;

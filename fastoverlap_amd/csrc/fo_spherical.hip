@@ -1865,7 +1865,7 @@ __global__ void __launch_bounds__(256, 3) k_sph_rotopt(const SphPair* __restrict
 __global__ void __launch_bounds__(256) k_sph_finish(
     const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp, const int* __restrict__ gath,
     const double* __restrict__ X1c, const double* __restrict__ X2c, const SphOpt* __restrict__ opt,
-    int norient, int lap_nmax, double* __restrict__ X2R, int* __restrict__ perm,
+    int norient, int lap_nmax, int pos_lds, double* __restrict__ X2R, int* __restrict__ perm,
     int* __restrict__ rowsol, double* __restrict__ X2out, SphResult* __restrict__ res) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[16];
@@ -1894,13 +1894,24 @@ __global__ void __launch_bounds__(256) k_sph_finish(
   __threadfence_block();
   __syncthreads();
   LapShared ls = lap_carve(smem, lap_nmax);
+  // the group's positions in gathered order, staged in LDS when they fit
+  double* g1 = (double*)(smem + ((lap_lds_bytes(lap_nmax) + 15) & ~(size_t)15));
+  double* g2 = g1 + 3 * lap_nmax;
   int* rs = rowsol + d.atom_off;
   for (int q = 0; q < d.ngroups; ++q) {
     const SphGroup G = grp[d.group_off + q];
     const int* ga = gath + d.gath_off + G.gath_off;
+    if (pos_lds) {
+      for (int e = t; e < 3 * G.size; e += blockDim.x) {
+        const int i = e / 3, k = e - 3 * i;
+        g1[e] = x1[3 * ga[i] + k];
+        g2[e] = xr[3 * ga[i] + k];
+      }
+      __syncthreads();
+    }
     auto cost = [&](int i, int j) {
-      const double* A = x1 + 3 * ga[i];
-      const double* B = xr + 3 * ga[j];
+      const double* A = pos_lds ? g1 + 3 * i : x1 + 3 * ga[i];
+      const double* B = pos_lds ? g2 + 3 * j : xr + 3 * ga[j];
       const double dx = A[0] - B[0], dy = A[1] - B[1], dz = A[2] - B[2];
       return dx * dx + dy * dy + dz * dz;
     };
@@ -2100,8 +2111,14 @@ void launch_sph_finish(hipStream_t s, int P, int lap_nmax, const SphPair* pd, co
                        SphResult* res) {
   // one wavefront per pair: the assignment search is a chain of small
   // reductions, cheapest without cross-wave barriers
-  hipLaunchKernelGGL(k_sph_finish, dim3(P), dim3(64), lap_lds_bytes(lap_nmax), s, pd, grp, gath,
-                     X1c, X2c, opt, norient, lap_nmax, X2R, perm, rowsol, X2out, res);
+  // positions of the largest group in LDS beside the assignment arrays when
+  // that stays within 64 KiB
+  const size_t lap = (lap_lds_bytes(lap_nmax) + 15) & ~(size_t)15;
+  const size_t pos = 6 * sizeof(double) * (size_t)lap_nmax;
+  const int pos_lds = lap + pos <= 65536 ? 1 : 0;
+  hipLaunchKernelGGL(k_sph_finish, dim3(P), dim3(64), lap + (pos_lds ? pos : 0), s, pd, grp,
+                     gath, X1c, X2c, opt, norient, lap_nmax, pos_lds, X2R, perm, rowsol, X2out,
+                     res);
 }
 
 }  // namespace fo
