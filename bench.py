@@ -108,12 +108,14 @@ def cpu_baseline(pairs, per_core):
     naive einsum coefficients, per-(m1,m2) iSOFT loop, pocketfft, L-BFGS-B,
     exact LAP) on the host cores, one process per core, over the first
     per_core * cores pairs of the workload."""
+    import multiprocessing as mp
     from concurrent.futures import ProcessPoolExecutor
     os.environ["OMP_NUM_THREADS"] = "1"
     cores = min(16, len(os.sched_getaffinity(0)))
     nsample = min(len(pairs), per_core * cores)
     sample = pairs[:nsample]
-    with ProcessPoolExecutor(cores) as ex:
+    # fresh interpreters: the workers must not inherit this process's GPU state
+    with ProcessPoolExecutor(cores, mp_context=mp.get_context("spawn")) as ex:
         list(ex.map(_noop, range(cores)))          # start the workers first
         t0 = time.perf_counter()
         list(ex.map(cpu_oracle_pair, sample, chunksize=1))
@@ -148,12 +150,11 @@ def _serve_stream(ctx, reqs, gaps):
             "failed": len(reqs) - ok, "batches": stats["batches"]}
 
 
-def served(ctx, n_requests, seed=7):
+def served(ctx, reqs, seed=7):
     """The served mix (70% clusters N 13..500 Jmax 15..30, 30% periodic
     N 64..1024) through the continuous-batching server: first all requests
     at once (capacity), then a Poisson stream at 80% of that capacity."""
-    from fastoverlap_amd import workloads as W
-    reqs = W.generate(W.config4, range(n_requests))
+    n_requests = len(reqs)
     cap = _serve_stream(ctx, reqs, np.zeros(n_requests))
     rate = 0.8 * cap["pairs_per_s"]
     rng = np.random.default_rng(seed)
@@ -214,7 +215,11 @@ def main():
 
     t = time.perf_counter()
     pairs = W.generate(W.config2, range(args.pairs), workers=args.gen_workers)
-    log(f"generated {len(pairs)} pairs in {time.perf_counter() - t:.1f}s")
+    # every workload is drawn (with forked workers) before the GPU is touched
+    serve_reqs = ([] if args.no_serve else
+                  W.generate(W.config4, range(args.serve_requests), workers=args.gen_workers))
+    log(f"generated {len(pairs)} pairs and {len(serve_reqs)} served requests in "
+        f"{time.perf_counter() - t:.1f}s")
     ctx = _lib.default_context()
     batches = make_batches(pairs, ctx, args.streams)
     for n, sel, b in batches:
@@ -328,7 +333,7 @@ def main():
         "roofline": roof,
     }
     if not args.no_serve:
-        out["serve"] = served(ctx, args.serve_requests)
+        out["serve"] = served(ctx, serve_reqs)
         log(f"served: {out['serve']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pairs, args.cpu_per_core)
