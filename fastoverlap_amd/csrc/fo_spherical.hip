@@ -1238,6 +1238,29 @@ FO_DEV double grid_eval64(const GridCtx& c, int ia, int ib, int ig, double* red)
 constexpr int kEvalBatch = 8;
 constexpr int kEvalBeta = 4;
 
+// Entry e of the (m1 >= 0, m2) entries of degree <= L in shell order: shell
+// l0 = max(m1, |m2|) holds 4 l0 + 1 entries from e = l0 (2 l0 - 1) on
+// (m1 = l0, m2 = -l0..l0, then m1 < l0 with m2 = +-l0).  The lanes of a
+// wavefront then start their degree recurrences at nearly the same l.
+FO_DEV void shell_entry(int e, int& l0, int& m1, int& m2) {
+  l0 = (int)((1.0 + sqrt(1.0 + 8.0 * e)) * 0.25);
+  while (l0 * (2 * l0 - 1) > e) --l0;
+  while ((l0 + 1) * (2 * l0 + 1) <= e) ++l0;
+  const int r = e - l0 * (2 * l0 - 1);
+  if (r <= 2 * l0) {
+    m1 = l0;
+    m2 = r - l0;
+  } else {
+    m1 = (r - 2 * l0 - 1) >> 1;
+    m2 = ((r - 2 * l0 - 1) & 1) ? -l0 : l0;
+  }
+}
+// compact index of degree l + 1 from that of degree l (same m1, m2) once the
+// entry exists (l >= l0); held at the last existing degree otherwise
+FO_DEV int next_kidx(int id, int l, int l0, int m1, int L) {
+  return (l >= l0 && l < L) ? id + (2 * l + 1) * (2 * l + 1) + 2 * m1 + 4 * l + 4 : id;
+}
+
 FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int* pg,
                               const int* pb, int nbeta, const int* betas, double* out,
                               double* red) {
@@ -1250,26 +1273,45 @@ FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int
   double acc[kEvalBatch];
 #pragma unroll
   for (int q = 0; q < kEvalBatch; ++q) acc[q] = 0.0;
-  for (int e = threadIdx.x; e < (L + 1) * W; e += blockDim.x) {
-    const int m1 = e / W, m2 = e % W - L;
-    const int l0 = max(m1, abs(m2));
+  // entries (m1 >= 0, m2) in shell order (shell_entry), so the degree loop of
+  // a wavefront is uniform and the operands of two degrees are loaded at once
+  const int ne = (L + 1) * W;
+  for (int e0 = 0; e0 < ne; e0 += blockDim.x) {
+    const int e = e0 + threadIdx.x;
+    const bool act = e < ne;
+    int l0 = L + 1, m1 = 0, m2 = 0;
+    if (act) shell_entry(e, l0, m1, m2);
     double sx[kEvalBeta], sy[kEvalBeta];
 #pragma unroll
     for (int b = 0; b < kEvalBeta; ++b) sx[b] = sy[b] = 0.0;
     double sl = (l0 & 1) ? sg : 1.0;
-    int id = kidx(l0, m1, m2);
-    for (int l = l0; l <= L; ++l) {
-      const double2 Iv = I[id];
+    const int lw = __builtin_amdgcn_readfirstlane(l0);  // smallest l0 of the wavefront
+    int id = act ? (int)kidx(l0, m1, m2) : 0;
+    for (int l = lw; l <= L; l += 2) {  // wavefront-uniform
+      int ids[2];
+      ids[0] = id;
+      ids[1] = next_kidx(id, l, l0, m1, L);
+      double2 Iv[2];
+      double dv[2][kEvalBeta];
 #pragma unroll
-      for (int b = 0; b < kEvalBeta; ++b) {
-        if (b < nbeta) {
-          const double dv = tb[b][id] * sl;
-          sx[b] += dv * Iv.x;
-          sy[b] += dv * Iv.y;
+      for (int q = 0; q < 2; ++q) {
+        Iv[q] = I[ids[q]];
+#pragma unroll
+        for (int b = 0; b < kEvalBeta; ++b) dv[q][b] = b < nbeta ? tb[b][ids[q]] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (l + q <= L && l + q >= l0) {
+#pragma unroll
+          for (int b = 0; b < kEvalBeta; ++b) {
+            const double d = dv[q][b] * sl;
+            sx[b] += d * Iv[q].x;
+            sy[b] += d * Iv[q].y;
+          }
+          sl *= sg;
         }
       }
-      sl *= sg;
-      id += (2 * l + 1) * (2 * l + 1) + 2 * m1 + 4 * l + 4;  // kidx(l+1, m1, m2)
+      id = next_kidx(ids[1], l + 1, l0, m1, L);
     }
     const double wm = m1 == 0 ? 1.0 : 2.0;
 #pragma unroll
@@ -1641,19 +1683,7 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
     const int e = e0 + threadIdx.x;
     const bool act = e < ne;
     int l0 = L + 1, m1 = 0, m2 = 0;
-    if (act) {
-      l0 = (int)((1.0 + sqrt(1.0 + 8.0 * e)) * 0.25);
-      while (l0 * (2 * l0 - 1) > e) --l0;
-      while ((l0 + 1) * (2 * l0 + 1) <= e) ++l0;
-      const int r = e - l0 * (2 * l0 - 1);
-      if (r <= 2 * l0) {
-        m1 = l0;
-        m2 = r - l0;
-      } else {
-        m1 = (r - 2 * l0 - 1) >> 1;
-        m2 = ((r - 2 * l0 - 1) & 1) ? -l0 : l0;
-      }
-    }
+    if (act) shell_entry(e, l0, m1, m2);
     const int mu = abs(m1 - m2), nu = abs(m1 + m2);
     // edge element g = s2^mu c2^nu and its b-derivatives
     const double fct = seedu[(m1 + Lt) * Wt + (m2 + Lt)];
@@ -1698,9 +1728,7 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
     // max(l, l0) clamped to L (the table covers every l <= Lt; terms of
     // degree L are not used).
     // compact index of degree l + 1 from that of l (same m1, m2)
-    auto next_id = [&](int id, int l) {
-      return (l >= l0 && l < L) ? id + (2 * l + 1) * (2 * l + 1) + 2 * m1 + 4 * l + 4 : id;
-    };
+    auto next_id = [&](int id, int l) { return next_kidx(id, l, l0, m1, L); };
     const int lw = __builtin_amdgcn_readfirstlane(l0);  // smallest l0 of the wavefront
     int id = act ? (int)kidx(l0, m1, m2) : 0;
     for (int l = lw; l <= L; l += kEvalUnroll) {  // wavefront-uniform
