@@ -20,9 +20,10 @@ struct LapShared {
   int* p;
   int* way;
   int* used;
+  int* ra;  // column assigned to row i by the initial column reduction (0: free)
 };
 FO_HD size_t lap_lds_bytes(int nmax) {
-  return (size_t)(nmax + 1) * (3 * sizeof(double) + 3 * sizeof(int));
+  return (size_t)(nmax + 1) * (3 * sizeof(double) + 4 * sizeof(int));
 }
 FO_DEV LapShared lap_carve(unsigned char* base, int nmax) {
   LapShared s;
@@ -32,6 +33,7 @@ FO_DEV LapShared lap_carve(unsigned char* base, int nmax) {
   s.p = (int*)(s.minv + (nmax + 1));
   s.way = s.p + (nmax + 1);
   s.used = s.way + (nmax + 1);
+  s.ra = s.used + (nmax + 1);
   return s;
 }
 
@@ -77,14 +79,42 @@ template <class CostF>
 FO_DEV bool lap_solve(int n, CostF cost, LapShared s, int* rowsol, double* red_v,
                       int* red_i) {
   const int t = threadIdx.x, bd = blockDim.x;
+  // column reduction (Jonker & Volgenant 1987): v[j] = min_i c(i, j), and
+  // column j goes to its minimising row (ties: smallest row) unless an
+  // earlier column took that row.  The duals (u = 0, v) are feasible and
+  // tight on these pairs, so the search below only runs for the free rows.
   for (int j = t; j <= n; j += bd) {
+    double mn = 0.0;
+    int mi = 0;
+    if (j > 0) {
+      mn = kLapInf;
+      for (int i = 1; i <= n; ++i) {
+        const double c = cost(i - 1, j - 1);
+        if (c < mn) {
+          mn = c;
+          mi = i;
+        }
+      }
+      if (!(mn < kLapInf)) mn = 0.0;  // non-finite column: left to the search
+    }
     s.u[j] = 0.0;
-    s.v[j] = 0.0;
+    s.v[j] = mn;
     s.p[j] = 0;
-    s.way[j] = 0;
+    s.way[j] = mi;  // candidate row, read once below
+    s.ra[j] = 0;
   }
   __syncthreads();
+  if (t == 0)
+    for (int j = 1; j <= n; ++j) {
+      const int i = s.way[j];
+      if (i > 0 && !s.ra[i]) {
+        s.ra[i] = j;
+        s.p[j] = i;
+      }
+    }
+  __syncthreads();
   for (int i = 1; i <= n; ++i) {
+    if (s.ra[i]) continue;  // uniform: every thread reads the same LDS word
     for (int j = t; j <= n; j += bd) {
       s.minv[j] = kLapInf;
       s.used[j] = 0;
