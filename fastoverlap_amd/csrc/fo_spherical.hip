@@ -417,40 +417,43 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
     const int Ns = G.size, go = G.gath_off;
     const double* Rl = R + d.rad_off + G.rad_off + (long long)l * Ns * Ns;
     // staging roles of this thread:
-    //   Y1 chunk:  j = j0 + (t & 15), complex rows (t >> 4) + 16 i, i < 2
+    //   Y1 chunk:  j = j0 + t % kJC, complex rows t / kJC + kYR i, i < kNY1
     //              (each gives the real rows 2r and 2r + 1)
-    //   K chunk:   k = k0 + (t & 31), j = j0 + (t >> 5) + 8 i, i < 2
+    //   K chunk:   k = k0 + (t & 31), j = j0 + (t >> 5) + 8 i, i < kNK
     //   Y2 tile:   k = k0 + (t & 31), m = (t >> 5) + 8 i, i < NY
-    const int yj = t & 15, yr0 = t >> 4;
+    constexpr int kYR = 256 / kJC;          // complex rows per Y1 staging round
+    constexpr int kNY1 = kMB * kJC / 256;   // Y1 entries staged per thread
+    constexpr int kNK = kJC / 8;            // K entries staged per thread
+    const int yj = t % kJC, yr0 = t / kJC;
     const int kk_t = t & 31, kj0 = t >> 5;
     const double2* Y1g = Y1l + go;
     const double2* Y2g = Y2l + go;
     for (int k0 = 0; k0 < Ns; k0 += kCT) {
       const int kt = k0 + kk_t;
       const bool kok = kt < Ns;
-      double2 sy[2];
-      double sk[2];
+      double2 sy[kNY1];
+      double sk[kNK];
       double2 yv[NY];
 #define FO_LOAD_CHUNK(J0)                                                           \
   {                                                                                 \
     const int j_ = (J0) + yj;                                                       \
-    _Pragma("unroll") for (int i = 0; i < 2; ++i) {                                 \
-      const int r = yr0 + 16 * i;                                                   \
+    _Pragma("unroll") for (int i = 0; i < kNY1; ++i) {                              \
+      const int r = yr0 + kYR * i;                                                  \
       sy[i] = (r < rows && j_ < Ns) ? Y1g[qlm(l, m1a + r) * Ng + j_]                \
                                     : make_double2(0.0, 0.0);                       \
     }                                                                               \
-    _Pragma("unroll") for (int i = 0; i < 2; ++i) {                                 \
+    _Pragma("unroll") for (int i = 0; i < kNK; ++i) {                               \
       const int j = (J0) + kj0 + 8 * i;                                             \
       sk[i] = (j < Ns && kok) ? Rl[j * Ns + kt] : 0.0;                              \
     }                                                                               \
   }
 #define FO_STORE_CHUNK(BUF)                                                         \
   {                                                                                 \
-    _Pragma("unroll") for (int i = 0; i < 2; ++i) {                                 \
-      S.u.a.y1[BUF][2 * (yr0 + 16 * i)][yj] = sy[i].x;                              \
-      S.u.a.y1[BUF][2 * (yr0 + 16 * i) + 1][yj] = sy[i].y;                          \
+    _Pragma("unroll") for (int i = 0; i < kNY1; ++i) {                              \
+      S.u.a.y1[BUF][2 * (yr0 + kYR * i)][yj] = sy[i].x;                             \
+      S.u.a.y1[BUF][2 * (yr0 + kYR * i) + 1][yj] = sy[i].y;                         \
     }                                                                               \
-    _Pragma("unroll") for (int i = 0; i < 2; ++i) S.u.a.kc[BUF][kj0 + 8 * i][kk_t] = sk[i]; \
+    _Pragma("unroll") for (int i = 0; i < kNK; ++i) S.u.a.kc[BUF][kj0 + 8 * i][kk_t] = sk[i]; \
   }
 // Y2 row m2 = m - l: Y_l^{m2} = P2 E2 for m2 >= 0, (-1)^|m2| conj(P2 E2) for m2 < 0
 #define FO_LOAD_Y2                                                                  \
@@ -658,10 +661,23 @@ FO_DEV void dft_reg(float2 (&v)[R]) {
 #pragma unroll
       for (int k = 0; k < len / 2; ++k) {
         const int ti = k * (16 / len);
-        // e^{+2 pi i ti/16}: cos = C16[ti], sin = C16[(ti + 12) % 16]
-        const float2 w = make_float2(C16[ti], C16[(ti + 12) & 15]);
+        // a e^{+2 pi i ti/16} (cos = C16[ti], sin = C16[(ti + 12) % 16]);
+        // the factors 1, i and the eighth turns are written out, as the
+        // compiler may not drop products with 0 (x * 0 can be -0 or NaN)
+        const float2 a = v[i + k + len / 2];
+        constexpr float c8 = 0.70710678118654752f;
+        float2 tt;
+        if (ti == 0)
+          tt = a;
+        else if (ti == 4)
+          tt = make_float2(-a.y, a.x);
+        else if (ti == 2)
+          tt = make_float2(c8 * (a.x - a.y), c8 * (a.x + a.y));
+        else if (ti == 6)
+          tt = make_float2(-c8 * (a.x + a.y), c8 * (a.x - a.y));
+        else
+          tt = cmul(a, make_float2(C16[ti], C16[(ti + 12) & 15]));
         const float2 u = v[i + k];
-        const float2 tt = cmul(v[i + k + len / 2], w);
         v[i + k] = make_float2(u.x + tt.x, u.y + tt.y);
         v[i + k + len / 2] = make_float2(u.x - tt.x, u.y - tt.y);
       }
@@ -1023,7 +1039,7 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
       }
       dft_reg<N2>(v);
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) v[k2] = cmul(v[k2], tw[k2]);
+      for (int k2 = 1; k2 < N2; ++k2) v[k2] = cmul(v[k2], tw[k2]);  // tw[0] = 1
       two_pass_exchange<N1, N2>(v, u, lane, xr);
       dft_reg<N1>(u);
       if (row && lane < N2)
@@ -1059,7 +1075,7 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
     }
     dft_reg<N2>(v);
 #pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) v[k2] = cmul(v[k2], tw[k2]);
+    for (int k2 = 1; k2 < N2; ++k2) v[k2] = cmul(v[k2], tw[k2]);  // tw[0] = 1
     two_pass_exchange<N1, N2>(v, u, lane, xr);
     dft_reg<N1>(u);
     if (col && lane < N2) {
