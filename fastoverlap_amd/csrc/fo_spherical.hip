@@ -580,19 +580,21 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
   }
 }
 
-// NCTMAX: column tiles of the largest degree in the launch (2l+1 <= 16 NCTMAX).
+// NCTMAX: column tiles of the largest degree in the launch (2l+1 <= 16 NCTMAX);
+// the launch covers the degrees l_lo..l_hi of every pair.
 template <int NCTMAX>
 __global__ void __launch_bounds__(256, 3) k_sph_contract_mfma(
     const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
     const double2* __restrict__ Y1, const double2* __restrict__ Y2,
     const double* __restrict__ R, const double* __restrict__ scale_eff,
-    double2* __restrict__ I64, float2* __restrict__ I32, int n, int P, int nbl) {
+    double2* __restrict__ I64, float2* __restrict__ I32, int n, int P, int nbl, int l_lo,
+    int l_hi) {
   __shared__ ContractLds<NCTMAX> S;
   int pp, b;
   if (!xcd_decode(P, nbl, pp, b)) return;
   const SphPair d = pd[pp];
-  const int L = d.L;
-  int l = 0;
+  const int L = min(d.L, l_hi);
+  int l = l_lo;
   for (; l <= L; ++l) {
     const int nb = (l + 1 + kMB - 1) / kMB;
     if (b < nb) break;
@@ -2063,19 +2065,27 @@ void launch_sph_radial(hipStream_t s, int P, int Lmax, long long maxjk, const Sp
 void launch_sph_contract(hipStream_t s, int P, int n, int Lmax, const SphPair* pd,
                          const SphGroup* grp, const double2* Y1, const double2* Y2,
                          const double* R, const double* scale_eff, double2* I64, float2* I32) {
-  int nb = 0;
-  for (int l = 0; l <= Lmax; ++l) nb += (l + 1 + kMB - 1) / kMB;
-  const dim3 g(xcd_grid(P, nb)), b(256);
-#define FO_CL(K)                                                                                \
-  hipLaunchKernelGGL(k_sph_contract_mfma<K>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,  \
-                     I32, n, P, nb)
-  if (Lmax <= 15)
-    FO_CL(2);
-  else if (Lmax <= 31)
-    FO_CL(4);
-  else
-    FO_CL(8);
-#undef FO_CL
+  // one launch per column-tile class, so the low degrees run with the
+  // smaller LDS tile (four workgroups per CU instead of three)
+  auto launch = [&](int K, int lo, int hi) {
+    hi = std::min(hi, Lmax);
+    if (lo > hi) return;
+    int nb = 0;
+    for (int l = lo; l <= hi; ++l) nb += (l + 1 + kMB - 1) / kMB;
+    const dim3 g(xcd_grid(P, nb)), b(256);
+    if (K == 2)
+      hipLaunchKernelGGL(k_sph_contract_mfma<2>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,
+                         I32, n, P, nb, lo, hi);
+    else if (K == 4)
+      hipLaunchKernelGGL(k_sph_contract_mfma<4>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,
+                         I32, n, P, nb, lo, hi);
+    else
+      hipLaunchKernelGGL(k_sph_contract_mfma<8>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,
+                         I32, n, P, nb, lo, hi);
+  };
+  launch(2, 0, 15);
+  launch(4, 16, 31);
+  launch(8, 32, 63);
 }
 
 // two-pass factorisation n = N1 N2 of the power-of-two grids (0: direct DFT)
