@@ -455,19 +455,23 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
     }                                                                               \
     _Pragma("unroll") for (int i = 0; i < kNK; ++i) S.u.a.kc[BUF][kj0 + 8 * i][kk_t] = sk[i]; \
   }
-// Y2 row m2 = m - l: Y_l^{m2} = P2 E2 for m2 >= 0, (-1)^|m2| conj(P2 E2) for m2 < 0
+// Y2 row m2 = m - l: Y_l^{m2} = P2 E2 for m2 >= 0, (-1)^|m2| conj(P2 E2) for m2 < 0.
+// Every lane loads (entries outside the tile read element 0 of the group's
+// block and are zeroed afterwards), so the loads issue back to back.
 #define FO_LOAD_Y2                                                                  \
   {                                                                                 \
     _Pragma("unroll") for (int i = 0; i < NY; ++i) {                                \
       const int m = kj0 + 8 * i;                                                    \
-      double2 y = make_double2(0.0, 0.0);                                           \
-      if (m < M && kok) {                                                           \
-        const int m2 = m - l, am = abs(m2);                                         \
-        const double2 yp = Y2g[qlm(l, am) * Ng + kt];                               \
-        const double sgn = (m2 < 0 && (am & 1)) ? -1.0 : 1.0;                       \
-        y = make_double2(sgn * yp.x, (m2 < 0 ? -sgn : 1.0) * yp.y);                 \
-      }                                                                             \
-      yv[i] = y;                                                                    \
+      const bool ok = m < M && kok;                                                 \
+      const int am = abs(m - l);                                                    \
+      yv[i] = Y2g[ok ? qlm(l, am) * Ng + kt : 0];                                   \
+    }                                                                               \
+    _Pragma("unroll") for (int i = 0; i < NY; ++i) {                                \
+      const int m = kj0 + 8 * i, m2 = m - l;                                        \
+      const bool ok = m < M && kok;                                                 \
+      const double sgn = (m2 < 0 && (m2 & 1)) ? -1.0 : 1.0;                         \
+      yv[i] = ok ? make_double2(sgn * yv[i].x, (m2 < 0 ? -sgn : 1.0) * yv[i].y)     \
+                 : make_double2(0.0, 0.0);                                          \
     }                                                                               \
   }
       FO_LOAD_CHUNK(0);
@@ -479,14 +483,33 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
       for (int c = 0; c < nch; ++c) {
         const int buf = c & 1;
         if (c + 1 < nch) FO_LOAD_CHUNK((c + 1) * kJC);
-        // A tiles of this wave: rows 16 rt, columns 16 actA
+        // A tiles of this wave: rows 16 rt, columns 16 actA.  The row-tile
+        // conditions are wave-uniform and outside the k loop, so every LDS
+        // operand of the chunk is read before the first MFMA waits on one.
+        {
+          const int rt0 = w >> 1, rt1 = rt0 + 2;
+          double bv[kJC / 4], ya[kJC / 4], yb[kJC / 4];
+          if (rt0 < nrtA) {
 #pragma unroll
-        for (int s4 = 0; s4 < kJC / 4; ++s4) {
-          const int jj = 4 * s4 + kq;
-          const double bv = S.u.a.kc[buf][jj][16 * actA + col];
-          const int rt0 = w >> 1, rt1 = (w >> 1) + 2;
-          if (rt0 < nrtA) a0 = mfma64(S.u.a.y1[buf][16 * rt0 + col][jj], bv, a0);
-          if (rt1 < nrtA) a1 = mfma64(S.u.a.y1[buf][16 * rt1 + col][jj], bv, a1);
+            for (int s4 = 0; s4 < kJC / 4; ++s4) {
+              const int jj = 4 * s4 + kq;
+              bv[s4] = S.u.a.kc[buf][jj][16 * actA + col];
+              ya[s4] = S.u.a.y1[buf][16 * rt0 + col][jj];
+            }
+            if (rt1 < nrtA) {
+#pragma unroll
+              for (int s4 = 0; s4 < kJC / 4; ++s4)
+                yb[s4] = S.u.a.y1[buf][16 * rt1 + col][4 * s4 + kq];
+#pragma unroll
+              for (int s4 = 0; s4 < kJC / 4; ++s4) {
+                a0 = mfma64(ya[s4], bv[s4], a0);
+                a1 = mfma64(yb[s4], bv[s4], a1);
+              }
+            } else {
+#pragma unroll
+              for (int s4 = 0; s4 < kJC / 4; ++s4) a0 = mfma64(ya[s4], bv[s4], a0);
+            }
+          }
         }
         if (c + 1 < nch) FO_STORE_CHUNK(buf ^ 1);
         __syncthreads();
@@ -512,22 +535,32 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
       for (int i = 0; i < NY; ++i) S.u.b.ys[kk_t][kj0 + 8 * i] = yv[i];
       __syncthreads();
       // second product, k-steps outer so consecutive MFMAs update
-      // different accumulators
+      // different accumulators.  Every k-step reads the operands of all NCT
+      // tile slots first (row clamped into the block, so the reads stay in
+      // the A tile), then issues the MFMAs of this wave's tiles, ctile =
+      // (w >> 1) + 2 qb below nrt * nct (wave-uniform).
+      {
+        const int ntl = nrt * nct;
 #pragma unroll 2
-      for (int s8 = 0; s8 < kCT / 4; ++s8) {
-        const int kk = 4 * s8 + kq;
+        for (int s8 = 0; s8 < kCT / 4; ++s8) {
+          const int kk = 4 * s8 + kq;
+          double2 av[NCT], yw[NCT];
 #pragma unroll
-        for (int qb = 0; qb < NCT; ++qb) {
-          const int ctile = (w >> 1) + 2 * qb;
-          const int rt = ctile / nct, ct = ctile % nct;
-          if (rt >= nrt) continue;  // wave-uniform
-          const int r = 16 * rt + col;
-          const double2 a = r < rows ? S.u.b.ac[r][kk] : make_double2(0.0, 0.0);
-          const double2 y = S.u.b.ys[kk][16 * ct + col];
-          // real part: Ar Y2r + Ai Y2i; imaginary part: Ai Y2r - Ar Y2i
-          const double a0 = part ? a.y : a.x, a1 = part ? -a.x : a.y;
-          cacc[qb] = mfma64(a0, y.x, cacc[qb]);
-          cacc[qb] = mfma64(a1, y.y, cacc[qb]);
+          for (int qb = 0; qb < NCT; ++qb) {
+            const int ctile = (w >> 1) + 2 * qb;
+            const int rt = min(ctile / nct, nrt - 1), ct = ctile % nct;
+            const int r = 16 * rt + col;
+            av[qb] = r < rows ? S.u.b.ac[r][kk] : make_double2(0.0, 0.0);
+            yw[qb] = S.u.b.ys[kk][16 * ct + col];
+          }
+#pragma unroll
+          for (int qb = 0; qb < NCT; ++qb) {
+            if ((w >> 1) + 2 * qb >= ntl) continue;  // wave-uniform
+            // real part: Ar Y2r + Ai Y2i; imaginary part: Ai Y2r - Ar Y2i
+            const double a0 = part ? av[qb].y : av[qb].x, a1 = part ? -av[qb].x : av[qb].y;
+            cacc[qb] = mfma64(a0, yw[qb].x, cacc[qb]);
+            cacc[qb] = mfma64(a1, yw[qb].y, cacc[qb]);
+          }
         }
       }
     }
@@ -583,7 +616,7 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
 // NCTMAX: column tiles of the largest degree in the launch (2l+1 <= 16 NCTMAX);
 // the launch covers the degrees l_lo..l_hi of every pair.
 template <int NCTMAX>
-__global__ void __launch_bounds__(256, 3) k_sph_contract_mfma(
+__global__ void __launch_bounds__(256, NCTMAX <= 2 ? 4 : 3) k_sph_contract_mfma(
     const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
     const double2* __restrict__ Y1, const double2* __restrict__ Y2,
     const double* __restrict__ R, const double* __restrict__ scale_eff,
