@@ -927,6 +927,33 @@ FO_DEV void plane_abound(const double (&asum)[2], bool two_sets, int norient, do
     for (int o = 0; o < norient; ++o) out[o] = (two_sets && o == 1 ? v[1] : v[0]) * (1.0 + 1.0 / 65536.0);
 }
 
+// Degree segments of the plane kernel's beta contraction.  Lane k1 owns
+// entries k = k1 + N1 k2 (k2 < N2); entry k2 of every lane is zero for all
+// degrees below the smallest |m2| of its N1 values.  Segment s covers the
+// degrees s N1 .. (s + 1) N1 - 1 (the last one everything above), and its
+// mask lists the k2 that are nonzero in some lane at some degree of it.
+template <int N1, int N2>
+FO_HD constexpr int k2_min_abs_m2(int k2) {
+  constexpr int n = N1 * N2;
+  const int lo = N1 * k2, hi = lo + N1 - 1;
+  return lo < n / 2 ? lo : n - hi;
+}
+template <int N1, int N2>
+FO_HD constexpr unsigned seg_mask(int s) {
+  if (s >= N2 / 2 - 1) return (1u << N2) - 1u;
+  unsigned m = 0;
+  for (int k2 = 0; k2 < N2; ++k2)
+    if (k2_min_abs_m2<N1, N2>(k2) <= (s + 1) * N1 - 1) m |= 1u << k2;
+  return m;
+}
+template <int S, int NS, class F>
+FO_DEV void static_for(F&& f) {
+  if constexpr (S < NS) {
+    f(std::integral_constant<int, S>{});
+    static_for<S + 1, NS>(f);
+  }
+}
+
 template <int N1, int N2, int MODE>
 __global__ void __launch_bounds__(256) k_isoft_fft(
     const SphPair* __restrict__ pd, int P, int C, long long Kp, const float* __restrict__ tab32,
@@ -983,13 +1010,17 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
 #pragma unroll
     for (int j2 = 0; j2 < N2; ++j2) a[j2] = bq[j2] = make_float2(0.f, 0.f);
     const int vo = (m1 * n + lane * N2) * 4;  // lane's entries in a degree block, table bytes
-    // one degree: s += d^l * I^l over the lane's N2 entries
-    auto step = [&](int l, float2(&sa)[N2], float2(&sb)[N2]) {
+    // one degree: s += d^l * I^l over the lane's N2 entries, of which only
+    // those in MASK can be nonzero (the others are skipped: table and
+    // coefficients both hold exact zeros there)
+    auto step = [&](auto MC, int l, float2(&sa)[N2], float2(&sb)[N2]) {
+      constexpr unsigned MASK = decltype(MC)::value;
       const int so = qlm(l, 0) * n * 4;  // degree block, uniform
       float dv[N2];
       float2 ca[N2];
 #pragma unroll
       for (int i = 0; i < N2 / 4; ++i) {
+        if (!((MASK >> (4 * i)) & 15u)) continue;
         const float4 w = load_f4(rtb, vo + 16 * i, so);
         dv[4 * i] = w.x;
         dv[4 * i + 1] = w.y;
@@ -998,6 +1029,7 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
       }
 #pragma unroll
       for (int i = 0; i < N2 / 2; ++i) {
+        if (!((MASK >> (2 * i)) & 3u)) continue;
         const float4 w = load_f4(ria, 2 * vo + 16 * i, 2 * so);
         ca[2 * i] = make_float2(w.x, w.y);
         ca[2 * i + 1] = make_float2(w.z, w.w);
@@ -1006,20 +1038,24 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
       const float wl = wantA ? sqrtf(l + 0.5f) * (m1 == 0 ? 1.f : 2.f) : 0.f;
       if (wantA)
 #pragma unroll
-        for (int j2 = 0; j2 < N2; ++j2) asum[0] += wl * sqrtf(ca[j2].x * ca[j2].x + ca[j2].y * ca[j2].y);
+        for (int j2 = 0; j2 < N2; ++j2)
+          if ((MASK >> j2) & 1u) asum[0] += wl * sqrtf(ca[j2].x * ca[j2].x + ca[j2].y * ca[j2].y);
       if constexpr (MODE != 0) {
         float2 cb[N2];
 #pragma unroll
         for (int i = 0; i < N2 / 2; ++i) {
+          if (!((MASK >> (2 * i)) & 3u)) continue;
           const float4 w = load_f4(rib, 2 * vo + 16 * i, 2 * so);
           cb[2 * i] = make_float2(w.x, w.y);
           cb[2 * i + 1] = make_float2(w.z, w.w);
         }
         if (wantA)
 #pragma unroll
-          for (int j2 = 0; j2 < N2; ++j2) asum[1] += wl * sqrtf(cb[j2].x * cb[j2].x + cb[j2].y * cb[j2].y);
+          for (int j2 = 0; j2 < N2; ++j2)
+            if ((MASK >> j2) & 1u) asum[1] += wl * sqrtf(cb[j2].x * cb[j2].x + cb[j2].y * cb[j2].y);
 #pragma unroll
         for (int j2 = 0; j2 < N2; ++j2) {
+          if (!((MASK >> j2) & 1u)) continue;
           sa[j2].x += dv[j2] * ca[j2].x;
           sa[j2].y += dv[j2] * ca[j2].y;
           sb[j2].x += dv[j2] * cb[j2].x;
@@ -1028,6 +1064,7 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
       } else {
 #pragma unroll
         for (int j2 = 0; j2 < N2; ++j2) {
+          if (!((MASK >> j2) & 1u)) continue;
           sa[j2].x += dv[j2] * ca[j2].x;
           sa[j2].y += dv[j2] * ca[j2].y;
         }
@@ -1036,23 +1073,46 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
     // the wavefront's rows start at degree m1w .. m1w + GPW - 1: row m1
     // takes part from l = m1 on; from l = m1w + GPW every row does.  Rows
     // beyond L load nothing, so every access stays inside rows(L) * n.
+    // Degrees run in segments of N1 with their masks; m1w is a multiple of
+    // GPW, so when GPW <= N1 the first GPW degrees lie in one segment and
+    // the degree pairs (l, l + 1) of the alignment mode never straddle one.
+    constexpr int NSEG = N2 / 2;
     if (row) {
       const int lp = min(m1w + GPW, L + 1);
       int l = m1w;
-      if constexpr (MODE == 0) {
-        for (; l < lp; l += 2) {
-          if (m1 <= l) step(l, a, a);
-          if (l + 1 <= L && m1 <= l + 1) step(l + 1, bq, bq);
+      // rows joining: the segment of m1w, or every entry if GPW > N1
+      auto head = [&](auto MC) {
+        if constexpr (MODE == 0) {
+          for (; l < lp; l += 2) {
+            if (m1 <= l) step(MC, l, a, a);
+            if (l + 1 <= L && m1 <= l + 1) step(MC, l + 1, bq, bq);
+          }
+        } else {
+          for (; l < lp; ++l)
+            if (m1 <= l) step(MC, l, a, bq);
         }
-        for (; l <= L; l += 2) {
-          step(l, a, a);
-          if (l + 1 <= L) step(l + 1, bq, bq);
-        }
+      };
+      if constexpr (GPW <= N1) {
+        const int s0 = min(m1w / N1, NSEG - 1);  // wavefront-uniform
+        static_for<0, NSEG>([&](auto S) {
+          if (S.value == s0) head(std::integral_constant<unsigned, seg_mask<N1, N2>(S.value)>{});
+        });
       } else {
-        for (; l < lp; ++l)
-          if (m1 <= l) step(l, a, bq);
-        for (; l <= L; ++l) step(l, a, bq);
+        head(std::integral_constant<unsigned, (1u << N2) - 1u>{});
       }
+      // every row: segment by segment
+      static_for<0, NSEG>([&](auto S) {
+        constexpr unsigned M = seg_mask<N1, N2>(S.value);
+        const int send = S.value == NSEG - 1 ? L : min(L, (S.value + 1) * N1 - 1);
+        if constexpr (MODE == 0) {
+          for (; l <= send; l += 2) {
+            step(std::integral_constant<unsigned, M>{}, l, a, a);
+            if (l + 1 <= L) step(std::integral_constant<unsigned, M>{}, l + 1, bq, bq);
+          }
+        } else {
+          for (; l <= send; ++l) step(std::integral_constant<unsigned, M>{}, l, a, bq);
+        }
+      });
     }
     // alignment mode: e = even-l sums, od = odd-l sums
     float2 e[N2], od[N2];
