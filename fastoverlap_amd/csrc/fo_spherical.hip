@@ -273,19 +273,32 @@ FO_DEV void radial_store(int L, double x, double gauss, double* out, long long s
     }
     return;
   }
+  // i_0(x) e^{-x} times the Gaussian fixes the normalisation of Miller's
+  // downward recurrence.  Pass 1 runs the recurrence to l = 0 without
+  // storing; pass 2 repeats the identical arithmetic and stores f_l * scal
+  // as it goes (no array of L + 1 values held in registers, so more
+  // wavefronts per SIMD overlap their stores).
+  const double i0 = -expm1(-2.0 * x) / (2.0 * x);
   const int ls = (int)ceil(sqrt((double)L * L + 40.0 * x)) + 10;
   const double ix = 1.0 / x;
   double fp1 = 0.0, f = 1.0;
+  for (int l = ls; l > 0; --l) {
+    const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
+    fp1 = f;
+    f = fm;
+  }
+  const double scal = i0 / f * gauss;
+  fp1 = 0.0;
+  f = 1.0;
   for (int l = ls; l > L; --l) {
     const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
     fp1 = f;
     f = fm;
   }
-  double fv[LB + 1];  // f_l, l <= L (L is uniform over the workgroup)
 #pragma unroll
   for (int l = LB; l >= 0; --l) {
     if (l <= L) {
-      fv[l] = f;
+      out[l * stride] = f * scal;
       if (l > 0) {
         const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
         fp1 = f;
@@ -293,15 +306,9 @@ FO_DEV void radial_store(int L, double x, double gauss, double* out, long long s
       }
     }
   }
-  const double i0 = -expm1(-2.0 * x) / (2.0 * x);
-  const double scal = i0 / f * gauss;
-#pragma unroll
-  for (int l = 0; l <= LB; ++l)
-    if (l <= L) out[l * stride] = fv[l] * scal;
 }
-
 template <int LB>
-__global__ void __launch_bounds__(256) k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
+__global__ void __launch_bounds__(256, 5) k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
                              const int* __restrict__ gath, const double* __restrict__ X1c,
                              const double* __restrict__ X2c, const double* __restrict__ scale_eff,
                              double* __restrict__ R) {
