@@ -50,6 +50,7 @@ void launch_sph_finish(hipStream_t, int, int, const SphPair*, const SphGroup*, c
                        const double*, const double*, const SphOpt*, int, double*, int*, int*,
                        double*, SphResult*);
 void launch_coef_to_plane(hipStream_t, int, int, int, const SphPair*, const double2*, float2*);
+void launch_sph_mirror(hipStream_t, int, int, const SphPair*, double2*);
 void launch_sph_energy(hipStream_t, int, const double2*, double, double, double, const double*,
                        const double*, int, double*);
 void launch_wigner_D(hipStream_t, int, double, double, double, double2*, double2*);
@@ -349,7 +350,9 @@ struct fo_sph_batch {
   int flags = 0;
   int maxgroup = 1;
   int fft_chunk = 0;  // 1: two-pass register transform in the plane kernel, 0: direct DFT
-  long long Ntot = 0, Ktot = 0;
+  long long Ntot = 0, Ktot = 0, C32tot = 0;
+  // the last enqueued coefficient stage left the rows m1 < 0 of I64A unset
+  bool mirror_pending = false;
   std::vector<SphPair> hp;
   std::vector<SphGroup> hg;
   std::vector<int> hgath;
@@ -585,6 +588,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   lc.center = (d->flags & FO_SPH_NO_CENTER) ? 0 : 1;
   b->Ntot = Ntot;
   b->Ktot = Ktot;
+  b->C32tot = C32tot;
   const int no = lc.norient;
   b->scr_stride = (long long)(Lmax + 1) * (2 * Lmax + 1) + (long long)(Lmax + 1) * n;
   b->gsum = gsum;
@@ -672,6 +676,10 @@ int sph_batch_bind(fo_sph_batch* b, void* mem, hipStream_t stream) {
   }
   FO_HIP(hipMemcpyAsync(b->pd, b->hp.data(), sizeof(SphPair) * b->lc.P, hipMemcpyHostToDevice,
                         b->stream));
+  // plane-layout entries |m2| > l are never written by the kernels: zero once
+  FO_HIP(hipMemsetAsync(b->I32A, 0, sizeof(float2) * (size_t)b->C32tot, b->stream));
+  if (general)
+    FO_HIP(hipMemsetAsync(b->I32B, 0, sizeof(float2) * (size_t)b->C32tot, b->stream));
   if (b->gsum)
     FO_HIP(hipMemcpyAsync(b->grp, b->hg.data(), sizeof(SphGroup) * b->gsum, hipMemcpyHostToDevice,
                           b->stream));
@@ -742,6 +750,7 @@ int fo_sph_batch_set_positions_device(fo_sph_batch* b, const void* p1, const voi
 int fo_sph_batch_set_coeffs(fo_sph_batch* b, const double* A, const double* Bc) {
   if (!b || !A) return set_err(FO_ERR_INVALID, "NULL argument");
   FO_HIP(hipMemcpyAsync(b->I64A, A, sizeof(double2) * b->Ktot, hipMemcpyHostToDevice, b->stream));
+  b->mirror_pending = false;
   launch_coef_to_plane(b->stream, b->lc.P, b->lc.n, b->lc.Lmax, b->pd, b->I64A, b->I32A);
   if (b->lc.mode == 1) {
     if (!Bc) return set_err(FO_ERR_INVALID, "general mode needs the second coefficient set");
@@ -787,6 +796,7 @@ int fo_sph_batch_set_rotation(fo_sph_batch* b, const double* R) {
 int fo_sph_batch_run(fo_sph_batch* b, int stages, int use_graph) {
   if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
   FO_HIP(hipSetDevice(b->ctx->device));
+  if (stages & FO_STAGE_COEFF) b->mirror_pending = true;
   if (use_graph && !b->timing) {
     auto it = b->graphs.find(stages);
     if (it == b->graphs.end()) {
@@ -883,6 +893,11 @@ int fo_sph_batch_scales(fo_sph_batch* b, double* scale) {
 
 int fo_sph_batch_coeffs(fo_sph_batch* b, double* coef) {
   if (!b || !coef) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (b->mirror_pending) {
+    launch_sph_mirror(b->stream, b->lc.P, b->lc.Lmax, b->pd, b->I64A);
+    FO_CHECK_LAUNCH();
+    b->mirror_pending = false;
+  }
   FO_HIP(hipMemcpyAsync(coef, b->I64A, sizeof(double2) * b->Ktot, hipMemcpyDeviceToHost, b->stream));
   FO_HIP(hipStreamSynchronize(b->stream));
   return FO_OK;

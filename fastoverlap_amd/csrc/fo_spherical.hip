@@ -602,22 +602,14 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
         I64[id] = v;
         I32[d.c32_off + (long long)qlm(l, m1) * n + plane_pos(plane_k(m2, n), n)] =
             make_float2((float)v.x, (float)v.y);
-        if (m1 > 0) {
-          const long long im = d.coef_off + kidx(l, -m1, -m2);
-          const double sg = ((m1 + m2) & 1) ? -1.0 : 1.0;
-          I64[im] = make_double2(sg * v.x, -sg * v.y);
-        }
       }
     }
     __syncthreads();
   }
-  // plane-layout entries |m2| > l of the block's rows are zero
-  for (int e = t; e < rows * n; e += 256) {
-    const int r = e / n, k = e - r * n;
-    const int m2 = k < n / 2 ? k : k - n;
-    if (abs(m2) > l)
-      I32[d.c32_off + (long long)qlm(l, m1a + r) * n + plane_pos(k, n)] = make_float2(0.f, 0.f);
-  }
+  // Only rows m1 >= 0 are written: the selection and the refinement read
+  // nothing else, the m1 < 0 rows of I64 are filled by k_sph_mirror when the
+  // coefficients are read out, and the plane-layout entries |m2| > l stay
+  // at the zeros the batch was bound with.
 }
 
 // NCTMAX: column tiles of the largest degree in the launch (2l+1 <= 16 NCTMAX);
@@ -2284,6 +2276,26 @@ namespace fo {
 
 // FP32 plane-layout copy of compact FP64 coefficients, one thread per
 // (row q(l, m1), k) entry of pair blockIdx.y (zero where |m2| > l)
+// Rows m1 < 0 of the compact C^l from the Hermitian identity
+// I^l_{-m1,-m2} = (-1)^{m1+m2} conj(I^l_{m1,m2}) (the coefficient sets of
+// real densities; fastoverlap/sphericalAlignment.py:269-273 builds them in
+// full).  One thread per (l, m1 > 0, m2) entry of a pair.
+__global__ void k_sph_mirror(const SphPair* __restrict__ pd, double2* __restrict__ I) {
+  const SphPair d = pd[blockIdx.y];
+  const int L = d.L;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  // entries (l, m1 > 0, m2) of degree l: l (2l + 1); prefix sum_{l' < l} = (l-1) l (4l+1) / 6
+  if (e >= (long long)L * (L + 1) * (4 * L + 5) / 6) return;
+  int l = (int)cbrt(1.5 * (double)e);
+  while (l > 0 && (long long)(l - 1) * l * (4 * l + 1) / 6 > e) --l;
+  while ((long long)l * (l + 1) * (4 * l + 5) / 6 <= e) ++l;
+  const int r = (int)(e - (long long)(l - 1) * l * (4 * l + 1) / 6);
+  const int m1 = 1 + r / (2 * l + 1), m2 = r % (2 * l + 1) - l;
+  const double2 v = I[d.coef_off + kidx(l, m1, m2)];
+  const double sg = ((m1 + m2) & 1) ? -1.0 : 1.0;
+  I[d.coef_off + kidx(l, -m1, -m2)] = make_double2(sg * v.x, -sg * v.y);
+}
+
 __global__ void k_coef_to_plane(const SphPair* __restrict__ pd, int n,
                                 const double2* __restrict__ a, float2* __restrict__ b) {
   const SphPair d = pd[blockIdx.y];
@@ -2411,6 +2423,12 @@ __global__ void k_ylm_angles(int L, int npts, const double* __restrict__ theta,
       }
     }
   }
+}
+
+void launch_sph_mirror(hipStream_t s, int P, int Lmax, const SphPair* pd, double2* I) {
+  const long long tot = (long long)Lmax * (Lmax + 1) * (4 * Lmax + 5) / 6;
+  if (tot == 0) return;
+  hipLaunchKernelGGL(k_sph_mirror, dim3(cdiv(tot, 256), P), dim3(256), 0, s, pd, I);
 }
 
 void launch_coef_to_plane(hipStream_t s, int P, int n, int Lmax, const SphPair* pd,
