@@ -60,7 +60,7 @@ struct SphPeak {
 struct SphOpt {
   double R[3];         // optimised Euler angles
   double f;            // objective -Re sum conj(I) D(R)
-  double gnorm;        // final gradient max-norm
+  double gnorm;        // gradient max-norm at the last evaluated point
   int iters;
   int evals;           // objective evaluations, line search included
 };

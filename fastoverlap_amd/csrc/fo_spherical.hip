@@ -1002,6 +1002,9 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
     const int m1 = m1_0 + grp;
     const bool row = m1 <= L;
     const int m1w = m1_0 + wrow;  // wavefront-uniform
+    // a wavefront whose rows all lie above L has nothing to transform (the
+    // pass has no block-wide barrier, and the exchange is wavefront-local)
+    if (m1w > L) continue;
     // sums over degrees l = m1w, m1w + 2, ... (A) and m1w + 1, m1w + 3, ... (B):
     // the even / odd-l partial sums (alignment mode) in some order, or, in
     // general mode, the two coefficient sets (A, B) over all degrees
@@ -1965,6 +1968,7 @@ __global__ void __launch_bounds__(256, 3) k_sph_rotopt(const SphPair* __restrict
     const double hs = fmax(fabs(H[0]), fmax(fabs(H[3]), fmax(fabs(H[5]), 1e-30)));
     double lam = 0.0;
     bool ok = chol3_solve(H, lam, g, dx);
+    const bool pure = ok;  // undamped Newton step (positive definite Hessian)
     for (int k = 0; !ok && k < 12; ++k) {
       lam = hs * pow(10.0, -8.0 + k);
       ok = chol3_solve(H, lam, g, dx);
@@ -1977,6 +1981,15 @@ __global__ void __launch_bounds__(256, 3) k_sph_rotopt(const SphPair* __restrict
     slope = g[0] * dx[0] + g[1] * dx[1] + g[2] * dx[2];
     const double dxn = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
     if (-0.5 * slope <= 1e-13 * fscale || dxn <= 1e-12) break;
+    // an undamped Newton step this short lands within O(|dx|^2) (~1e-10
+    // rad) of the optimum: take it without evaluating there, with the
+    // objective of the quadratic model (error O(|dx|^3))
+    if (pure && dxn <= 1e-5) {
+      for (int q = 0; q < 3; ++q) x[q] += dx[q];
+      cur.v[0] += 0.5 * slope;
+      ++it;
+      break;
+    }
     step = 1.0;
     ls = 0;
     for (int q = 0; q < 3; ++q) xt[q] = x[q] + dx[q];
@@ -1987,7 +2000,7 @@ __global__ void __launch_bounds__(256, 3) k_sph_rotopt(const SphPair* __restrict
     r.R[1] = x[1];
     r.R[2] = x[2];
     r.f = cur.v[0];
-    r.gnorm = fmax(fabs(cur.v[1]), fmax(fabs(cur.v[2]), fabs(cur.v[3])));
+    r.gnorm = fmax(fabs(cur.v[1]), fmax(fabs(cur.v[2]), fabs(cur.v[3])));  // last evaluated point
     r.iters = it;
     r.evals = nev;
     opt[(long long)p * norient + o] = r;

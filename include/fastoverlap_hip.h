@@ -120,9 +120,10 @@ int fo_sph_batch_results(fo_sph_batch* b, double* dist, double* X1, double* X2, 
  * FP64 candidates, fully re-evaluated planes */
 int fo_sph_batch_peaks(fo_sph_batch* b, int* ind, double* fun, double* peak_value,
                        double* margin, double* delta, int* ncand, int* noverflow);
-/* rotation refinement per (pair, orientation): R[3], objective, final
- * gradient max-norm, Newton iterations, objective evaluations (incl. the
- * line search) */
+/* rotation refinement per (pair, orientation): R[3], objective, gradient
+ * max-norm at the last evaluated point (a final Newton step shorter than
+ * 1e-5 rad is taken without evaluating, with the model objective), Newton
+ * iterations, objective evaluations (incl. the line search) */
 int fo_sph_batch_opt(fo_sph_batch* b, double* R, double* fun, double* gnorm, int* iters,
                      int* evals);
 /* kernel width used per pair (resolved on the device when <= 0 was given) */
