@@ -759,61 +759,58 @@ FO_DEV float dpp_mov(float x) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, true));
 }
 
-// Wavefront all-reduce of (value, flat index) pairs, larger value first and
-// ties to the smaller index: butterflies inside each row of 16 lanes by DPP
-// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror),
-// then the four row results through readlane.  Every lane must be active.
-template <int CTRL>
-FO_DEV void argmax_dpp_step(float& v, int& i) {
-  const float ov = dpp_mov<CTRL>(v);
-  const int oi = __builtin_amdgcn_update_dpp(0, i, CTRL, 0xf, 0xf, true);
-  argmax_merge(v, i, ov, oi);
+// Records (largest value v, its flat index i, largest of the other values
+// s): larger value first, ties to the smaller index; the loser's largest
+// value joins the others (so s counts a repeated maximum).
+FO_DEV void top2_merge(float& v, int& i, float& s, float ov, int oi, float os) {
+  const bool take = ov > v || (ov == v && oi < i);
+  s = fmaxf(fmaxf(s, os), take ? v : ov);
+  v = take ? ov : v;
+  i = take ? oi : i;
 }
-FO_DEV void wave_argmax_dpp(float& v, int& i) {
-  argmax_dpp_step<0xB1>(v, i);
-  argmax_dpp_step<0x4E>(v, i);
-  argmax_dpp_step<0x141>(v, i);
-  argmax_dpp_step<0x140>(v, i);
-  float rv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-  int ri = __builtin_amdgcn_readlane(i, 0);
-#pragma unroll
-  for (int r = 16; r < 64; r += 16)
-    argmax_merge(rv, ri, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), r)),
-                 __builtin_amdgcn_readlane(i, r));
-  v = rv;
-  i = ri;
-}
-FO_DEV float wave_max_dpp(float v) {
-  v = fmaxf(v, dpp_mov<0xB1>(v));
-  v = fmaxf(v, dpp_mov<0x4E>(v));
-  v = fmaxf(v, dpp_mov<0x141>(v));
-  v = fmaxf(v, dpp_mov<0x140>(v));
-  float r = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-#pragma unroll
-  for (int q = 16; q < 64; q += 16)
-    r = fmaxf(r, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q)));
-  return r;
-}
-
 // Plane candidates of a 256-thread workgroup: wavefront w fills entry w of
-// the plane's record with its largest value and a bound on its other values
-// (no LDS, no barrier).  Every thread must call it.
+// the plane's record with its largest value and the largest of its other
+// values (no LDS, no barrier).  One butterfly of (v, i, s) records per
+// orientation inside each row of 16 lanes by DPP (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror), the orientations
+// interleaved, then the four row results through readlane.  Every thread
+// must call it.
+template <int CTRL>
+FO_DEV void top2_dpp_step(float (&v)[2], int (&i)[2], float (&s)[2]) {
+  float ov[2], os[2];
+  int oi[2];
+#pragma unroll
+  for (int o = 0; o < 2; ++o) {
+    ov[o] = dpp_mov<CTRL>(v[o]);
+    oi[o] = __builtin_amdgcn_update_dpp(0, i[o], CTRL, 0xf, 0xf, true);
+    os[o] = dpp_mov<CTRL>(s[o]);
+  }
+#pragma unroll
+  for (int o = 0; o < 2; ++o) top2_merge(v[o], i[o], s[o], ov[o], oi[o], os[o]);
+}
 FO_DEV void plane_candidates(const CandTrack& ct, int norient, PlaneCand* out, int ostride) {
   const int wl = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int o = 0; o < norient; ++o) {
-    const float bv = o == 0 ? ct.c0.bv : ct.c1.bv;
-    const int bi = o == 0 ? ct.c0.bi : ct.c1.bi;
-    const float dm = o == 0 ? ct.c0.dm : ct.c1.dm;
-    float v = bv;
-    int i = bi;
-    wave_argmax_dpp(v, i);
-    // the lane that holds the winner bounds the rest by its second value
-    const float rest = wave_max_dpp(i == bi && v == bv ? dm : fmaxf(bv, dm));
-    if (wl == 0) {
+  float v[2] = {ct.c0.bv, ct.c1.bv}, s[2] = {ct.c0.dm, ct.c1.dm};
+  int i[2] = {ct.c0.bi, ct.c1.bi};
+  top2_dpp_step<0xB1>(v, i, s);
+  top2_dpp_step<0x4E>(v, i, s);
+  top2_dpp_step<0x141>(v, i, s);
+  top2_dpp_step<0x140>(v, i, s);
+#pragma unroll
+  for (int o = 0; o < 2; ++o) {
+    float rv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[o]), 0));
+    float rs = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s[o]), 0));
+    int ri = __builtin_amdgcn_readlane(i[o], 0);
+#pragma unroll
+    for (int r = 16; r < 64; r += 16)
+      top2_merge(rv, ri, rs, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[o]), r)),
+                 __builtin_amdgcn_readlane(i[o], r),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s[o]), r)));
+    if (o < norient && wl == 0) {
       PlaneCand* dst = out + (long long)o * ostride;
-      dst->val[w] = v;
-      dst->idx[w] = i;
-      dst->rest[w] = rest;
+      dst->val[w] = rv;
+      dst->idx[w] = ri;
+      dst->rest[w] = rs;
     }
   }
 }
