@@ -127,6 +127,48 @@ def test_fft_and_dft_planes_agree(gpu):
         _check_pair(name, o, r["dist"][0], r["perm"][0], r["inverted"][0], pk["ind"][0], g)
 
 
+def test_fft_planes_grid_16_against_oracle(gpu):
+    """The two-pass plane transform at n = 16 (4 x 4), a grid side the C2
+    mix does not use: the rows of a wavefront span several degree segments.
+    Against the oracle: argmax, permutation, distance, FP32 grid within 1e-5."""
+    c = W.config2(3)
+    n, L = 16, 7
+    say(f"case: pair 3 N={len(c['pos1'])} L={L} n={n} against the oracle")
+    o = O.spherical_align(c["pos1"], c["pos2"], L, c["scale"], perm=c["groups"], bw=n // 2,
+                          fast=True, keep_grids=True)
+    with SphericalBatch([len(c["pos1"])], [L], 0.3, n, [c["groups"]], _lib.SPH_KEEP_GRID) as b:
+        b.set_positions(c["pos1"], c["pos2"])
+        b.run()
+        r = b.results()
+        pk = b.peaks()
+        grids = b.grid(0)
+    _check_pair(f"n={n}", o, r["dist"][0], r["perm"][0], r["inverted"][0], pk["ind"][0], grids)
+
+
+def test_fft_planes_grid_256_against_dft_planes(gpu):
+    """The two-pass plane transform at n = 256 (16 x 16, eight degree
+    segments) against the direct-DFT plane kernel on the same pair (the
+    oracle's 256^3 transform takes minutes on the host): identical argmax,
+    permutation and distance, FP32 grids within 1e-5 of each other."""
+    c = W.config2(5)
+    n, L = 256, 12
+    say(f"case: pair 5 N={len(c['pos1'])} L={L} n={n}: FFT planes vs direct-DFT planes")
+    out = {}
+    for name, flag in (("fft", 0), ("dft", _lib.SPH_DFT)):
+        with SphericalBatch([len(c["pos1"])], [L], 0.3, n, [c["groups"]],
+                            _lib.SPH_KEEP_GRID | flag) as b:
+            b.set_positions(c["pos1"], c["pos2"])
+            b.run()
+            out[name] = (b.results(), b.peaks(), b.grid(0))
+    (rf, pf, gf), (rd, pdd, gd) = out["fft"], out["dft"]
+    np.testing.assert_array_equal(pf["ind"][0], pdd["ind"][0])
+    np.testing.assert_array_equal(rf["perm"][0], rd["perm"][0])
+    assert bool(rf["inverted"][0]) == bool(rd["inverted"][0])
+    assert abs(rf["dist"][0] - rd["dist"][0]) <= DIST_TOL
+    for k in range(2):
+        assert np.abs(gf[k] - gd[k]).max() <= GRID_TOL * np.abs(gd[k]).max(), k
+
+
 def test_lj38_known_answer_dropin(gpu, lj38):
     say("case: LJ38 SphericalAlign(0.3, Jmax=14) drop-in (n=30, generic DFT path)")
     p1, p2 = lj38
