@@ -247,11 +247,10 @@ __global__ void k_leg_table(int Lt, double2* __restrict__ leg, double* __restric
 // for x < 1; otherwise Miller's backward recurrence started at
 // l_s = ceil(sqrt(L^2 + 40x)) + 10 (where the minimal-solution
 // contamination is below 1e-17) and normalised by i_0(x) e^{-x} =
-// -expm1(-2x)/(2x).  One pass: the values f_l, l <= L, are kept in
-// registers (LB >= L, unrolled) until the normalisation is known at l = 0.
+// -expm1(-2x)/(2x).  Two passes of the same recurrence: the first finds
+// the normalisation at l = 0, the second stores the values as it goes.
 // Layout: R[rad_off + grp.rad_off + (l*Ns + j)*Ns + k].
 // ---------------------------------------------------------------------------
-template <int LB>
 FO_DEV void radial_store(int L, double x, double gauss, double* out, long long stride) {
   if (x == 0.0) {
     out[0] = gauss;
@@ -282,6 +281,7 @@ FO_DEV void radial_store(int L, double x, double gauss, double* out, long long s
   const int ls = (int)ceil(sqrt((double)L * L + 40.0 * x)) + 10;
   const double ix = 1.0 / x;
   double fp1 = 0.0, f = 1.0;
+#pragma unroll 2
   for (int l = ls; l > 0; --l) {
     const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
     fp1 = f;
@@ -290,24 +290,24 @@ FO_DEV void radial_store(int L, double x, double gauss, double* out, long long s
   const double scal = i0 / f * gauss;
   fp1 = 0.0;
   f = 1.0;
+#pragma unroll 2
   for (int l = ls; l > L; --l) {
     const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
     fp1 = f;
     f = fm;
   }
-#pragma unroll
-  for (int l = LB; l >= 0; --l) {
-    if (l <= L) {
-      out[l * stride] = f * scal;
-      if (l > 0) {
-        const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
-        fp1 = f;
-        f = fm;
-      }
+  // degrees L .. 0 (L is uniform over the workgroup: a uniform loop)
+  double* o = out + (long long)L * stride;
+  for (int l = L; l >= 0; --l) {
+    *o = f * scal;
+    o -= stride;
+    if (l > 0) {
+      const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
+      fp1 = f;
+      f = fm;
     }
   }
 }
-template <int LB>
 __global__ void __launch_bounds__(256, 5) k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
                              const int* __restrict__ gath, const double* __restrict__ X1c,
                              const double* __restrict__ X2c, const double* __restrict__ scale_eff,
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(256, 5) k_sph_radial(const SphPair* __restrict
     const double r2 = sqrt(x2[0] * x2[0] + x2[1] * x2[1] + x2[2] * x2[2]);
     const double x = r1 * r2 * i2s2;
     const double gauss = exp(-(r1 - r2) * (r1 - r2) * i4s2);
-    radial_store<LB>(d.L, x, gauss, R + d.rad_off + g[q].rad_off + (long long)j * Ns + k,
+    radial_store(d.L, x, gauss, R + d.rad_off + g[q].rad_off + (long long)j * Ns + k,
                  (long long)Ns * Ns);
   }
 }
@@ -2152,16 +2152,9 @@ void launch_sph_radial(hipStream_t s, int P, int Lmax, long long maxjk, const Sp
                        const SphGroup* grp, const int* gath, const double* X1c,
                        const double* X2c, const double* scale_eff, double* R) {
   const int bx = std::max(1, std::min(cdiv(maxjk, 256), 1024));
-#define FO_RAD(LB)                                                                              \
-  hipLaunchKernelGGL(k_sph_radial<LB>, dim3(bx, P), dim3(256), 0, s, pd, grp, gath, X1c, X2c,   \
-                     scale_eff, R)
-  if (Lmax <= 15)
-    FO_RAD(15);
-  else if (Lmax <= 31)
-    FO_RAD(31);
-  else
-    FO_RAD(63);
-#undef FO_RAD
+  (void)Lmax;  // one kernel for every degree
+  hipLaunchKernelGGL(k_sph_radial, dim3(bx, P), dim3(256), 0, s, pd, grp, gath, X1c, X2c,
+                     scale_eff, R);
 }
 
 void launch_sph_contract(hipStream_t s, int P, int n, int Lmax, const SphPair* pd,
