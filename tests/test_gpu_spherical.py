@@ -169,6 +169,34 @@ def test_fft_planes_grid_256_against_dft_planes(gpu):
         assert np.abs(gf[k] - gd[k]).max() <= GRID_TOL * np.abs(gd[k]).max(), k
 
 
+def test_chunked_coefficient_stage_is_bit_identical(gpu, monkeypatch):
+    """FO_COEF_CHUNK_MB caps the harmonics / radial footprint by building and
+    contracting the coefficients in chunks of pairs through one buffer; the
+    per-pair arithmetic is unchanged, so every result is bit-identical."""
+    pairs = [W.config2(p) for p in range(200, 216)]
+    sel = [c for c in pairs if c["grid_n"] == 64]
+    out = []
+    for chunk in (None, "1"):
+        if chunk is None:
+            monkeypatch.delenv("FO_COEF_CHUNK_MB", raising=False)
+        else:
+            monkeypatch.setenv("FO_COEF_CHUNK_MB", chunk)
+        say(f"case: {len(sel)} C2 pairs n=64, coefficient chunks of {chunk or 'unlimited'} MB")
+        with SphericalBatch([len(c["pos1"]) for c in sel], [c["L"] for c in sel], 0.3, 64,
+                            [c["groups"] for c in sel]) as b:
+            b.set_positions([c["pos1"] for c in sel], [c["pos2"] for c in sel])
+            b.run()
+            out.append((b.results(), b.peaks(), b.coeffs()))
+    monkeypatch.delenv("FO_COEF_CHUNK_MB", raising=False)
+    (r0, p0, c0), (r1, p1, c1) = out
+    np.testing.assert_array_equal(r0["dist"], r1["dist"])
+    np.testing.assert_array_equal(p0["ind"], p1["ind"])
+    for a, e in zip(r0["perm"], r1["perm"]):
+        np.testing.assert_array_equal(a, e)
+    for a, e in zip(c0, c1):
+        np.testing.assert_array_equal(a, e)
+
+
 def test_lj38_known_answer_dropin(gpu, lj38):
     say("case: LJ38 SphericalAlign(0.3, Jmax=14) drop-in (n=30, generic DFT path)")
     p1, p2 = lj38
