@@ -1,6 +1,7 @@
 """Benchmark: aligned cluster pairs/s of the batched GPU path on the
 4096-pair mixed workload, beside the plain per-call GPU drop-in, the FP64 CPU
-oracle and, with --serve, a served Poisson stream (p50/p99 latency).
+oracle, the periodic path on C1 pairs and, with --serve, a served Poisson
+stream (p50/p99 latency).
 
 One "step" = one pass of the full alignment (coefficients, inverse SOFT,
 exact argmax, rotation refinement, LAP, quaternion fit, both orientations)
@@ -189,6 +190,49 @@ def plain_mix(reqs):
     return len(reqs) / (time.perf_counter() - t0)
 
 
+def periodic_batched(ctx, pairs, steps, warmup, plain_sample):
+    """Supplement: the periodic path on C1 pairs (256-atom 80:20 binary
+    mixtures, box 6, k-cube n = 8, 64^3 translation grid), all pairs in one
+    batch (one hipGraph replay per step), beside sequential plain
+    PeriodicAlign calls on a sample with the same FFT side, and how many of
+    those give the batch's displacement index and permutation."""
+    from fastoverlap_amd.batch import PeriodicBatch
+    from fastoverlap_amd.periodic import PeriodicAlign
+    c0 = pairs[0]
+    P = len(pairs)
+    b = PeriodicBatch([len(c["pos1"]) for c in pairs], [c["box"] for c in pairs],
+                      [c["scale"] for c in pairs], c0["nk"], c0["F"],
+                      [c["groups"] for c in pairs], 10, 0, ctx)
+    b.set_positions([c["pos1"] for c in pairs], [c["pos2"] for c in pairs])
+    for _ in range(warmup):
+        b.run(graph=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run(graph=True, sync=False)
+    b.sync()
+    dt = (time.perf_counter() - t0) / steps
+    b.enable_timing(True)
+    b.run(graph=False)
+    kt = b.last_timing()
+    b.enable_timing(False)
+    r, pk = b.results(), b.peaks()
+    b.close()
+    agree = 0
+    t0 = time.perf_counter()
+    for i, c in enumerate(pairs[:plain_sample]):
+        pa = PeriodicAlign(len(c["pos1"]), c["box"], c["groups"], scale=c["scale"], n=c["nk"])
+        pa.fshape = (c["F"],) * 3
+        _, _, _, perm, _ = pa.align(c["pos1"], c["pos2"])
+        agree += int(np.array_equal(pa.last["peaks"]["ind"][0], pk["ind"][i])
+                     and np.array_equal(perm, r["perm"][i]))
+    plain = plain_sample / (time.perf_counter() - t0)
+    return {"workload": f"C1: {P} periodic pairs, N=256 (204:52), box 6.0, k-cube n=8, "
+                        f"64^3 grid, sigma 0.3", "pairs": P, "ms_per_step": dt * 1e3,
+            "pairs_per_s": P / dt, "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+            "plain_pairs_per_s": plain, "speedup_vs_plain": (P / dt) / plain,
+            "plain_identical_ind_perm": f"{agree}/{plain_sample}"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +251,9 @@ def main():
                     help="also run the served mix (off by default: a run of the full "
                          "command that included it ended in an unexplained GPU fault)")
     ap.add_argument("--no-serve", action="store_true", help="(default) skip the served mix")
+    ap.add_argument("--periodic-pairs", type=int, default=128,
+                    help="C1 pairs of the periodic supplement (0: skip it)")
+    ap.add_argument("--no-periodic", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -222,7 +269,10 @@ def main():
     run_serve = args.serve and not args.no_serve
     serve_reqs = ([] if not run_serve else
                   W.generate(W.config4, range(args.serve_requests), workers=args.gen_workers))
-    log(f"generated {len(pairs)} pairs and {len(serve_reqs)} served requests in "
+    nper = 0 if args.no_periodic else args.periodic_pairs
+    per_pairs = W.generate(W.config1, range(nper), workers=args.gen_workers) if nper else []
+    log(f"generated {len(pairs)} pairs, {len(per_pairs)} periodic pairs and "
+        f"{len(serve_reqs)} served requests in "
         f"{time.perf_counter() - t:.1f}s")
     ctx = _lib.default_context()
     batches = make_batches(pairs, ctx, args.streams)
@@ -336,6 +386,10 @@ def main():
         "plain_identical_argmax_perm": f"{agree}/{len(sample)}",
         "roofline": roof,
     }
+    if per_pairs:
+        out["periodic"] = periodic_batched(ctx, per_pairs, args.steps, args.warmup,
+                                           min(4, len(per_pairs)))
+        log(f"periodic: {out['periodic']}")
     if run_serve:
         out["serve"] = served(ctx, serve_reqs)
         log(f"served: {out['serve']}")
