@@ -68,13 +68,16 @@ __global__ void __launch_bounds__(256) k_per_sf(
             }
           }
           __syncthreads();
+          // e^{-i kx x} (fixed kx per workgroup) folded into the y phases once
+          // per tile: the product the sum below used to form per entry
+          for (int w = t; w < jt * NB; w += blockDim.x) ey[w] = cmul(ex[w / NB], ey[w]);
+          __syncthreads();
           for (int q = 0; q < kSFChunk; ++q) {
             const int e = e0 + q * blockDim.x + t;
             if (e >= NB * NB) break;
             const int b = e / NB, c = e % NB;
             double2 sacc = acc[s][q];
-            for (int j = 0; j < jt; ++j)
-              sacc = cadd(sacc, cmul(cmul(ex[j], ey[j * NB + b]), ez[j * NB + c]));
+            for (int j = 0; j < jt; ++j) sacc = cadd(sacc, cmul(ey[j * NB + b], ez[j * NB + c]));
             acc[s][q] = sacc;
           }
         }
