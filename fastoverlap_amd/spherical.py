@@ -178,13 +178,15 @@ class BaseSphericalAlignment:
         L = self.Jmax
         if calcCoeffs is None:
             flags = 0 if invert else _lib.SPH_NO_INVERT
-            with SphericalBatch([len(pos1)], [L], [scale], self._grid_n(), groups, flags) as b:
-                b.set_positions(pos1, pos2)
-                b.run(_lib.STAGE_ALL)
-                r = b.results()
-                diag = dict(peaks=b.peaks(), opt=b.opt())
-                if self.calcScale:
-                    self.scale = float(b.scales()[0])
+            b, reused = self._plain_batch(len(pos1), L, scale, groups, flags)
+            b.set_positions(pos1, pos2)
+            # a repeated shape replays the captured launch graph; a new one
+            # runs eagerly (a capture costs more than one eager pass)
+            b.run(_lib.STAGE_ALL, graph=reused)
+            r = b.results()
+            diag = dict(peaks=b.peaks(), opt=b.opt())
+            if self.calcScale:
+                self.scale = float(b.scales()[0])
         else:
             A = dense_to_compact(np.asarray(calcCoeffs(False)))
             flags = _lib.SPH_GENERAL if invert else _lib.SPH_NO_INVERT
@@ -201,6 +203,25 @@ class BaseSphericalAlignment:
         self.last = dict(perm=r["perm"][0], R=r["R"][0], rmat=r["rmat"][0],
                          inverted=bool(r["inverted"][0]), **diag)
         return out
+
+    def _plain_batch(self, n, L, scale, groups, flags):
+        """The device batch of the last plain call and whether it is reused:
+        it is kept while the shape repeats (atoms, degree, grid side, kernel
+        width, flags, permutation groups), so repeated alignments of one
+        system reuse its buffers and its captured launch graph.  Like the
+        reference's instances (mutable scratch), an instance is not safe for
+        concurrent calls."""
+        key = (n, L, self._grid_n(), scale, flags,
+               tuple(tuple(int(i) for i in g) for g in groups[0]))
+        cached = getattr(self, "_plain", None)
+        if cached is not None and cached[0] == key:
+            return cached[1], True
+        if cached is not None:
+            cached[1].close()
+            self._plain = None
+        b = SphericalBatch([n], [L], [scale], self._grid_n(), groups, flags)
+        self._plain = (key, b)
+        return b, False
 
     def findRotation(self, Ilmm):
         """iSOFT -> exact argmax -> refinement (``:190-194``): (R, fun)."""

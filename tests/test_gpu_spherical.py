@@ -197,6 +197,27 @@ def test_chunked_coefficient_stage_is_bit_identical(gpu, monkeypatch):
         np.testing.assert_array_equal(a, e)
 
 
+def test_dropin_repeated_shape_reuses_its_batch(gpu):
+    """Repeated plain calls with one system size reuse the instance's device
+    batch (graph replays after the first call); every result is identical to
+    a fresh instance's."""
+    c = W.config0()
+    rng = np.random.default_rng(5)
+    partners = [c["pos2"]] + [c["pos2"].dot(O.euler_matrix(*(rng.random(3) * [6.2, 3.1, 6.2])).T)
+                              for _ in range(2)]
+    sa = fo.SphericalAlign(0.3, 15)
+    for k, p2 in enumerate(partners + partners[:1]):
+        say(f"case: C0 partner {k % 3}, call {k} on one instance")
+        d, _, X2 = sa(c["pos1"], p2)
+        perm, ind = sa.last["perm"].copy(), sa.last["peaks"]["ind"].copy()
+        fresh = fo.SphericalAlign(0.3, 15)
+        df, _, X2f = fresh(c["pos1"], p2)
+        assert d == df, (k, d, df)
+        np.testing.assert_array_equal(perm, fresh.last["perm"])
+        np.testing.assert_array_equal(ind, fresh.last["peaks"]["ind"])
+        np.testing.assert_array_equal(X2, X2f)
+
+
 def test_lj38_known_answer_dropin(gpu, lj38):
     say("case: LJ38 SphericalAlign(0.3, Jmax=14) drop-in (n=30, generic DFT path)")
     p1, p2 = lj38
