@@ -124,3 +124,21 @@ def test_periodic_batch_independence(gpu):
         r1 = b1.results()
     assert r1["dist"][0] == r["dist"][4]
     np.testing.assert_array_equal(r1["perm"][0], r["perm"][4])
+
+
+def test_dropin_repeated_calls_reuse_the_batch(gpu):
+    """Repeated PeriodicAlign calls on one instance reuse its device batch
+    (graph replays after the first call); every result is identical to a
+    fresh instance's."""
+    c = W.config1()
+    partners = [c["pos2"], c["pos2"] + np.array([0.7, -1.3, 2.2]), c["pos2"]]
+    pa = fo.PeriodicAlign(256, c["box"], c["groups"], scale=0.3, n=8)
+    for k, p2 in enumerate(partners):
+        say(f"case: C1 partner {k} on one instance")
+        d, X1, X2, perm, disp = pa.align(c["pos1"], p2)
+        fresh = fo.PeriodicAlign(256, c["box"], c["groups"], scale=0.3, n=8)
+        df, X1f, X2f, permf, dispf = fresh.align(c["pos1"], p2)
+        assert d == df, (k, d, df)
+        np.testing.assert_array_equal(perm, permf)
+        np.testing.assert_array_equal(disp, dispf)
+        np.testing.assert_array_equal(X2, X2f)
