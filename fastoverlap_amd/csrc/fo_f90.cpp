@@ -7,6 +7,7 @@
 // (f90/fastutils.f90:117-167).  Errors are returned as status codes (the
 // reference STOPs); the numerical work is the batch pipeline of fo_api.cpp.
 #include <cmath>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -50,6 +51,29 @@ int groups_from_f90(int natoms, int ngroups, const int* npermsize, const int* pe
       return fo::set_error(FO_ERR_INVALID, "PERMGROUP entry out of range (1-based indices)");
   }
   return FO_OK;
+}
+
+// The batch of the last call of each kind, kept while the next call has
+// the same shape (GMIN / OPTIM align one system size over and over): the
+// call then uploads positions and replays the batch's captured launch
+// graph.  The key lists every input that shapes the batch.  Calls are
+// serialised (the reference's Fortran modules are not re-entrant either);
+// the batches live until a different shape replaces them.
+struct CachedBatch {
+  std::vector<double> key;
+  fo_sph_batch* sph = nullptr;
+  fo_per_batch* per = nullptr;
+};
+std::mutex g_call_mu;
+CachedBatch g_sph_cache, g_per_cache;
+
+std::vector<double> shape_key(std::initializer_list<double> head, const std::vector<int>& sizes,
+                              const std::vector<int>& atoms) {
+  std::vector<double> k(head);
+  k.push_back((double)sizes.size());
+  for (int v : sizes) k.push_back(v);
+  for (int v : atoms) k.push_back(v);
+  return k;
 }
 
 int next_fast_len(int target) {
@@ -96,18 +120,25 @@ int fo_f90_cluster_align(double* coordsb, double* coordsa, int natoms, int L, do
   d.group_sizes = sizes.data();
   d.group_atoms = atoms.data();
   d.flags = invert ? 0 : FO_SPH_NO_INVERT;
-  fo_sph_batch* b = nullptr;
-  rc = fo_sph_batch_create(ctx, &d, &b);
-  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_call_mu);
+  const std::vector<double> key = shape_key({(double)natoms, (double)L, sc, (double)d.flags}, sizes, atoms);
+  const bool reused = g_sph_cache.sph && g_sph_cache.key == key;
+  if (!reused) {
+    if (g_sph_cache.sph) fo_sph_batch_destroy(g_sph_cache.sph);
+    g_sph_cache.sph = nullptr;
+    rc = fo_sph_batch_create(ctx, &d, &g_sph_cache.sph);
+    if (rc) return rc;
+    g_sph_cache.key = key;
+  }
+  fo_sph_batch* b = g_sph_cache.sph;
   std::vector<double> X1(3 * (size_t)natoms), X2(3 * (size_t)natoms);
   std::vector<int> perm(natoms);
   double dist = 0, R[3], rmat[9];
   int inv = 0;
   rc = fo_sph_batch_set_positions(b, coordsb, coordsa);
-  if (!rc) rc = fo_sph_batch_run(b, FO_STAGE_ALL, 0);
+  if (!rc) rc = fo_sph_batch_run(b, FO_STAGE_ALL, reused ? 1 : 0);
   if (!rc) rc = fo_sph_batch_results(b, &dist, X1.data(), X2.data(), perm.data(), R, rmat, &inv);
   if (!rc) rc = fo_sph_batch_scales(b, &sc);
-  fo_sph_batch_destroy(b);
   if (rc) return rc;
   for (size_t i = 0; i < X1.size(); ++i) {
     coordsb[i] = X1[i];
@@ -159,16 +190,24 @@ int fo_f90_bulk_align(double* coordsb, double* coordsa, int natoms, double boxlx
   d.group_atoms = atoms.data();
   d.niter = 10;
   d.flags = 0;
-  fo_per_batch* b = nullptr;
-  rc = fo_per_batch_create(ctx, &d, &b);
-  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_call_mu);
+  const std::vector<double> key =
+      shape_key({(double)natoms, boxlx, boxly, boxlz, sc, (double)nk, (double)d.F}, sizes, atoms);
+  const bool reused = g_per_cache.per && g_per_cache.key == key;
+  if (!reused) {
+    if (g_per_cache.per) fo_per_batch_destroy(g_per_cache.per);
+    g_per_cache.per = nullptr;
+    rc = fo_per_batch_create(ctx, &d, &g_per_cache.per);
+    if (rc) return rc;
+    g_per_cache.key = key;
+  }
+  fo_per_batch* b = g_per_cache.per;
   std::vector<double> X2(3 * (size_t)natoms);
   std::vector<int> perm(natoms);
   double dist = 0, disp[3];
   rc = fo_per_batch_set_positions(b, coordsb, coordsa);
-  if (!rc) rc = fo_per_batch_run(b, FO_PSTAGE_ALL, 0);
+  if (!rc) rc = fo_per_batch_run(b, FO_PSTAGE_ALL, reused ? 1 : 0);
   if (!rc) rc = fo_per_batch_results(b, &dist, nullptr, X2.data(), perm.data(), disp);
-  fo_per_batch_destroy(b);
   if (rc) return rc;
   for (size_t i = 0; i < X2.size(); ++i) coordsa[i] = X2[i];
   if (distance) *distance = dist;

@@ -27,6 +27,9 @@ def test_fortran_align_matches_dropin(gpu, lj38, blj256):
     cdist, cstat = float(out["CLUSTER"][0]), int(out["CLUSTER"][1])
     bdist, bstat = float(out["BULK"][0]), int(out["BULK"][1])
     assert cstat == 0 and bstat == 0
+    # second calls with the same system reuse the first call's batch
+    assert out["CLUSTER2"] == out["CLUSTER"] and out["CPERM2"] == out["CPERM"]
+    assert out["BULK2"] == out["BULK"]
     p1, p2 = lj38
     o = O.spherical_align(p1, p2, 14, 0.3, fast=True)
     assert abs(cdist - o.dist) <= 1e-8
