@@ -345,11 +345,16 @@ __global__ void __launch_bounds__(256, 5) k_sph_radial(const SphPair* __restrict
 // is real, so I^l_{-m1,-m2} = (-1)^{m1+m2} conj(I^l_{m1m2}) and the m1 < 0
 // rows are stored from the computed ones.
 // ---------------------------------------------------------------------------
-constexpr int kMB = 32;   // m1 rows per workgroup
+// m1 rows per workgroup: 16 for the column-tile class of degrees <= 15
+// (at most 16 rows), 32 above
+template <int NCT>
+constexpr int contract_rows() {
+  return NCT <= 2 ? 16 : 32;
+}
 
 // ---------------------------------------------------------------------------
 // MFMA form of the contraction (v_mfma_f64_16x16x4f64).  One workgroup per
-// (pair, l, block of kMB rows m1 >= 0).  Per species group and tile of kCT
+// (pair, l, block of MB rows m1 >= 0).  Per species group and tile of kCT
 // atoms k of structure 2:
 //   stage Y2[m][k-tile] (built from P2, E2) in LDS
 //   A[rr][k] = sum_j Y1'[rr][j] K_l[j][k]: real rows rr = 2r + {Re, Im} of
@@ -377,15 +382,15 @@ constexpr int kJC = 16;  // atoms of structure 1 per staged chunk
 // LDS of one contraction workgroup; NCT = 16-wide column tiles of Y2.  The
 // chunk buffers of the first product and the A / Y2 tiles of the second are
 // never live together.
-template <int NCT>
+template <int NCT, int MB = contract_rows<NCT>()>
 struct ContractLds {
   union {
     struct {
-      double y1[2][2 * kMB][kJC + 1];  // [buffer][real row][j]
+      double y1[2][2 * MB][kJC + 1];   // [buffer][real row][j]
       double kc[2][kJC][kCT + 1];      // [buffer][j][k]
     } a;
     struct {
-      double2 ac[kMB][kCT + 1];        // A tile [row][k]
+      double2 ac[MB][kCT + 1];         // A tile [row][k]
       double2 ys[kCT][16 * NCT + 1];   // Y2 tile [k][m]
     } b;
   } u;
@@ -397,11 +402,12 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
                                const double* __restrict__ R, double sc,
                                double2* __restrict__ I64, float2* __restrict__ I32, int n,
                                ContractLds<NCT>& S) {
+  constexpr int MB = contract_rows<NCT>();  // m1 rows per workgroup
   constexpr int MP = 16 * NCT;            // padded columns of Y2
   constexpr int NY = kCT * MP / 256;      // staged Y2 entries per thread
   const int M = 2 * l + 1;
-  const int m1a = b * kMB;
-  const int rows = min(kMB, l + 1 - m1a);
+  const int m1a = b * MB;
+  const int rows = min(MB, l + 1 - m1a);
   const int nrt = (rows + 15) >> 4;       // complex row tiles of C
   const int nrtA = (2 * rows + 15) >> 4;  // real row tiles of A
   const int nct = (M + 15) >> 4;          // column tiles of C
@@ -429,7 +435,7 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
     //   K chunk:   k = k0 + (t & 31), j = j0 + (t >> 5) + 8 i, i < kNK
     //   Y2 tile:   k = k0 + (t & 31), m = (t >> 5) + 8 i, i < NY
     constexpr int kYR = 256 / kJC;          // complex rows per Y1 staging round
-    constexpr int kNY1 = kMB * kJC / 256;   // Y1 entries staged per thread
+    constexpr int kNY1 = MB * kJC / 256;    // Y1 entries staged per thread
     constexpr int kNK = kJC / 8;            // K entries staged per thread
     const int yj = t % kJC, yr0 = t / kJC;
     const int kk_t = t & 31, kj0 = t >> 5;
@@ -575,7 +581,7 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
   // C tiles: real parts in waves 0, 2, imaginary parts in waves 1, 3;
   // exchange through LDS so one lane writes both parts of an entry
   __syncthreads();
-  double2 (*cb)[kCT + 1] = S.u.b.ac;  // reuse as [32][33] staging per column tile pass
+  double2 (*cb)[kCT + 1] = S.u.b.ac;  // reuse as [MB][kCT + 1] staging per column tile pass
   const double norm = 4.0 * pow(kPi, 2.5) * sc * sc * sc;
   for (int ct = 0; ct < nct; ++ct) {
     // tiles (rt, ct) for rt < nrt: ctile = rt * nct + ct
@@ -592,7 +598,7 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
       }
     }
     __syncthreads();
-    for (int e = t; e < kMB * 16; e += 256) {
+    for (int e = t; e < MB * 16; e += 256) {
       const int r = e >> 4, mc = e & 15, m = 16 * ct + mc;
       if (r < rows && m < M) {
         const int m1 = m1a + r, m2 = m - l;
@@ -615,7 +621,7 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
 // NCTMAX: column tiles of the largest degree in the launch (2l+1 <= 16 NCTMAX);
 // the launch covers the degrees l_lo..l_hi of every pair.
 template <int NCTMAX>
-__global__ void __launch_bounds__(256, NCTMAX <= 2 ? 4 : 3) k_sph_contract_mfma(
+__global__ void __launch_bounds__(256, NCTMAX <= 2 ? 5 : 3) k_sph_contract_mfma(
     const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
     const double2* __restrict__ Y1, const double2* __restrict__ Y2,
     const double* __restrict__ R, const double* __restrict__ scale_eff,
@@ -628,7 +634,8 @@ __global__ void __launch_bounds__(256, NCTMAX <= 2 ? 4 : 3) k_sph_contract_mfma(
   const int L = min(d.L, l_hi);
   int l = l_lo;
   for (; l <= L; ++l) {
-    const int nb = (l + 1 + kMB - 1) / kMB;
+    constexpr int MB = contract_rows<NCTMAX>();
+    const int nb = (l + 1 + MB - 1) / MB;
     if (b < nb) break;
     b -= nb;
   }
@@ -2161,12 +2168,13 @@ void launch_sph_contract(hipStream_t s, int P, int n, int Lmax, const SphPair* p
                          const SphGroup* grp, const double2* Y1, const double2* Y2,
                          const double* R, const double* scale_eff, double2* I64, float2* I32) {
   // one launch per column-tile class, so the low degrees run with the
-  // smaller LDS tile (four workgroups per CU instead of three)
+  // smaller tiles (16 rows: five workgroups per CU instead of three)
   auto launch = [&](int K, int lo, int hi) {
     hi = std::min(hi, Lmax);
     if (lo > hi) return;
     int nb = 0;
-    for (int l = lo; l <= hi; ++l) nb += (l + 1 + kMB - 1) / kMB;
+    const int MB = K <= 2 ? contract_rows<2>() : contract_rows<4>();
+    for (int l = lo; l <= hi; ++l) nb += (l + 1 + MB - 1) / MB;
     const dim3 g(xcd_grid(P, nb)), b(256);
     if (K == 2)
       hipLaunchKernelGGL(k_sph_contract_mfma<2>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,
