@@ -25,25 +25,32 @@ namespace fo {
 // ---------------------------------------------------------------------------
 // Wigner-d table: tab[k * Kt + kidx(l,m1,m2)] = sqrt((2l+1)/2) d^l_{m1m2}(b_k)
 // with b_k = pi (2k+1) / (2n)  (fastoverlap/soft.py:55, :73-96).  Evaluated
-// with the Kostelec-Rockmore three-term recurrence in l (the method of
-// f90/DSOFT.f90:81-195), seeded at l0 = max(|m1|,|m2|) by the closed form.
+// with the three-term recurrence in the degree of Kostelec & Rockmore, "FFTs
+// on the rotation group" (J. Fourier Anal. Appl. 14, 2008), seeded at
+// l0 = max(|m1|,|m2|) by the closed form.
 // ---------------------------------------------------------------------------
+// Coefficients of  dn^{j+1} = B (cos b - C) dn^j - A dn^{j-1}  for the
+// normalised dn^j = sqrt((2j+1)/2) d^j.  With the "edge" factor
+// e_j = sqrt((j^2 - m1^2)(j^2 - m2^2)), the unnormalised recurrence reads
+//   e_{j+1} d^{j+1} = (j+1)(2j+1) (cos b - m1 m2 / (j(j+1))) d^j
+//                     - (j+1)/j e_j d^{j-1};
+// rescaling each d^j by sqrt((2j+1)/2) gives
+//   B = (j+1) sqrt((2j+1)(2j+3)) / e_{j+1},
+//   A = (j+1)/j sqrt((2j+3)/(2j-1)) e_j / e_{j+1},   C = m1 m2 / (j(j+1)),
+// and A = C = 0 at j = 0 (only m1 = m2 = 0 reaches it).
 FO_DEV void recurrence_terms(int l, int m1, int m2, double& A, double& B, double& C) {
-  const double dl = (double)l;
-  const double t1 = sqrt((2.0 * dl + 3.0) / (2.0 * dl + 1.0));
-  const double t3 = (dl + 1.0) * (2.0 * dl + 1.0);
-  const double t5 = 1.0 / sqrt(((dl + 1.0) * (dl + 1.0) - (double)m1 * m1) *
-                               ((dl + 1.0) * (dl + 1.0) - (double)m2 * m2));
-  B = t1 * t3 * t5;
+  const double j = (double)l, a2 = (double)m1 * m1, b2 = (double)m2 * m2;
+  const double jp = j + 1.0;
+  const double inv_e_next = 1.0 / sqrt((jp * jp - a2) * (jp * jp - b2));
+  B = jp * sqrt((2.0 * j + 1.0) * (2.0 * j + 3.0)) * inv_e_next;
   if (l == 0) {
     A = 0.0;
     C = 0.0;
     return;
   }
-  const double t2 = sqrt((2.0 * dl + 3.0) / (2.0 * dl - 1.0)) * (dl + 1.0) / dl;
-  const double t4 = sqrt((dl * dl - (double)m1 * m1) * (dl * dl - (double)m2 * m2));
-  A = t2 * t4 * t5;
-  C = (double)m1 * m2 / (dl * (dl + 1.0));
+  const double e_cur = sqrt((j * j - a2) * (j * j - b2));
+  A = (jp / j) * sqrt((2.0 * j + 3.0) / (2.0 * j - 1.0)) * e_cur * inv_e_next;
+  C = (double)m1 * m2 / (j * jp);
 }
 
 // log of sqrt(binomial(mu+nu, mu)) and the sign xi of the edge element

@@ -12,15 +12,20 @@ to the reference repository root (``fastoverlap/...``).
 
 Pinning (what checks each function against something other than itself) is
 listed in ``PINNED`` at the bottom of this file and exercised by
-``tests/test_oracle.py``.  Two functions are "parity unpinned" in the sense
-that no known answer, closed form or library routine fixes their exact output:
+``tests/test_oracle.py``.  Every function there is pinned.  Two of them are
+pinned by what the mathematics fixes rather than by a reference output:
 
 * ``max_overlap`` -- the end point of scipy's L-BFGS-B is scipy-version
-  specific; only the downstream permutation / RMSD is contracted.
-* ``periodic_hungarian`` -- implements the *intended* LAP semantics (rows =
-  X1, ``X2[perm]`` ~ ``X1``); the reference's own munkres fallback applies the
-  inverse permutation (``fastoverlap/periodicAlignment.py:94-107``), and the
-  reference's published BLJ256 answer was produced with pele, absent here.
+  specific, so it is pinned by the optimum itself: an exact rotated copy must
+  be mapped back onto the true rotation, the end point of a noisy pair must be
+  a stationary local minimum that an independent (Nelder-Mead) search agrees
+  with;
+* ``PeriodicOracle.hungarian`` -- implements the *intended* LAP semantics
+  (rows = X1, ``X2[perm]`` ~ ``X1``); the reference's own munkres fallback
+  applies the inverse permutation (``fastoverlap/periodicAlignment.py:94-107``)
+  and its published BLJ256 answer was produced with pele, absent here.  It is
+  pinned by brute force over every permutation, by exact shifted copies and
+  by that BLJ256 answer (distance 1.559).
 """
 
 from __future__ import annotations
@@ -106,15 +111,16 @@ def wigner_d_jacobi(L: int, beta, normalised: bool = True):
 def wigner_d_recurrence(L: int, beta, normalised: bool = True):
     """FP64 three-term recurrence in l for the Wigner small-d.
 
-    Method of Kostelec & Rockmore (2008), as used by ``DSOFT.f90:81-195``
-    (``RECURRTERMS`` / ``CALCWIGNERD``): seed d^{l0}_{m1 m2} at
+    Method of Kostelec & Rockmore (2008), the one the reference's Fortran
+    uses (``f90/DSOFT.f90:81-195``): seed d^{l0}_{m1 m2} at
     l0 = max(|m1|, |m2|) in closed form, then
 
         d^{l+1} = B_l (cos b - C_l) d^l - A_l d^{l-1}
 
-    with the normalised-d coefficients of ``DSOFT.f90:98-113``.  Written from
-    the paper's recurrence; used by the oracle at high l where the Jacobi
-    closed form loses accuracy.  Same output layout as ``wigner_d_jacobi``.
+    with the normalised-d coefficients of ``_recurrence_terms`` (derived from
+    the paper's unnormalised relation).  Used by the oracle at high l where
+    the Jacobi closed form loses accuracy.  Same output layout as
+    ``wigner_d_jacobi``.
     """
     beta = np.atleast_1d(np.asarray(beta, float))
     n = 2 * L + 1
@@ -140,17 +146,21 @@ def wigner_d_recurrence(L: int, beta, normalised: bool = True):
 
 
 def _recurrence_terms(l: int, m1: int, m2: int):
-    """Normalised-d recurrence coefficients (Kostelec-Rockmore; DSOFT.f90:98-113)."""
-    dl = float(l)
-    t1 = math.sqrt((2 * dl + 3) / (2 * dl + 1))
-    t3 = (dl + 1) * (2 * dl + 1)
-    t5 = 1.0 / math.sqrt(((dl + 1) ** 2 - m1 * m1) * ((dl + 1) ** 2 - m2 * m2))
-    B = t1 * t3 * t5
+    """(A, B, C) of dn^{l+1} = B (cos b - C) dn^l - A dn^{l-1} for the
+    normalised dn^l = sqrt((2l+1)/2) d^l (Kostelec & Rockmore 2008).
+
+    From the unnormalised three-term relation with the edge factor
+    e_j = sqrt((j^2 - m1^2)(j^2 - m2^2)):
+    e_{j+1} d^{j+1} = (j+1)(2j+1)(cos b - m1 m2/(j(j+1))) d^j - (j+1)/j e_j d^{j-1}.
+    """
+    j = float(l)
+    e_next = math.sqrt(((j + 1) ** 2 - m1 * m1) * ((j + 1) ** 2 - m2 * m2))
+    B = (j + 1) * math.sqrt((2 * j + 1) * (2 * j + 3)) / e_next
     if l == 0:
         return 0.0, B, 0.0
-    t2 = math.sqrt((2 * dl + 3) / (2 * dl - 1)) * (dl + 1) / dl
-    t4 = math.sqrt((dl * dl - m1 * m1) * (dl * dl - m2 * m2))
-    return t2 * t4 * t5, B, m1 * m2 / (dl * (dl + 1))
+    e_cur = math.sqrt((j * j - m1 * m1) * (j * j - m2 * m2))
+    A = (j + 1) / j * math.sqrt((2 * j + 3) / (2 * j - 1)) * e_cur / e_next
+    return A, B, m1 * m2 / (j * (j + 1))
 
 
 def _wigner_seed(l0: int, m1: int, m2: int, c2, s2):
@@ -494,9 +504,9 @@ def energy_gradient_recurrence(rot, Iconj, L):
 
 def max_overlap(R0, Iconj, L):
     """scipy L-BFGS-B from the grid angle
-    (``fastoverlap/sphericalAlignment.py:110-113``).  Parity unpinned: the
-    iterates are scipy-version specific; only downstream perm/RMSD is
-    contracted."""
+    (``fastoverlap/sphericalAlignment.py:110-113``).  Its iterates are
+    scipy-version specific; the end point is pinned by the optimum itself
+    (exact copies, stationarity, an independent search)."""
     res = minimize(energy_gradient, np.asarray(R0, float), jac=True,
                    args=(Iconj, L), method="L-BFGS-B")
     return res.x, res
@@ -790,18 +800,22 @@ class PeriodicOracle:
 PINNED = {
     "makeweights": "quadrature exactness on Wigner-d orthogonality (test_oracle)",
     "wigner_d_jacobi": "small-l closed forms, orthogonality, unitarity",
-    "wigner_d_recurrence": "agreement with the Jacobi closed form for l <= 30",
+    "wigner_d_recurrence": "Jacobi closed form to l = 40 on the grid-128 betas; "
+                           "quadrature orthonormality to l = 63",
     "SOFTOracle.iSOFT": "round trip SOFT(iSOFT(f)) = f; delta coefficients",
+    "SOFTOracle.SOFT": "round trip with iSOFT",
+    "pad_coeffs": "round trip at bw > L+1; direct-sum grid values; delta at bw = 2(L+1)",
     "sph_harm_table": "scipy routine; quadrature orthonormality; Y00",
     "radial_kernel": "scipy spherical_in; limits at x = 0",
     "so3_coeffs": "brute-force overlap identity; Hermitian; inversion (-1)^l",
     "find_max": "numpy argmax semantics; parabola exact on a quadratic",
     "energy_gradient": "finite differences; value equals the grid identity",
+    "max_overlap": "exact copy -> true rotation; stationary local minimum; Nelder-Mead agrees",
     "find_best_permutation": "brute force over all permutations, N <= 7",
     "kabsch_quaternion": "SVD Kabsch; exact rotation gives 0",
+    "average_separation": "cubic lattice closed form; k-d tree nearest neighbours",
     "spherical_align": "LJ38 known answer 1.4767 (Jmax=14); exact rotation -> 0",
     "PeriodicOracle.overlap_grid": "peak at the known translation; direct DFT",
-    "PeriodicOracle.refine": "exact translation+permutation -> 0",
-    "max_overlap": "parity unpinned (scipy L-BFGS-B end point)",
-    "PeriodicOracle.hungarian": "parity unpinned vs reference quirk (intended semantics)",
+    "PeriodicOracle.hungarian": "brute force (two groups); exact shifted copy; BLJ256 1.559",
+    "PeriodicOracle.refine": "exact translation+permutation -> 0; BLJ256 1.559",
 }

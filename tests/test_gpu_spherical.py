@@ -2,9 +2,11 @@
 
 Contract: identical integer argmax per orientation, identical permutation,
 RMSD-defining distance within 1e-8 of the oracle, FP32 overlap grid within
-1e-5 of the grid maximum.  Every argmax comparison is made only where the
-oracle's own FP64 top-two margin exceeds 1e-10 of the maximum (a tie that
-FP64 itself cannot resolve); such cases are counted and must stay rare.
+1e-5 of the grid maximum.  A different argmax is accepted only where FP64
+itself cannot order the two points: the oracle's grid value at the GPU's
+index must lie within 1e-10 of the oracle's maximum.  Such ties are judged
+by value (never skipped) and counted in the 256-pair fixture test, where
+they must stay below 2 % of the grids.
 """
 
 import sys
@@ -26,11 +28,6 @@ GRID_TOL = 1e-5
 def say(*a):
     print(*a, flush=True)
     sys.stdout.flush()
-
-
-def oracle_margin(grid):
-    g = np.sort(grid.real.ravel())[::-1]
-    return (g[0] - g[1]) / abs(g[0])
 
 
 def test_wigner_table_matches_oracle(gpu):
@@ -82,10 +79,16 @@ def _oracle_pair(c, keep_grids=False):
 
 
 def _check_pair(tag, o, r_dist, r_perm, r_inv, ind, grids=None):
+    """Identical argmax per orientation, except at an FP64 tie: then the
+    oracle's grid value at the GPU's index must be within 1e-10 of the
+    oracle's maximum (checked by value, never skipped)."""
     for k in range(2):
-        m = oracle_margin(o.grids[k]) if o.grids else 1.0
-        if m > 1e-10:
-            assert tuple(ind[k]) == o.ind[k], (tag, k, ind[k], o.ind[k])
+        if tuple(ind[k]) == o.ind[k]:
+            continue
+        assert o.grids, (tag, k, ind[k], o.ind[k])
+        g = o.grids[k].real
+        top = g[o.ind[k]]
+        assert top - g[tuple(ind[k])] <= 1e-10 * abs(top), (tag, k, ind[k], o.ind[k])
     assert bool(r_inv) == o.inverted, tag
     np.testing.assert_array_equal(r_perm, o.perm, err_msg=tag)
     assert abs(r_dist - o.dist) <= DIST_TOL, (tag, r_dist, o.dist)
@@ -241,8 +244,64 @@ def test_exact_rotation_gives_zero(gpu):
     assert dist < 1e-6
 
 
+def _golden_c2():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "C2_pairs.npz"))
+
+
+def test_c2_256_pairs_against_oracle_fixture(gpu):
+    """The first 256 pairs of the 4096-pair mixed workload (every Jmax in
+    15..25, both grid classes), run as the bench runs them (one batch per
+    grid class), against the FP64 oracle's stored answers
+    (tests/golden/make_c2_golden.py): identical permutation and orientation,
+    distance within 1e-8, and per orientation the identical integer argmax.
+    A different argmax is accepted only as an FP64 tie: the oracle's grid
+    value at the GPU's index must be within 1e-10 of the oracle's maximum.
+    Such ties are counted and must stay below 2 % of the grids."""
+    z = _golden_c2()
+    P = len(z["pair"])
+    pairs = [W.config2(int(p)) for p in z["pair"]]
+    chk = np.array([np.abs(c["pos1"]).sum() + 2.0 * np.abs(c["pos2"]).sum() for c in pairs])
+    np.testing.assert_allclose(chk, z["checksum"], rtol=1e-13,
+                               err_msg="workload generator diverged from the fixture")
+    poff = np.concatenate([[0], np.cumsum(z["natoms"])])
+    ties, checked = 0, 0
+    Ls = set()
+    for n in (32, 64):
+        idx = [i for i in range(P) if int(z["grid_n"][i]) == n]
+        sel = [pairs[i] for i in idx]
+        say(f"case: C2 fixture batch n={n} pairs={len(sel)}")
+        with SphericalBatch([len(c["pos1"]) for c in sel], [c["L"] for c in sel], 0.3, n,
+                            [c["groups"] for c in sel]) as b:
+            b.set_positions([c["pos1"] for c in sel], [c["pos2"] for c in sel])
+            b.run(graph=True)
+            r = b.results()
+            pk = b.peaks()
+        for j, i in enumerate(idx):
+            tag = f"C2 pair {int(z['pair'][i])} n={n} L={int(z['L'][i])}"
+            Ls.add(int(z["L"][i]))
+            for k in range(2):
+                checked += 1
+                got = tuple(int(v) for v in pk["ind"][j][k])
+                if got == tuple(z["ind"][i][k]):
+                    continue
+                flat = (got[0] * n + got[1]) * n + got[2]
+                where = np.nonzero(z["top_idx"][i][k] == flat)[0]
+                assert len(where), (tag, k, got, tuple(z["ind"][i][k]))
+                top = z["top_val"][i][k]
+                assert top[0] - top[where[0]] <= 1e-10 * abs(top[0]), (tag, k, got)
+                ties += 1
+            assert bool(r["inverted"][j]) == bool(z["inverted"][i]), tag
+            np.testing.assert_array_equal(r["perm"][j], z["perm"][poff[i]:poff[i + 1]],
+                                          err_msg=tag)
+            assert abs(r["dist"][j] - z["dist"][i]) <= DIST_TOL, (tag, r["dist"][j], z["dist"][i])
+    say(f"C2 fixture: {P} pairs, {checked} orientation grids, {ties} FP64 ties")
+    assert Ls == set(range(15, 26))
+    assert ties <= 0.02 * checked
+
+
 def test_c2_batch_parity_and_batch_independence(gpu):
-    pairs = [W.config2(p) for p in range(24)]
+    pairs = [W.config2(p) for p in range(300, 312)]
     for n in (32, 64):
         sel = [c for c in pairs if c["grid_n"] == n]
         if not sel:
@@ -271,29 +330,36 @@ def test_c2_batch_parity_and_batch_independence(gpu):
 
 
 def test_c3_large_jmax_against_fixture(gpu):
-    """1000 atoms (600:400), Jmax 40, grid 128, width 0.2: the FP64 Wigner-d
-    table at high l, the FP32 beta contraction over 41 degrees and the FP64
-    re-evaluation, against the oracle's stored answer
-    (tests/golden/make_c3_golden.py)."""
+    """Four 1000-atom pairs (600:400), Jmax 40, grid 128, width 0.2, in one
+    batch: the FP64 Wigner-d table at high l, the FP32 beta contraction over
+    41 degrees and the FP64 re-evaluation, against the oracle's stored answers
+    (tests/golden/make_c3_golden.py).  Besides argmax, orientation,
+    permutation and distance, the FP32 grid's error against the FP64 grid is
+    checked on each orientation's 16 largest values."""
     import os
-    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "C3_pair0.npz"))
-    sizes = z["group_sizes"]
-    groups = [np.arange(sizes[0]), np.arange(sizes[0], sizes[0] + sizes[1])]
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "C3_pairs.npz"))
+    P = len(z["pos1"])
     L, n, s = int(z["L"]), int(z["grid_n"]), float(z["scale"])
-    say(f"case: C3 N=1000 L={L} n={n} sigma={s}")
-    with SphericalBatch([1000], [L], [s], n, [groups], _lib.SPH_KEEP_GRID) as b:
-        b.set_positions(z["pos1"], z["pos2"])
+    groups = [[np.arange(sz[0]), np.arange(sz[0], sz[0] + sz[1])] for sz in z["group_sizes"]]
+    say(f"case: C3 {P} pairs N=1000 L={L} n={n} sigma={s}")
+    with SphericalBatch([1000] * P, [L] * P, [s] * P, n, groups, _lib.SPH_KEEP_GRID) as b:
+        b.set_positions(list(z["pos1"]), list(z["pos2"]))
         b.run()
         r = b.results()
         pk = b.peaks()
-        grids = b.grid(0)
-    for k in range(2):
-        assert tuple(pk["ind"][0][k]) == tuple(z["ind"][k]), (k, pk["ind"][0][k], z["ind"][k])
-        got = grids[k].ravel()[z["top_idx"][k]]
-        assert np.abs(got - z["top_val"][k]).max() <= GRID_TOL * z["gmax"][k], k
-    assert bool(r["inverted"][0]) == bool(z["inverted"])
-    np.testing.assert_array_equal(r["perm"][0], z["perm"])
-    assert abs(r["dist"][0] - float(z["dist"])) <= DIST_TOL, (r["dist"][0], float(z["dist"]))
+        grids = [b.grid(p) for p in range(P)]
+    worst = 0.0
+    for p in range(P):
+        for k in range(2):
+            assert tuple(pk["ind"][p][k]) == tuple(z["ind"][p][k]), (p, k, pk["ind"][p][k])
+            got = grids[p][k].ravel()[z["top_idx"][p][k]]
+            err = np.abs(got - z["top_val"][p][k]).max() / z["gmax"][p][k]
+            worst = max(worst, err)
+            assert err <= GRID_TOL, (p, k, err)
+        assert bool(r["inverted"][p]) == bool(z["inverted"][p]), p
+        np.testing.assert_array_equal(r["perm"][p], z["perm"][p])
+        assert abs(r["dist"][p] - float(z["dist"][p])) <= DIST_TOL, (p, r["dist"][p])
+    say(f"C3: FP32 grid error on the top values <= {worst:.2e} of the grid maximum")
 
 
 def test_graph_replay_matches_eager(gpu):
