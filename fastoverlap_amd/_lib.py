@@ -24,7 +24,7 @@ ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "unsuppo
 SPH_NO_INVERT, SPH_KEEP_GRID, SPH_NO_CENTER, SPH_GENERAL, SPH_DFT = 1, 2, 4, 8, 16
 STAGE_PREP, STAGE_COEFF, STAGE_GRID, STAGE_ROTOPT, STAGE_FINISH, STAGE_ALL = 1, 2, 4, 8, 16, 31
 # periodic flags / stages
-PER_KEEP_GRID, PER_INPUT_COEFFS, PER_KEEP_COEFFS = 2, 4, 8
+PER_KEEP_GRID, PER_INPUT_COEFFS, PER_KEEP_COEFFS, PER_KEEP_FFT = 2, 4, 8, 16
 PSTAGE_SF, PSTAGE_GRID, PSTAGE_REFINE, PSTAGE_ALL = 1, 2, 4, 7
 
 
@@ -95,6 +95,9 @@ _PROTOS = {
     "fo_sph_batch_grid": (C.c_int, [_vp, C.c_int, _pf]),
     "fo_sph_batch_norient": (C.c_int, [_vp]),
     "fo_sph_batch_timing": (C.c_int, [_vp, C.c_int, _pf]),
+    "fo_sph_prepare": (C.c_int, [_vp, C.c_int]),
+    "fo_sph_align_noalloc": (C.c_int, [_vp, C.POINTER(SphBatchDesc), _vp, _vp, _vp, _vp, _vp,
+                                       _vp, _vp]),
     "fo_sph_harm": (C.c_int, [_vp, C.c_int, C.c_int, _pd, _pd, _pd]),
     "fo_sph_energy": (C.c_int, [_vp, C.c_int, _pd, _pd, _pd, _pd]),
     "fo_sph_wigner_D": (C.c_int, [_vp, C.c_int, _pd, _pd, _pd]),

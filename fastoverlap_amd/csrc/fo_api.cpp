@@ -171,13 +171,18 @@ static void ctx_free(fo_context* c, void* p, size_t bytes) {
   c->used -= bytes;
 }
 
-static int get_sph_table(fo_context* c, int n, const SphTable** out) {
+// build = false: only look the table up (FO_ERR_STATE when it was never
+// built), so that a caller that must not allocate never reaches hipMalloc.
+static int get_sph_table(fo_context* c, int n, const SphTable** out, bool build = true) {
   std::lock_guard<std::mutex> g(c->mu);
   auto it = c->sph.find(n);
   if (it != c->sph.end()) {
     *out = &it->second;
     return FO_OK;
   }
+  if (!build)
+    return set_err(FO_ERR_STATE, "no Wigner table for grid side " + std::to_string(n) +
+                                     " in this context (fo_sph_prepare builds it)");
   SphTable t;
   t.n = n;
   t.Lt = std::min(n / 2 - 1, kMaxL);
@@ -211,8 +216,10 @@ static int get_sph_table(fo_context* c, int n, const SphTable** out) {
   return FO_OK;
 }
 
-static int get_rec_table(fo_context* c, const RecTable** out) {
+static int get_rec_table(fo_context* c, const RecTable** out, bool build = true) {
   std::lock_guard<std::mutex> g(c->mu);
+  if (!c->rect.rec && !build)
+    return set_err(FO_ERR_STATE, "recurrence tables not built (fo_sph_prepare builds them)");
   if (!c->rect.rec) {
     const size_t b1 = sizeof(double) * 3 * (size_t)kcount(kRecL);
     const size_t b2 = sizeof(double) * (size_t)(2 * kRecL + 1) * (2 * kRecL + 1);
@@ -304,8 +311,21 @@ int fo_context_create(int device, unsigned long long hbm_budget, fo_context** ou
     delete c;
     return set_err(FO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
+  // device address of this device's allocation-free scratch (once per device)
+  fo::noalloc_attach(device);
   *out = c;
   return FO_OK;
+}
+
+int fo_sph_prepare(fo_context* c, int grid_n) {
+  if (!c) return set_err(FO_ERR_INVALID, "ctx is NULL");
+  if (grid_n < 2 || grid_n > 256) return set_err(FO_ERR_UNSUPPORTED, "grid_n must be in [2, 256]");
+  FO_HIP(hipSetDevice(c->device));
+  const SphTable* t = nullptr;
+  const RecTable* r = nullptr;
+  int rc = get_sph_table(c, grid_n, &t);
+  if (rc) return rc;
+  return get_rec_table(c, &r);
 }
 
 int fo_context_destroy(fo_context* c) {
@@ -466,7 +486,8 @@ static size_t coef_chunk_bytes() {
   return SIZE_MAX;
 }
 
-int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out) {
+int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out,
+                   bool build_tables) {
   if (!c || !d || !out) return set_err(FO_ERR_INVALID, "NULL argument");
   if (d->npairs < 1) return set_err(FO_ERR_INVALID, "npairs must be >= 1");
   if (!d->natoms || !d->L || !d->ngroups)
@@ -568,9 +589,9 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   b->fft_chunk = (d->flags & FO_SPH_DFT) ? 0 : isoft_plane_use_fft(n, Lmax, c->max_lds);
   if (lap_lds_bytes(b->maxgroup) > c->max_lds)
     return set_err(FO_ERR_UNSUPPORTED, "permutation group too large for the LDS assignment solver");
-  int rc = get_sph_table(c, n, &b->tab);
+  int rc = get_sph_table(c, n, &b->tab, build_tables);
   if (rc) return rc;
-  rc = get_rec_table(c, &b->rect);
+  rc = get_rec_table(c, &b->rect, build_tables);
   if (rc) return rc;
   if (Lmax > b->tab->Lt) return set_err(FO_ERR_UNSUPPORTED, "Jmax exceeds the table degree");
   SphLaunch& lc = b->lc;
@@ -626,6 +647,14 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
 }
 
 size_t sph_batch_bytes(const fo_sph_batch* b) { return b->bytes; }
+
+void sph_batch_outputs(const fo_sph_batch* b, SphOutputs* o) {
+  o->res = b->res;
+  o->X2 = b->X2out;
+  o->perm = b->perm;
+  o->natoms = b->Ntot;
+  o->npairs = b->lc.P;
+}
 
 int sph_batch_bind(fo_sph_batch* b, void* mem, hipStream_t stream) {
   fo_context* c = b->ctx;
@@ -1317,6 +1346,7 @@ int per_batch_bind(fo_per_batch* b, void* mem, hipStream_t stream) {
 int ctx_alloc_bytes(fo_context* c, size_t bytes, void** p) { return ctx_alloc(c, bytes, p); }
 void ctx_free_bytes(fo_context* c, void* p, size_t bytes) { ctx_free(c, p, bytes); }
 int ctx_device(const fo_context* c) { return c->device; }
+hipStream_t ctx_stream(const fo_context* c) { return c->stream; }
 int set_error(int code, const char* msg) { return set_err(code, msg); }
 
 }  // namespace fo

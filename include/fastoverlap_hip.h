@@ -136,6 +136,29 @@ int fo_sph_batch_norient(const fo_sph_batch* b);
  * select, rotopt, finish]; requires timing enabled */
 int fo_sph_batch_timing(fo_sph_batch* b, int enable, float* ms8);
 
+/* Build and keep resident the tables of grid side grid_n (Wigner-d table,
+ * twiddles, recurrence coefficients) -- the tables fastoverlap/soft.py:48-96
+ * builds per SOFT instance.  Allocates; a set-up call for
+ * fo_sph_align_noalloc (every other entry point builds tables on demand). */
+int fo_sph_prepare(fo_context* ctx, int grid_n);
+
+/* Allocation-free alignment (no counterpart in the reference): the same
+ * pipeline as fo_sph_batch_create + _set_positions_device + _run +
+ * _results for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * calls no hipMalloc/hipFree: every intermediate lives in a fixed 64 MiB
+ * device array per GPU, held until the process ends; a batch needing more
+ * returns FO_ERR_NOMEM.  The grid's tables must be resident
+ * (fo_sph_prepare, else FO_ERR_STATE).  flags: FO_SPH_NO_INVERT,
+ * FO_SPH_NO_CENTER, FO_SPH_DFT.  Inputs d_pos1/d_pos2 and the outputs
+ * (d_dist[npairs], d_X2[3N], d_perm[N], d_rmat[9*npairs],
+ * d_inverted[npairs]; any output may be NULL) are DEVICE pointers.
+ * Synchronous; calls on one GPU share its array and are serialised inside
+ * the process (overlapping use of one GPU's array would race). */
+int fo_sph_align_noalloc(fo_context* ctx, const fo_sph_batch_desc* desc, const double* d_pos1,
+                         const double* d_pos2, double* d_dist, double* d_X2, int* d_perm,
+                         double* d_rmat, int* d_inverted);
+
 /* ---- single-call spherical helpers (drop-in SphericalAlign) ------------ */
 /* sphHarm: Y[l, m mod (2L+1), j], complex interleaved, (L+1)*(2L+1)*npts */
 int fo_sph_harm(fo_context* ctx, int L, int npts, const double* theta, const double* phi,
