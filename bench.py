@@ -76,7 +76,7 @@ def measured_traffic(kernel, npairs):
     return k["hbm_read_bytes_per_step"] + k["hbm_write_bytes_per_step"]
 
 
-def make_batches(pairs, ctx, streams=1):
+def make_batches(pairs, ctx, streams=1, flags=0):
     """One batch per grid class, each split into `streams` sub-batches with
     their own HIP streams so that their kernels overlap on the device."""
     from fastoverlap_amd.batch import SphericalBatch
@@ -87,7 +87,8 @@ def make_batches(pairs, ctx, streams=1):
         for part in range(k):
             sel = sel_all[part::k]
             b = SphericalBatch([len(c["pos1"]) for c in sel], [c["L"] for c in sel],
-                               [c["scale"] for c in sel], n, [c["groups"] for c in sel], 0, ctx)
+                               [c["scale"] for c in sel], n, [c["groups"] for c in sel], flags,
+                               ctx)
             out.append((n, sel, b))
     return out
 
@@ -158,9 +159,13 @@ def oracle_parity(sample, answers, bres):
             "max_abs_dist_err": worst}
 
 
+SERVE_BUDGET = 32 << 30   # HBM admission budget of the served stream (bytes)
+
+
 def _serve_stream(ctx, reqs, gaps):
     from fastoverlap_amd.serve import Server
-    with Server(ctx, max_batch=256, streams=3, max_wait_ms=2.0) as srv:
+    with Server(ctx, max_batch=256, streams=3, max_wait_ms=2.0,
+                hbm_budget=SERVE_BUDGET) as srv:
         srv.align(reqs[0])  # warm the tables / code paths of both kinds
         srv.align(reqs[-1])
         ids = []
@@ -179,7 +184,8 @@ def _serve_stream(ctx, reqs, gaps):
     return {"pairs_per_s": ok / wall, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)),
             "mean_batch": float(np.mean([r["batch_size"] for r in res])),
-            "failed": len(reqs) - ok, "batches": stats["batches"]}
+            "failed": len(reqs) - ok, "batches": stats["batches"],
+            "hbm_budget_bytes": SERVE_BUDGET, "peak_arena_bytes": stats["peak_bytes"]}
 
 
 def served(ctx, reqs, seed=7):
@@ -285,6 +291,8 @@ def main():
     ap.add_argument("--periodic-pairs", type=int, default=128,
                     help="C1 pairs of the periodic supplement (0: skip it)")
     ap.add_argument("--no-periodic", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the default launch shapes instead of the autotuned ones")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -306,7 +314,8 @@ def main():
         f"{len(serve_reqs)} served requests in "
         f"{time.perf_counter() - t:.1f}s")
     ctx = _lib.default_context()
-    batches = make_batches(pairs, ctx, args.streams)
+    batches = make_batches(pairs, ctx, args.streams,
+                           0 if args.no_autotune else _lib.SPH_AUTOTUNE)
     for n, sel, b in batches:
         b.set_positions([c["pos1"] for c in sel], [c["pos2"] for c in sel])
         log(f"grid {n}: {len(sel)} pairs, {b.footprint() / 2**20:.0f} MiB on device")
@@ -326,6 +335,24 @@ def main():
     ms_per_step = elapsed / args.steps * 1e3
     value = len(pairs) * args.steps / elapsed
     log(f"batched: {ms_per_step:.2f} ms/step, {value:.0f} pairs/s")
+
+    # the autotuner's choice per grid class, and the default shapes timed on
+    # the same batches for comparison
+    tuning = {"autotune": not args.no_autotune,
+              "shapes": {str(n): b.tuning() for n, _, b in batches}}
+    if not args.no_autotune:
+        chosen = [b.tuning() for _, _, b in batches]
+        for _, _, b in batches:
+            b.set_tuning((16, 15, 0))
+        step()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        tuning["ms_per_step_default_shapes"] = (time.perf_counter() - t0) / args.steps * 1e3
+        for (_, _, b), sh in zip(batches, chosen):
+            b.set_tuning(sh)
+        step()
+        log(f"autotune: {tuning}")
 
     # per-kernel times: HIP events recorded on each batch's stream between the
     # stages, averaged over `steps` eager passes of the same work
@@ -426,6 +453,7 @@ def main():
         "speedup_vs_plain": value / plain,
         "plain_identical_argmax_perm": f"{agree}/{len(sample)}",
         "roofline": roof,
+        "tuning": tuning,
     }
     if per_pairs:
         out["periodic"] = periodic_batched(ctx, per_pairs, args.steps, args.warmup,

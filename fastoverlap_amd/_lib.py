@@ -21,7 +21,8 @@ ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "unsuppo
           5: "state", 6: "not found", 7: "cancelled"}
 
 # spherical flags / stages
-SPH_NO_INVERT, SPH_KEEP_GRID, SPH_NO_CENTER, SPH_GENERAL, SPH_DFT = 1, 2, 4, 8, 16
+SPH_NO_INVERT, SPH_KEEP_GRID, SPH_NO_CENTER, SPH_GENERAL, SPH_DFT, SPH_AUTOTUNE = \
+    1, 2, 4, 8, 16, 32
 STAGE_PREP, STAGE_COEFF, STAGE_GRID, STAGE_ROTOPT, STAGE_FINISH, STAGE_ALL = 1, 2, 4, 8, 16, 31
 # periodic flags / stages
 PER_KEEP_GRID, PER_INPUT_COEFFS, PER_KEEP_COEFFS, PER_KEEP_FFT = 2, 4, 8, 16
@@ -98,10 +99,14 @@ _PROTOS = {
     "fo_sph_prepare": (C.c_int, [_vp, C.c_int]),
     "fo_sph_align_noalloc": (C.c_int, [_vp, C.POINTER(SphBatchDesc), _vp, _vp, _vp, _vp, _vp,
                                        _vp, _vp]),
+    "fo_sph_batch_find_peaks": (C.c_int, [_vp, C.c_int, C.c_int, _pd, _pi, _pf]),
+    "fo_sph_batch_start": (C.c_int, [_vp, _pd]),
+    "fo_sph_batch_tuning": (C.c_int, [_vp, _pi, C.c_int]),
     "fo_sph_harm": (C.c_int, [_vp, C.c_int, C.c_int, _pd, _pd, _pd]),
     "fo_sph_energy": (C.c_int, [_vp, C.c_int, _pd, _pd, _pd, _pd]),
     "fo_sph_wigner_D": (C.c_int, [_vp, C.c_int, _pd, _pd, _pd]),
     "fo_sph_wigner_table": (C.c_int, [_vp, C.c_int, _pd]),
+    "fo_soft_forward": (C.c_int, [_vp, C.c_int, _pd, _pd]),
     "fo_lap_align": (C.c_int, [_vp, C.c_int, _pd, _pd, C.c_int, _pi, _pi, C.c_int, _pd, _pi,
                                _pd]),
     "fo_per_batch_create": (C.c_int, [_vp, C.POINTER(PerBatchDesc), C.POINTER(_vp)]),
@@ -119,6 +124,9 @@ _PROTOS = {
     "fo_per_batch_product": (C.c_int, [_vp, _pd]),
     "fo_per_batch_grid": (C.c_int, [_vp, C.c_int, _pf]),
     "fo_per_batch_timing": (C.c_int, [_vp, C.c_int, _pf]),
+    "fo_per_batch_fft": (C.c_int, [_vp, C.c_int, _pf]),
+    "fo_per_batch_find_peaks": (C.c_int, [_vp, C.c_int, C.c_int, _pd, _pi, _pf]),
+    "fo_per_batch_set_coeffs_indexed": (C.c_int, [_vp, _pd, C.c_int, _pi, _pi]),
     "fo_server_create": (C.c_int, [_vp, C.POINTER(ServerConfig), C.POINTER(_vp)]),
     "fo_server_destroy": (C.c_int, [_vp]),
     "fo_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(C.c_longlong)]),

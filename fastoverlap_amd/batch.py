@@ -192,6 +192,24 @@ class SphericalBatch:
                     margin=mar.reshape(sh), delta=dl.reshape(sh), ncand=nc.reshape(sh),
                     noverflow=no.reshape(sh))
 
+    def tuning(self):
+        """Launch shapes in use: (plane pairs per XCD chunk, highest degree
+        contracted with 16-row tiles, radial workgroups per pair)."""
+        v = np.zeros(3, np.int32)
+        call("fo_sph_batch_tuning", self.handle, iptr(v), 0)
+        return tuple(int(x) for x in v)
+
+    def set_tuning(self, params):
+        v = as_i32(params)
+        call("fo_sph_batch_tuning", self.handle, iptr(v), 1)
+
+    def start(self):
+        """Interpolated grid maximum per (pair, orientation) as Euler angles
+        (the refinement's start, fastoverlap/soft.py:127-130)."""
+        R0 = np.zeros((self.P, self.norient, 3))
+        call("fo_sph_batch_start", self.handle, dptr(R0))
+        return R0
+
     def opt(self):
         m = self.P * self.norient
         R = np.zeros((m, 3))
@@ -218,6 +236,20 @@ class SphericalBatch:
         g = np.zeros((self.norient, n, n, n), np.float32)
         call("fo_sph_batch_grid", self.handle, pair, fptr(g))
         return g
+
+    def find_peaks(self, npeaks=10, width=2, residual=False):
+        """Subtractive Gaussian peak search on every kept grid (needs
+        SPH_KEEP_GRID and a run of STAGE_GRID).  Returns fit (P, norient,
+        npeaks, 11): A, mu, the six s_ij, the centre as fractional grid
+        indices; count (P, norient) of successful fits; optionally the
+        residual grids (P, norient, n, n, n)."""
+        P, no, n = self.P, self.norient, self.grid_n
+        fit = np.zeros((P, no, npeaks, 11))
+        cnt = np.zeros((P, no), np.int32)
+        res = np.zeros((P, no, n, n, n), np.float32) if residual else None
+        call("fo_sph_batch_find_peaks", self.handle, int(npeaks), int(width), dptr(fit),
+             iptr(cnt), fptr(res))
+        return fit, cnt, res
 
     def footprint(self) -> int:
         b = C.c_ulonglong()
@@ -415,6 +447,36 @@ class PeriodicBatch:
         g = np.zeros((F, F, F), np.float32)
         call("fo_per_batch_grid", self.handle, pair, fptr(g))
         return g
+
+    def fft(self, pair: int):
+        """Complex f = fftn(C, (F, F, F)) of one pair (needs PER_KEEP_FFT)."""
+        F = self.F
+        g = np.zeros((F, F, F, 2), np.float32)
+        call("fo_per_batch_fft", self.handle, pair, fptr(g))
+        return g[..., 0].astype(np.float64) + 1j * g[..., 1].astype(np.float64)
+
+    def find_peaks(self, npeaks=10, width=2, residual=False):
+        """Subtractive Gaussian peak search on every kept |f| grid (needs
+        PER_KEEP_GRID and a run of PSTAGE_GRID): fit (P, npeaks, 11), count
+        (P,), optionally the residual grids (P, F, F, F)."""
+        P, F = self.P, self.F
+        fit = np.zeros((P, npeaks, 11))
+        cnt = np.zeros(P, np.int32)
+        res = np.zeros((P, F, F, F), np.float32) if residual else None
+        call("fo_per_batch_find_peaks", self.handle, int(npeaks), int(width), dptr(fit),
+             iptr(cnt), fptr(res))
+        return fit, cnt, res
+
+    def set_coeffs_indexed(self, Cs, idx1, idx2):
+        """Per-structure coefficients Cs (nstruct, ngroups, 2nk+1, 2nk+1,
+        2nk+1) shared by the pairs: pair p uses structure idx1[p] as its first
+        and idx2[p] as its second (PER_INPUT_COEFFS; every pair has the same
+        groups)."""
+        Cs = np.ascontiguousarray(Cs, np.complex128)
+        i1 = as_i32(idx1)
+        i2 = as_i32(idx2)
+        call("fo_per_batch_set_coeffs_indexed", self.handle, Cs.ctypes.data_as(_lib._pd),
+             int(Cs.shape[0]), iptr(i1), iptr(i2))
 
     def footprint(self) -> int:
         b = C.c_ulonglong()

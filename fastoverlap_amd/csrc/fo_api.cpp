@@ -8,6 +8,7 @@
 #include "fastoverlap_hip.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -15,6 +16,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "fo_common.h"
@@ -31,15 +33,15 @@ void launch_sph_prep(hipStream_t, int, const SphPair*, const double*, const doub
 void launch_sph_ylm(hipStream_t, int, const SphPair*, const int*, const double*, const double*,
                     const double2*, const double*, double2*, double2*);
 void launch_sph_radial(hipStream_t, int, int, long long, const SphPair*, const SphGroup*, const int*,
-                       const double*, const double*, const double*, double*);
+                       const double*, const double*, const double*, double*, int);
 void launch_sph_contract(hipStream_t, int, int, int, const SphPair*, const SphGroup*,
                          const double2*, const double2*, const double*, const double*, double2*,
-                         float2*);
+                         float2*, int);
 size_t isoft_plane_smem(int, int, int);
 int isoft_plane_use_fft(int, int, size_t);
 void launch_isoft_plane(hipStream_t, int, int, int, int, long long, const SphPair*, const float*,
                         const float2*, const float2*, const float2*, int, int, PlaneCand*, float*,
-                        double*);
+                        double*, int);
 void launch_sph_select(hipStream_t, int, int, long long, int, const SphPair*, const double*,
                        const double2*, const double2*, const double2*, int, const PlaneCand*,
                        double2*, long long, const double*, SphPeak*);
@@ -50,6 +52,9 @@ void launch_sph_finish(hipStream_t, int, int, const SphPair*, const SphGroup*, c
                        const double*, const double*, const SphOpt*, int, double*, int*, int*,
                        double*, SphResult*);
 void launch_coef_to_plane(hipStream_t, int, int, int, const SphPair*, const double2*, float2*);
+void launch_soft_forward(hipStream_t, int, long long, const double*, const double2*, const double2*,
+                         double*, double2*, double2*);
+size_t soft_forward_smem(int);
 void launch_sph_mirror(hipStream_t, int, int, const SphPair*, double2*);
 void launch_sph_energy(hipStream_t, int, const double2*, double, double, double, const double*,
                        const double*, int, double*);
@@ -62,7 +67,12 @@ void launch_per_product(hipStream_t, int, int, const PerPair*, const double2*, c
                         double2*);
 size_t per_plane_smem(int, int);
 void launch_per_plane(hipStream_t, int, int, int, const PerPair*, const double2*, const float2*,
-                      PlaneCand*, float*);
+                      PlaneCand*, float*, float2*);
+void launch_per_product_indexed(hipStream_t, int, int, const PerPair*, const double2*, const int*,
+                                const int*, double2*);
+void launch_find_peaks(hipStream_t, int, const float*, long long, float*, int, int, int, int, int,
+                       double*, int*);
+int find_peaks_max_window();
 void launch_per_select(hipStream_t, int, int, int, const PerPair*, const double2*, const double2*,
                        const PlaneCand*, double2*, long long, PerPeak*);
 void launch_per_refine(hipStream_t, int, int, const PerPair*, const PerGroup*, const int*,
@@ -144,6 +154,9 @@ struct fo_context {
   std::mutex call_mu;  // serialises the single-call helpers
   std::map<int, SphTable> sph;
   std::map<int, PerTw> per;
+  // launch shapes chosen by the autotuner, keyed by (grid side, largest
+  // degree, pair-count bucket, plane path)
+  std::map<std::tuple<int, int, int, int>, std::array<int, 3>> tuned;
   RecTable rect;
   std::vector<void*> owned;  // table allocations
 };
@@ -267,6 +280,72 @@ static int get_per_tw(fo_context* c, int F, const PerTw** out) {
   FO_HIP(hipStreamSynchronize(c->stream));
   c->per[F] = t;
   *out = &c->per[F];
+  return FO_OK;
+}
+
+// Device buffers of the multi-peak search, allocated on a batch's first
+// peak search (and grown when more peaks are asked for): one working copy of
+// every kept grid, the fitted parameters and the fit counts.
+struct PeakScratch {
+  void* base = nullptr;
+  size_t bytes = 0;
+  float* work = nullptr;
+  double* fit = nullptr;
+  int* count = nullptr;
+  int npeaks = 0;
+};
+
+static int peak_scratch(fo_context* c, PeakScratch& ps, long long ngrids, long long gsize,
+                        int npeaks) {
+  if (ps.base && ps.npeaks >= npeaks) return FO_OK;
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t bw = up(sizeof(float) * (size_t)ngrids * gsize);
+  const size_t bf = up(sizeof(double) * 11 * (size_t)ngrids * npeaks);
+  const size_t bc = up(sizeof(int) * (size_t)ngrids);
+  if (ps.base) ctx_free(c, ps.base, ps.bytes);
+  ps = PeakScratch{};
+  int rc = ctx_alloc(c, bw + bf + bc, &ps.base);
+  if (rc) return rc;
+  ps.bytes = bw + bf + bc;
+  ps.work = (float*)ps.base;
+  ps.fit = (double*)((char*)ps.base + bw);
+  ps.count = (int*)((char*)ps.base + bw + bf);
+  ps.npeaks = npeaks;
+  return FO_OK;
+}
+
+// Run the peak search over ngrids kept grids of gsize floats each and copy
+// the results out (fit: ngrids x npeaks x 11, count: ngrids, residual: the
+// working grids after the subtractions, optional).
+static int run_peak_search(fo_context* c, hipStream_t s, PeakScratch& ps, const float* grids,
+                           long long ngrids, int n0, int n1, int n2, int npeaks, int width,
+                           double* fit, int* count, float* residual) {
+  if (npeaks < 1 || npeaks > 64) return set_err(FO_ERR_INVALID, "npeaks must be in [1, 64]");
+  if (width < 1 || width > find_peaks_max_window())
+    return set_err(FO_ERR_UNSUPPORTED, "peak fit half width must be in [1, " +
+                                           std::to_string(find_peaks_max_window()) + "]");
+  const long long gsize = (long long)n0 * n1 * n2;
+  int rc = peak_scratch(c, ps, ngrids, gsize, npeaks);
+  if (rc) return rc;
+  launch_find_peaks(s, (int)ngrids, grids, gsize, ps.work, n0, n1, n2, npeaks, width, ps.fit,
+                    ps.count);
+  FO_CHECK_LAUNCH();
+  // the kernel leaves unfound peaks untouched: report zeros for them
+  std::vector<double> hf((size_t)ngrids * ps.npeaks * 11);
+  std::vector<int> hc((size_t)ngrids);
+  FO_HIP(hipMemcpyAsync(hf.data(), ps.fit, sizeof(double) * hf.size(), hipMemcpyDeviceToHost, s));
+  FO_HIP(hipMemcpyAsync(hc.data(), ps.count, sizeof(int) * hc.size(), hipMemcpyDeviceToHost, s));
+  if (residual)
+    FO_HIP(hipMemcpyAsync(residual, ps.work, sizeof(float) * (size_t)(ngrids * gsize),
+                          hipMemcpyDeviceToHost, s));
+  FO_HIP(hipStreamSynchronize(s));
+  for (long long gi = 0; gi < ngrids; ++gi) {
+    count[gi] = hc[gi];
+    for (int k = 0; k < npeaks; ++k)
+      for (int q = 0; q < 11; ++q)
+        fit[(gi * npeaks + k) * 11 + q] =
+            k < hc[gi] ? hf[((size_t)gi * npeaks + k) * 11 + q] : 0.0;
+  }
   return FO_OK;
 }
 
@@ -411,6 +490,12 @@ struct fo_sph_batch {
   } off{};
   int gsum = 0;
   bool own_mem = false, own_stream = false;
+  PeakScratch pk;  // multi-peak search buffers (allocated on first use)
+  // launch shapes (arithmetic-neutral): plane pairs per XCD chunk, highest
+  // degree contracted with 16-row tiles (-1: none), radial workgroups per
+  // pair; 0 = default.  autotune: pick them on the first full run.
+  std::array<int, 3> tune{{0, 15, 0}};
+  bool autotune = false, tuned = false;
 };
 
 static int sph_enqueue(fo_sph_batch* b, int stages) {
@@ -437,10 +522,11 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
       cmark(3 * c);
       launch_sph_ylm(s, Pc, pdc, b->gath, b->X1c, b->X2c, b->rect->leg, b->rect->cm, b->Y1, b->Y2);
       cmark(3 * c + 1);
-      launch_sph_radial(s, Pc, C.Lmax, C.maxjk, pdc, b->grp, b->gath, b->X1c, b->X2c, scc, b->R);
+      launch_sph_radial(s, Pc, C.Lmax, C.maxjk, pdc, b->grp, b->gath, b->X1c, b->X2c, scc, b->R,
+                        b->tune[2]);
       cmark(3 * c + 2);
       launch_sph_contract(s, Pc, L.n, C.Lmax, pdc, b->grp, b->Y1, b->Y2, b->R, scc, b->I64A,
-                          b->I32A);
+                          b->I32A, b->tune[1]);
     }
     cmark(3 * b->chunks.size());
   }
@@ -451,7 +537,7 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
   if (stages & FO_STAGE_GRID) {
     launch_isoft_plane(s, P, L.n, L.Lmax, b->fft_chunk, b->tab->Kp, b->pd, b->tab->tab32,
                        b->tab->tw32, b->I32A, b->I32B, L.mode, L.norient, b->cand, b->grid,
-                       b->abound);
+                       b->abound, b->tune[0]);
     mark(5);
     launch_sph_select(s, P, L.n, b->tab->Kt, L.norient, b->pd, b->tab->tab64, b->tab->tw64,
                       b->I64A, b->I64B, L.mode, b->cand, b->scr, b->scr_stride, b->abound,
@@ -606,6 +692,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   lc.mode = general ? 1 : 0;
   if (general) lc.norient = 2;
   lc.keep_grid = (d->flags & FO_SPH_KEEP_GRID) ? 1 : 0;
+  b->autotune = (d->flags & FO_SPH_AUTOTUNE) != 0;
   lc.center = (d->flags & FO_SPH_NO_CENTER) ? 0 : 1;
   b->Ntot = Ntot;
   b->Ktot = Ktot;
@@ -747,6 +834,7 @@ int fo_sph_batch_destroy(fo_sph_batch* b) {
   }
   if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
   if (b->own_mem) ctx_free(b->ctx, b->base, b->bytes);
+  if (b->pk.base) ctx_free(b->ctx, b->pk.base, b->pk.bytes);
   delete b;
   return FO_OK;
 }
@@ -822,9 +910,107 @@ int fo_sph_batch_set_rotation(fo_sph_batch* b, const double* R) {
   return FO_OK;
 }
 
+// The project's autotuner.  On the first full run of an FO_SPH_AUTOTUNE
+// batch, each launch shape is chosen in turn (coordinate descent, the others
+// held at their current choice) by timing its kernel over the batch's own
+// data with HIP events on the batch stream: every candidate runs once to
+// warm up and three times timed, the fastest mean wins.  Every candidate
+// computes the same results (only the workgroup-to-work mapping and the
+// tile shapes differ, not any reduction order), and the stages rewrite their
+// outputs from unchanged inputs, so tuning leaves the batch's data as one
+// eager run would.  The choice is cached in the context per (grid side,
+// largest degree, pair-count bucket, plane path) and reused by later batches.
+static int sph_autotune(fo_sph_batch* b) {
+  fo_context* c = b->ctx;
+  const SphLaunch& L = b->lc;
+  int bucket = 1;
+  while (bucket < L.P) bucket <<= 1;
+  const auto key = std::make_tuple(L.n, L.Lmax, bucket, b->fft_chunk);
+  {
+    std::lock_guard<std::mutex> g(c->mu);
+    auto it = c->tuned.find(key);
+    if (it != c->tuned.end()) {
+      b->tune = it->second;
+      b->tuned = true;
+      return FO_OK;
+    }
+  }
+  hipStream_t s = b->stream;
+  // inputs of every stage (and a warm start)
+  int rc = sph_enqueue(b, FO_STAGE_PREP | FO_STAGE_COEFF | FO_STAGE_GRID);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  FO_HIP(hipEventCreate(&e0));
+  FO_HIP(hipEventCreate(&e1));
+  auto time_stage = [&](int knob, float* ms) -> int {
+    auto launch = [&]() {
+      if (knob == 0) {
+        launch_isoft_plane(s, L.P, L.n, L.Lmax, b->fft_chunk, b->tab->Kp, b->pd, b->tab->tab32,
+                           b->tab->tw32, b->I32A, b->I32B, L.mode, L.norient, b->cand, b->grid,
+                           b->abound, b->tune[0]);
+      } else {
+        for (const CoefChunk& C : b->chunks) {
+          const SphPair* pdc = b->pd + C.p0;
+          const double* scc = b->scale + C.p0;
+          if (knob == 2)
+            launch_sph_radial(s, C.p1 - C.p0, C.Lmax, C.maxjk, pdc, b->grp, b->gath, b->X1c,
+                              b->X2c, scc, b->R, b->tune[2]);
+          else
+            launch_sph_contract(s, C.p1 - C.p0, L.n, C.Lmax, pdc, b->grp, b->Y1, b->Y2, b->R,
+                                scc, b->I64A, b->I32A, b->tune[1]);
+        }
+      }
+    };
+    launch();
+    FO_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < 3; ++r) launch();
+    FO_HIP(hipEventRecord(e1, s));
+    FO_CHECK_LAUNCH();
+    FO_HIP(hipEventSynchronize(e1));
+    FO_HIP(hipEventElapsedTime(ms, e0, e1));
+    return FO_OK;
+  };
+  const std::vector<int> cand[3] = {
+      {16, 1, 2, 4, 8, 32}, {15, -1, 7}, {1024, 64, 256, 4096}};
+  for (int knob = 0; knob < 3; ++knob) {
+    if (knob == 0 && !(L.P > 8)) continue;  // one chunk either way
+    int best = b->tune[knob];
+    float best_ms = 0.f;
+    bool first = true;
+    for (int v : cand[knob]) {
+      b->tune[knob] = v;
+      float ms = 0.f;
+      rc = time_stage(knob, &ms);
+      if (rc) break;
+      if (first || ms < best_ms) {
+        best = v;
+        best_ms = ms;
+        first = false;
+      }
+    }
+    b->tune[knob] = best;
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  // the data were rewritten with other shapes: the coefficient rows m1 < 0
+  // are refreshed on read-out as after any coefficient stage
+  b->mirror_pending = true;
+  b->tuned = true;
+  std::lock_guard<std::mutex> g(c->mu);
+  c->tuned[key] = b->tune;
+  return FO_OK;
+}
+
 int fo_sph_batch_run(fo_sph_batch* b, int stages, int use_graph) {
   if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
   FO_HIP(hipSetDevice(b->ctx->device));
+  const int full = FO_STAGE_PREP | FO_STAGE_COEFF | FO_STAGE_GRID;
+  if (b->autotune && !b->tuned && (stages & full) == full) {
+    int rc = sph_autotune(b);
+    if (rc) return rc;
+  }
   if (stages & FO_STAGE_COEFF) b->mirror_pending = true;
   if (use_graph && !b->timing) {
     auto it = b->graphs.find(stages);
@@ -939,6 +1125,42 @@ int fo_sph_batch_grid(fo_sph_batch* b, int pair, float* grid) {
   const size_t n3 = (size_t)b->lc.n * b->lc.n * b->lc.n * b->lc.norient;
   FO_HIP(hipMemcpyAsync(grid, b->grid + n3 * pair, sizeof(float) * n3, hipMemcpyDeviceToHost, b->stream));
   FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_sph_batch_find_peaks(fo_sph_batch* b, int npeaks, int width, double* fit, int* count,
+                            float* residual) {
+  if (!b || !fit || !count) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (!b->grid) return set_err(FO_ERR_STATE, "batch created without FO_SPH_KEEP_GRID");
+  FO_HIP(hipSetDevice(b->ctx->device));
+  const int n = b->lc.n;
+  return run_peak_search(b->ctx, b->stream, b->pk, b->grid, (long long)b->lc.P * b->lc.norient,
+                         n, n, n, npeaks, width, fit, count, residual);
+}
+
+int fo_sph_batch_tuning(fo_sph_batch* b, int* params, int set) {
+  if (!b || !params) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (set) {
+    if (params[0] < 0 || params[0] > 64 || params[1] < -1 || params[1] > 15 || params[2] < 0)
+      return set_err(FO_ERR_INVALID, "launch shape out of range");
+    for (int k = 0; k < 3; ++k) b->tune[k] = params[k];
+    b->tuned = true;
+    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+  } else {
+    for (int k = 0; k < 3; ++k) params[k] = b->tune[k];
+  }
+  return FO_OK;
+}
+
+int fo_sph_batch_start(fo_sph_batch* b, double* R0) {
+  if (!b || !R0) return set_err(FO_ERR_INVALID, "NULL argument");
+  const size_t m = (size_t)b->lc.P * b->lc.norient;
+  std::vector<SphPeak> pk(m);
+  FO_HIP(hipMemcpyAsync(pk.data(), b->peaks, sizeof(SphPeak) * m, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < m; ++i)
+    for (int k = 0; k < 3; ++k) R0[3 * i + k] = pk[i].R0[k];
   return FO_OK;
 }
 
@@ -1086,6 +1308,42 @@ int fo_sph_wigner_table(fo_context* c, int bw, double* Ds) {
   return FO_OK;
 }
 
+int fo_soft_forward(fo_context* c, int bw, const double* data, double* flmm) {
+  if (!c || !data || !flmm || bw < 1 || bw - 1 > kMaxL)
+    return set_err(FO_ERR_INVALID, "bad argument");
+  const int n = 2 * bw, L = bw - 1, W = 2 * L + 1;
+  if (soft_forward_smem(n) > c->max_lds)
+    return set_err(FO_ERR_UNSUPPORTED, "bandwidth too large for the forward transform");
+  const SphTable* t = nullptr;
+  int rc = get_sph_table(c, n, &t);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(c->call_mu);
+  FO_HIP(hipSetDevice(c->device));
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t n3 = (size_t)n * n * n, K = (size_t)kcount(L);
+  const size_t bg = up(sizeof(double2) * n3), bs = up(sizeof(double2) * (size_t)W * n * W);
+  const size_t bk = up(sizeof(double2) * K), bw8 = up(sizeof(double) * n);
+  void* base = nullptr;
+  rc = ctx_alloc(c, bg + bs + bk + bw8, &base);
+  if (rc) return rc;
+  char* p = (char*)base;
+  double2* dg = (double2*)p;
+  double2* S2 = (double2*)(p + bg);
+  double2* df = (double2*)(p + bg + bs);
+  double* w = (double*)(p + bg + bs + bk);
+  hipError_t e = hipMemcpyAsync(dg, data, sizeof(double2) * n3, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    launch_soft_forward(c->stream, n, t->Kt, t->tab64, t->tw64, dg, w, S2, df);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(flmm, df, sizeof(double2) * K, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  ctx_free(c, base, bg + bs + bk + bw8);
+  if (e != hipSuccess) return set_err(FO_ERR_HIP, std::string("forward SOFT: ") + hipGetErrorString(e));
+  return FO_OK;
+}
+
 int fo_lap_align(fo_context* c, int N, const double* X1, const double* X2, int ngroups,
                  const int* gsizes, const int* gatoms, int metric, const double* box, int* perm,
                  double* dist) {
@@ -1163,10 +1421,17 @@ struct fo_per_batch {
   hipEvent_t ev[5] = {};
   bool ev_made = false;
   struct {
-    size_t pd, grp, at, p1, p2, x1, x2, cp, c1, c2, cand, pk, res, perm, sp, rs, scr, grid;
+    size_t pd, grp, at, p1, p2, x1, x2, cp, c1, c2, cand, pk, res, perm, sp, rs, scr, grid, cgrid;
   } off{};
   bool keepc = false;
   bool own_mem = false, own_stream = false;
+  float2* cgrid = nullptr;   // complex f (FO_PER_KEEP_FFT)
+  // per-structure coefficients shared by the pairs (set_coeffs_indexed)
+  void* csbuf = nullptr;
+  size_t csbytes = 0;
+  double2* Cs = nullptr;
+  int *idx1 = nullptr, *idx2 = nullptr;
+  PeakScratch pk;
 };
 
 static int per_enqueue(fo_per_batch* b, int stages) {
@@ -1177,7 +1442,9 @@ static int per_enqueue(fo_per_batch* b, int stages) {
   };
   mark(0);
   if (stages & FO_PSTAGE_SF) {
-    if (b->flags & FO_PER_INPUT_COEFFS)
+    if ((b->flags & FO_PER_INPUT_COEFFS) && b->Cs)
+      launch_per_product_indexed(s, b->P, b->nk, b->pd, b->Cs, b->idx1, b->idx2, b->Cprod);
+    else if (b->flags & FO_PER_INPUT_COEFFS)
       launch_per_product(s, b->P, b->nk, b->pd, b->C1, b->C2, b->Cprod);
     else
       launch_per_sf(s, b->P, b->nk, b->pd, b->grp, b->atoms, b->pos1, b->pos2, b->Cprod,
@@ -1186,7 +1453,8 @@ static int per_enqueue(fo_per_batch* b, int stages) {
   }
   mark(1);
   if (stages & FO_PSTAGE_GRID) {
-    launch_per_plane(s, b->P, b->nk, b->F, b->pd, b->Cprod, b->tw->tw32, b->cand, b->grid);
+    launch_per_plane(s, b->P, b->nk, b->F, b->pd, b->Cprod, b->tw->tw32, b->cand, b->grid,
+                     b->cgrid);
     mark(2);
     launch_per_select(s, b->P, b->nk, b->F, b->pd, b->Cprod, b->tw->tw64, b->cand, b->scr,
                       b->scr_stride, b->peaks);
@@ -1292,6 +1560,7 @@ int per_batch_plan(fo_context* c, const fo_per_batch_desc* d, fo_per_batch** out
   o.rs = lay.add<int>(Ntot + 1);
   o.scr = lay.add<double2>((size_t)b->P * b->scr_stride);
   o.grid = (b->flags & FO_PER_KEEP_GRID) ? lay.add<float>((size_t)b->P * b->F * b->F * b->F) : 0;
+  o.cgrid = (b->flags & FO_PER_KEEP_FFT) ? lay.add<float2>((size_t)b->P * b->F * b->F * b->F) : 0;
   b->bytes = lay.off;
   *out = b.release();
   return FO_OK;
@@ -1329,6 +1598,7 @@ int per_batch_bind(fo_per_batch* b, void* mem, hipStream_t stream) {
   b->rowsol = (int*)(B + o.rs);
   b->scr = (double2*)(B + o.scr);
   b->grid = (b->flags & FO_PER_KEEP_GRID) ? (float*)(B + o.grid) : nullptr;
+  b->cgrid = (b->flags & FO_PER_KEEP_FFT) ? (float2*)(B + o.cgrid) : nullptr;
   if (stream) {
     b->stream = stream;
     b->own_stream = false;
@@ -1377,6 +1647,8 @@ int fo_per_batch_destroy(fo_per_batch* b) {
     for (auto& e : b->ev) (void)hipEventDestroy(e);
   if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
   if (b->own_mem) ctx_free(b->ctx, b->base, b->bytes);
+  if (b->csbuf) ctx_free(b->ctx, b->csbuf, b->csbytes);
+  if (b->pk.base) ctx_free(b->ctx, b->pk.base, b->pk.bytes);
   delete b;
   return FO_OK;
 }
@@ -1407,6 +1679,12 @@ int fo_per_batch_set_positions_device(fo_per_batch* b, const void* p1, const voi
 int fo_per_batch_set_coeffs(fo_per_batch* b, const double* C1, const double* C2) {
   if (!b || !C1 || !C2) return set_err(FO_ERR_INVALID, "NULL argument");
   if (!b->C1) return set_err(FO_ERR_STATE, "batch created without FO_PER_INPUT_COEFFS");
+  if (b->Cs) {
+    // back to each pair's own coefficients (a captured graph read the shared ones)
+    b->Cs = nullptr;
+    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+  }
   const size_t m = (size_t)b->gsum * b->cube;
   FO_HIP(hipMemcpyAsync(b->C1, C1, sizeof(double2) * m, hipMemcpyHostToDevice, b->stream));
   FO_HIP(hipMemcpyAsync(b->C2, C2, sizeof(double2) * m, hipMemcpyHostToDevice, b->stream));
@@ -1517,6 +1795,63 @@ int fo_per_batch_grid(fo_per_batch* b, int pair, float* g) {
   FO_HIP(hipMemcpyAsync(g, b->grid + f3 * pair, sizeof(float) * f3, hipMemcpyDeviceToHost, b->stream));
   FO_HIP(hipStreamSynchronize(b->stream));
   return FO_OK;
+}
+
+int fo_per_batch_fft(fo_per_batch* b, int pair, float* f) {
+  if (!b || !f) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (!b->cgrid) return set_err(FO_ERR_STATE, "batch created without FO_PER_KEEP_FFT");
+  if (pair < 0 || pair >= b->P) return set_err(FO_ERR_INVALID, "pair out of range");
+  const size_t f3 = (size_t)b->F * b->F * b->F;
+  FO_HIP(hipMemcpyAsync(f, b->cgrid + f3 * pair, sizeof(float2) * f3, hipMemcpyDeviceToHost,
+                        b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_per_batch_set_coeffs_indexed(fo_per_batch* b, const double* Cs, int nstruct,
+                                    const int* idx1, const int* idx2) {
+  if (!b || !Cs || !idx1 || !idx2) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (!(b->flags & FO_PER_INPUT_COEFFS))
+    return set_err(FO_ERR_STATE, "batch created without FO_PER_INPUT_COEFFS");
+  if (nstruct < 1) return set_err(FO_ERR_INVALID, "nstruct must be >= 1");
+  const int ng = b->hp[0].ngroups;
+  for (int p = 0; p < b->P; ++p) {
+    if (b->hp[p].ngroups != ng)
+      return set_err(FO_ERR_INVALID, "shared coefficients need the same groups in every pair");
+    if (idx1[p] < 0 || idx1[p] >= nstruct || idx2[p] < 0 || idx2[p] >= nstruct)
+      return set_err(FO_ERR_INVALID, "structure index out of range");
+  }
+  FO_HIP(hipSetDevice(b->ctx->device));
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t bc = up(sizeof(double2) * (size_t)nstruct * ng * b->cube);
+  const size_t bi = up(sizeof(int) * (size_t)b->P);
+  if (b->csbuf) ctx_free(b->ctx, b->csbuf, b->csbytes);
+  b->csbuf = nullptr;
+  b->Cs = nullptr;
+  int rc = ctx_alloc(b->ctx, bc + 2 * bi, &b->csbuf);
+  if (rc) return rc;
+  b->csbytes = bc + 2 * bi;
+  b->Cs = (double2*)b->csbuf;
+  b->idx1 = (int*)((char*)b->csbuf + bc);
+  b->idx2 = (int*)((char*)b->csbuf + bc + bi);
+  FO_HIP(hipMemcpyAsync(b->Cs, Cs, sizeof(double2) * (size_t)nstruct * ng * b->cube,
+                        hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipMemcpyAsync(b->idx1, idx1, sizeof(int) * b->P, hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipMemcpyAsync(b->idx2, idx2, sizeof(int) * b->P, hipMemcpyHostToDevice, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  // a captured launch sequence would still read the pairs' own coefficients
+  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  b->graphs.clear();
+  return FO_OK;
+}
+
+int fo_per_batch_find_peaks(fo_per_batch* b, int npeaks, int width, double* fit, int* count,
+                            float* residual) {
+  if (!b || !fit || !count) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (!b->grid) return set_err(FO_ERR_STATE, "batch created without FO_PER_KEEP_GRID");
+  FO_HIP(hipSetDevice(b->ctx->device));
+  return run_peak_search(b->ctx, b->stream, b->pk, b->grid, b->P, b->F, b->F, b->F, npeaks,
+                         width, fit, count, residual);
 }
 
 int fo_per_batch_timing(fo_per_batch* b, int enable, float* ms4) {

@@ -129,12 +129,38 @@ __global__ void k_per_product(const PerPair* __restrict__ pd, int nk,
   }
 }
 
+// The same product from per-structure coefficients shared by many pairs:
+// pair p multiplies structure idx1[p] with structure idx2[p] (alignGroup's
+// coefficient reuse, fastoverlap/periodicAlignment.py:466-473).  Cs holds
+// nstruct x ngroups cubes; every pair has the same ngroups.
+__global__ void k_per_product_indexed(const PerPair* __restrict__ pd, int nk,
+                                      const double2* __restrict__ Cs, const int* __restrict__ idx1,
+                                      const int* __restrict__ idx2, double2* __restrict__ Cprod) {
+  const int NB = 2 * nk + 1, p = blockIdx.y;
+  const PerPair d = pd[p];
+  const long long cube = (long long)NB * NB * NB;
+  const double2* A = Cs + (long long)idx1[p] * d.ngroups * cube;
+  const double2* B = Cs + (long long)idx2[p] * d.ngroups * cube;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < cube;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int a = (int)(e / (NB * NB)), b = (int)((e / NB) % NB), c = (int)(e % NB);
+    double2 s = make_double2(0.0, 0.0);
+    for (int g = 0; g < d.ngroups; ++g)
+      s = cadd(s, cmul(A[(long long)g * cube + e], cconj(B[(long long)g * cube + e])));
+    const double kx = 2.0 * kPi * (a - nk) / d.box[0];
+    const double ky = 2.0 * kPi * (b - nk) / d.box[1];
+    const double kz = 2.0 * kPi * (c - nk) / d.box[2];
+    Cprod[d.c_off + e] = cscale(s, exp(-(kx * kx + ky * ky + kz * kz) * d.scale * d.scale));
+  }
+}
+
 // |f(x,y,z)|, f = sum_{a,b,c} C[a,b,c] e^{-2 pi i (a x + b y + c z)/F}, for
 // one x-plane per workgroup (numpy fftn with C at the array corner,
 // fastoverlap/periodicAlignment.py:439-440).
 __global__ void __launch_bounds__(256) k_per_plane(
     const PerPair* __restrict__ pd, int nk, int F, const double2* __restrict__ Cprod,
-    const float2* __restrict__ twg, PlaneCand* __restrict__ cand, float* __restrict__ grid) {
+    const float2* __restrict__ twg, PlaneCand* __restrict__ cand, float* __restrict__ grid,
+    float2* __restrict__ cgrid) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int NB = 2 * nk + 1;
   const int x = blockIdx.x, p = blockIdx.y, t = threadIdx.x, bd = blockDim.x;
@@ -194,6 +220,7 @@ __global__ void __launch_bounds__(256) k_per_plane(
     const float val = sqrtf(acc.x * acc.x + acc.y * acc.y);
     const int flat = (x * F + y) * F + z;
     if (grid) grid[(long long)p * F * F * F + flat] = val;
+    if (cgrid) cgrid[(long long)p * F * F * F + flat] = acc;
     if (val > bv) {
       dm = fmaxf(dm, bv);
       bv = val;
@@ -558,9 +585,18 @@ size_t per_plane_smem(int nk, int F) {
 }
 
 void launch_per_plane(hipStream_t s, int P, int nk, int F, const PerPair* pd,
-                      const double2* Cprod, const float2* tw32, PlaneCand* cand, float* grid) {
+                      const double2* Cprod, const float2* tw32, PlaneCand* cand, float* grid,
+                      float2* cgrid) {
   hipLaunchKernelGGL(k_per_plane, dim3(F, P), dim3(256), per_plane_smem(nk, F), s, pd, nk, F,
-                     Cprod, tw32, cand, grid);
+                     Cprod, tw32, cand, grid, cgrid);
+}
+
+void launch_per_product_indexed(hipStream_t s, int P, int nk, const PerPair* pd,
+                                const double2* Cs, const int* idx1, const int* idx2,
+                                double2* Cprod) {
+  const long long cube = (long long)(2 * nk + 1) * (2 * nk + 1) * (2 * nk + 1);
+  hipLaunchKernelGGL(k_per_product_indexed, dim3(std::min(pcdiv(cube, 256), 256), P), dim3(256),
+                     0, s, pd, nk, Cs, idx1, idx2, Cprod);
 }
 
 void launch_per_select(hipStream_t s, int P, int nk, int F, const PerPair* pd,

@@ -1,16 +1,27 @@
 // Continuous-batching server for alignment requests (spherical and periodic).
 //
 // Requests are copied at submit and queued in arrival order.  Each of
-// `streams` worker threads owns a HIP stream and a grow-only device arena;
-// whenever a worker is free it forms the next batch from the class (kind +
-// grid shape) of the oldest pending request: requests of that class are
-// admitted in arrival order while the in-flight device footprint stays within
-// the byte budget, up to max_batch pairs.  When other batches are running
+// `streams` worker threads owns a HIP stream and a device arena; whenever a
+// worker is free it forms the next batch from the class (kind + grid shape)
+// of the oldest pending request: requests of that class are admitted in
+// arrival order, up to max_batch pairs, while the device memory the workers'
+// arenas hold stays within the byte budget.  When other batches are running
 // and the batch is not full, the worker waits up to max_wait_ms for more
 // arrivals of that class.  Batches on different workers overlap on the
 // device.  Every per-pair computation is done by per-pair workgroups with
 // fixed reduction orders, so a result does not depend on the batch it was
 // served in.
+//
+// Admission by HBM footprint: each request carries `est`, an upper bound of
+// what it adds to any batch of its class (the device bytes of a one-pair
+// batch, with the per-pair selection scratch sized for the largest degree
+// the class allows), so a batch's exact footprint never exceeds the sum of
+// its requests' estimates.  `committed` counts the bytes the arenas hold; a
+// worker admits a batch only if the arena it would grow to fits the budget
+// beside the other arenas.  With a budget set, a worker with nothing to run
+// releases its arena, so memory moves to where requests are waiting; a
+// request that cannot fit even alone, with no other arena held, fails with
+// FO_ERR_NOMEM while the server keeps serving.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,7 +50,7 @@ struct Req {
   std::vector<double> pos1, pos2;
   std::vector<int> gsizes, gatoms;
   int key_a = 0, key_b = 0;  // class: spherical (grid_n, invert), periodic (nk, F)
-  size_t est = 0;            // device bytes of a one-pair batch
+  size_t est = 0;            // upper bound of the bytes it adds to a batch of its class
   Clock::time_point t_submit, t_admit, t_done;
   bool done = false;
   fo_result res{};
@@ -83,7 +94,8 @@ struct fo_server {
   std::condition_variable cv_work, cv_done;
   std::map<long long, std::shared_ptr<Req>> reqs;
   std::deque<std::shared_ptr<Req>> pending;
-  size_t inflight = 0, peak = 0;
+  size_t committed = 0, peak = 0;  // bytes held by the worker arenas
+  int running = 0;                 // batches admitted and not finished
   long long next_id = 1, completed = 0, batches = 0;
   bool stop = false;
   std::vector<std::thread> workers;
@@ -158,22 +170,25 @@ void make_per_desc(const std::vector<std::shared_ptr<Req>>& v, PerDescBuf& b) {
   b.d.flags = 0;
 }
 
+// A worker's device memory.  Its capacity is accounted in the server's
+// `committed` bytes by the worker (under the server lock) before it grows.
 struct Arena {
   fo_context* ctx = nullptr;
   void* p = nullptr;
   size_t cap = 0;
   int ensure(size_t bytes) {
     if (bytes <= cap) return FO_OK;
-    if (p) fo::ctx_free_bytes(ctx, p, cap);
-    p = nullptr;
-    cap = 0;
+    release();
     int rc = fo::ctx_alloc_bytes(ctx, bytes, &p);
     if (rc == FO_OK) cap = bytes;
     return rc;
   }
-  ~Arena() {
+  void release() {
     if (p) fo::ctx_free_bytes(ctx, p, cap);
+    p = nullptr;
+    cap = 0;
   }
+  ~Arena() { release(); }
 };
 
 // Run one batch; fills each request's result.  Returns a status code.
@@ -185,7 +200,9 @@ int run_batch(fo_server* s, std::vector<std::shared_ptr<Req>>& v, hipStream_t st
     fo_sph_batch* b = nullptr;
     int rc = fo::sph_batch_plan(s->ctx, &db.d, &b);
     if (rc) return rc;
-    rc = ar.ensure(fo::sph_batch_bytes(b));
+    rc = fo::sph_batch_bytes(b) <= ar.cap
+             ? FO_OK
+             : fo::set_error(FO_ERR_NOMEM, "batch larger than its admitted footprint");
     if (!rc) rc = fo::sph_batch_bind(b, ar.p, stream);
     if (!rc) rc = fo_sph_batch_set_positions(b, db.pos1.data(), db.pos2.data());
     if (!rc) rc = fo_sph_batch_run(b, FO_STAGE_ALL, 0);
@@ -220,7 +237,9 @@ int run_batch(fo_server* s, std::vector<std::shared_ptr<Req>>& v, hipStream_t st
   fo_per_batch* b = nullptr;
   int rc = fo::per_batch_plan(s->ctx, &db.d, &b);
   if (rc) return rc;
-  rc = ar.ensure(fo::per_batch_bytes(b));
+  rc = fo::per_batch_bytes(b) <= ar.cap
+           ? FO_OK
+           : fo::set_error(FO_ERR_NOMEM, "batch larger than its admitted footprint");
   if (!rc) rc = fo::per_batch_bind(b, ar.p, stream);
   if (!rc) rc = fo_per_batch_set_positions(b, db.pos1.data(), db.pos2.data());
   if (!rc) rc = fo_per_batch_run(b, FO_PSTAGE_ALL, 0);
@@ -251,31 +270,46 @@ void worker_main(fo_server* s) {
   if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return;
   Arena ar;
   ar.ctx = s->ctx;
-  const size_t budget = s->cfg.hbm_budget ? s->cfg.hbm_budget : (size_t)-1;
+  const bool limited = s->cfg.hbm_budget != 0;
+  const size_t budget = limited ? (size_t)s->cfg.hbm_budget : (size_t)-1;
+  // hand the arena back (with a budget only: it is kept for reuse otherwise)
+  auto release = [&](std::unique_lock<std::mutex>& lk) {
+    if (!limited || !ar.cap) return;
+    const size_t cap = ar.cap;
+    lk.unlock();
+    ar.release();
+    lk.lock();
+    s->committed -= cap;
+    s->cv_work.notify_all();
+  };
   for (;;) {
     std::vector<std::shared_ptr<Req>> batch;
-    size_t reserve = 0;
+    size_t need = 0;  // arena bytes the batch may use (sum of estimates)
     {
       std::unique_lock<std::mutex> lk(s->mu);
       for (;;) {
         if (s->stop) break;
         if (s->pending.empty()) {
-          s->cv_work.wait(lk);
+          release(lk);
+          if (s->pending.empty() && !s->stop) s->cv_work.wait(lk);
           continue;
         }
         const std::shared_ptr<Req> head = s->pending.front();
+        // bytes this worker may hold: the budget less the other arenas
+        const size_t others = s->committed - ar.cap;
+        const size_t avail = budget - std::min(budget, others);
         batch.clear();
-        reserve = 0;
+        need = 0;
         for (const auto& q : s->pending) {
           if (!same_class(*q, *head)) continue;
-          if (s->inflight + reserve + q->est > budget) break;
+          if (need + q->est > avail) break;
           batch.push_back(q);
-          reserve += q->est;
+          need += q->est;
           if ((int)batch.size() >= s->cfg.max_batch) break;
         }
         if (batch.empty()) {
-          if (s->inflight == 0) {
-            // can never fit: fail the head request
+          if (others == 0 && head->est > budget) {
+            // can never fit, even alone: fail the head request
             head->res.status = FO_ERR_NOMEM;
             head->done = true;
             head->t_done = Clock::now();
@@ -283,11 +317,12 @@ void worker_main(fo_server* s) {
             s->cv_done.notify_all();
             continue;
           }
-          s->cv_work.wait(lk);  // wait for memory
+          release(lk);  // wait for memory with nothing held
+          s->cv_work.wait(lk);
           continue;
         }
         const double waited = ms_between(head->t_submit, Clock::now());
-        if ((int)batch.size() < s->cfg.max_batch && s->inflight > 0 &&
+        if ((int)batch.size() < s->cfg.max_batch && s->running > 0 &&
             waited < s->cfg.max_wait_ms) {
           s->cv_work.wait_for(lk, std::chrono::duration<double, std::milli>(
                                       s->cfg.max_wait_ms - waited));
@@ -302,10 +337,23 @@ void worker_main(fo_server* s) {
         q->t_admit = now;
         s->pending.erase(std::find(s->pending.begin(), s->pending.end(), q));
       }
-      s->inflight += reserve;
-      s->peak = std::max(s->peak, s->inflight);
+      s->running += 1;
+      // the arena grows (if it must) to the admitted bytes: account first
+      if (need > ar.cap) {
+        s->committed += need - ar.cap;
+        s->peak = std::max(s->peak, s->committed);
+      }
     }
-    const int rc = run_batch(s, batch, stream, ar);
+    int rc = FO_OK;
+    if (need > ar.cap) {
+      rc = ar.ensure(need);
+      if (rc) {
+        // committed counted `need` for this worker; a failed grow holds nothing
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->committed -= need - ar.cap;
+      }
+    }
+    if (!rc) rc = run_batch(s, batch, stream, ar);
     {
       std::lock_guard<std::mutex> lk(s->mu);
       const auto now = Clock::now();
@@ -317,13 +365,18 @@ void worker_main(fo_server* s) {
         q->res.queue_ms = ms_between(q->t_submit, q->t_admit);
         q->done = true;
       }
-      s->inflight -= reserve;
       s->completed += (long long)batch.size();
       s->batches += 1;
+      s->running -= 1;
     }
     s->cv_done.notify_all();
     s->cv_work.notify_all();
   }
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->committed -= ar.cap;
+  }
+  ar.release();
   (void)hipStreamDestroy(stream);
 }
 
@@ -388,7 +441,15 @@ int fo_server_submit(fo_server* s, const fo_request* r, long long* id) {
     fo_sph_batch* b = nullptr;
     rc = fo::sph_batch_plan(s->ctx, &db.d, &b);
     if (rc) return rc;
-    q->est = fo::sph_batch_bytes(b);
+    // every array of a batch is a sum over its pairs except the selection
+    // scratch, sized per pair for the batch's largest degree: take the
+    // largest degree the class's grid allows, so the sum of the requests'
+    // estimates bounds any batch of them
+    const long long n = q->key_a, Lc = std::min<long long>(n / 2 - 1, 63), L = r->L;
+    const long long no = r->invert ? 2 : 1;
+    const long long scr = (Lc + 1) * (2 * Lc + 1) + (Lc + 1) * n - (L + 1) * (2 * L + 1) -
+                          (L + 1) * n;
+    q->est = fo::sph_batch_bytes(b) + (size_t)(no * std::max(scr, 0LL)) * 16 + 256;
     fo_sph_batch_destroy(b);
   } else if (r->kind == 1) {
     const double V = r->box[0] * r->box[1] * r->box[2];

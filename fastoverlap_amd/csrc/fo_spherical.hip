@@ -2164,8 +2164,10 @@ void launch_sph_ylm(hipStream_t s, int P, const SphPair* pd, const int* gath, co
 
 void launch_sph_radial(hipStream_t s, int P, int Lmax, long long maxjk, const SphPair* pd,
                        const SphGroup* grp, const int* gath, const double* X1c,
-                       const double* X2c, const double* scale_eff, double* R) {
-  const int bx = std::max(1, std::min(cdiv(maxjk, 256), 1024));
+                       const double* X2c, const double* scale_eff, double* R, int bx_cap) {
+  // workgroups per pair: one per 256 (j, k) entries up to bx_cap (the rest
+  // is a grid-stride loop); every entry is computed the same way
+  const int bx = std::max(1, std::min(cdiv(maxjk, 256), bx_cap > 0 ? bx_cap : 1024));
   (void)Lmax;  // one kernel for every degree
   hipLaunchKernelGGL(k_sph_radial, dim3(bx, P), dim3(256), 0, s, pd, grp, gath, X1c, X2c,
                      scale_eff, R);
@@ -2173,9 +2175,12 @@ void launch_sph_radial(hipStream_t s, int P, int Lmax, long long maxjk, const Sp
 
 void launch_sph_contract(hipStream_t s, int P, int n, int Lmax, const SphPair* pd,
                          const SphGroup* grp, const double2* Y1, const double2* Y2,
-                         const double* R, const double* scale_eff, double2* I64, float2* I32) {
-  // one launch per column-tile class, so the low degrees run with the
-  // smaller tiles (16 rows: five workgroups per CU instead of three)
+                         const double* R, const double* scale_eff, double2* I64, float2* I32,
+                         int split) {
+  // one launch per column-tile class: degrees <= split (at most 15) run with
+  // the smaller tiles (16 rows: five workgroups per CU instead of three),
+  // the rest with 32-row tiles.  Each output's reduction order is the same
+  // in both tile shapes.
   auto launch = [&](int K, int lo, int hi) {
     hi = std::min(hi, Lmax);
     if (lo > hi) return;
@@ -2193,8 +2198,9 @@ void launch_sph_contract(hipStream_t s, int P, int n, int Lmax, const SphPair* p
       hipLaunchKernelGGL(k_sph_contract_mfma<8>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,
                          I32, n, P, nb, lo, hi);
   };
-  launch(2, 0, 15);
-  launch(4, 16, 31);
+  split = std::min(split, 15);
+  if (split >= 0) launch(2, 0, split);
+  launch(4, split + 1, 31);
   launch(8, 32, 63);
 }
 
@@ -2226,7 +2232,7 @@ int isoft_plane_use_fft(int n, int Lmax, size_t max_lds) {
 void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long long Kp,
                         const SphPair* pd, const float* tab32, const float2* tw32,
                         const float2* IA, const float2* IB, int mode, int norient,
-                        PlaneCand* cand, float* grid, double* abound) {
+                        PlaneCand* cand, float* grid, double* abound, int chunk) {
   const size_t sm = isoft_plane_smem(n, Lmax, use_fft);
   int n1, n2;
   if (!use_fft || !fft_shape(n, n1, n2)) {
@@ -2234,8 +2240,9 @@ void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long
                        IB, mode, norient, cand, grid, abound);
     return;
   }
-  // pair chunk per XCD (16 measured best on the 4096-pair n=64 class)
-  const int C = std::max(1, std::min(16, (P + 7) / 8));
+  // pairs per chunk of one XCD (workgroups of a chunk walk its pairs
+  // beta-major); 16 unless chosen by the autotuner
+  const int C = std::max(1, std::min(chunk > 0 ? chunk : 16, (P + 7) / 8));
   const dim3 g((unsigned)(8 * ((P + 8 * C - 1) / (8 * C)) * C * n)), b(256);
 #define FO_FFT(N1, N2)                                                                         \
   if (mode == 0)                                                                                \
@@ -2448,6 +2455,85 @@ void launch_sph_mirror(hipStream_t s, int P, int Lmax, const SphPair* pd, double
   if (tot == 0) return;
   hipLaunchKernelGGL(k_sph_mirror, dim3(cdiv(tot, 256), P), dim3(256), 0, s, pd, I);
 }
+
+// ---------------------------------------------------------------------------
+// Forward SOFT (fastoverlap/soft.py:98-113), FP64:
+//   S2[m1][k][m2] = n^-2 sum_{j1,j2} g[j1][k][j2] e^{-2 pi i (m1 j1 + m2 j2)/n}
+//   f^l_{m1m2}    = sum_k w_k dn^l_{m1m2}(b_k) S2[m1][k][m2]
+// for |m1|, |m2| <= L = n/2 - 1; w_k are the quadrature weights of
+// fastoverlap/soft.py:63-71.
+// ---------------------------------------------------------------------------
+__global__ void k_soft_weights(int bw, double* __restrict__ w) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 2 * bw; k += gridDim.x * blockDim.x) {
+    const double a = (2.0 * k + 1.0) / (4.0 * bw);  // (2k+1) pi / (4 bw), in units of pi
+    double s = 0.0;
+    for (int j = 0; j < bw; ++j) s += 2.0 / (2.0 * j + 1.0) * sinpi((2.0 * j + 1.0) * a);
+    w[k] = s * sinpi(a) / bw;
+  }
+}
+
+// one workgroup per beta plane k: the 2D DFT of g[., k, .] at |m| <= L
+__global__ void k_soft_fwd_plane(int n, const double2* __restrict__ g,
+                                 const double2* __restrict__ tw, double2* __restrict__ S2) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = blockIdx.x, L = n / 2 - 1, W = 2 * L + 1, t = threadIdx.x;
+  double2* T = (double2*)smem;  // [n][W]: sum over j2 for each j1, m2
+  for (int e = t; e < n * W; e += blockDim.x) {
+    const int j1 = e / W, m2 = e % W - L;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j2 = 0; j2 < n; ++j2) {
+      // e^{-2 pi i m2 j2 / n} = conj(tw[(m2 j2) mod n])
+      const int q = ((m2 * j2) % n + n) % n;
+      acc = cadd(acc, cmul(g[((long long)j1 * n + k) * n + j2], cconj(tw[q])));
+    }
+    T[e] = acc;
+  }
+  __syncthreads();
+  const double inv = 1.0 / ((double)n * n);
+  for (int e = t; e < W * W; e += blockDim.x) {
+    const int m1 = e / W - L, m2i = e % W;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j1 = 0; j1 < n; ++j1) {
+      const int q = ((m1 * j1) % n + n) % n;
+      acc = cadd(acc, cmul(T[j1 * W + m2i], cconj(tw[q])));
+    }
+    S2[((long long)(m1 + L) * n + k) * W + m2i] = cscale(acc, inv);
+  }
+}
+
+// one thread per compact (l, m1, m2), l <= L
+__global__ void k_soft_fwd_beta(int n, long long Kt, const double* __restrict__ tab64,
+                                const double* __restrict__ w, const double2* __restrict__ S2,
+                                double2* __restrict__ f) {
+  const int L = n / 2 - 1, W = 2 * L + 1;
+  const long long K = kcount(L);
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < K;
+       e += (long long)gridDim.x * blockDim.x) {
+    int l = 0;
+    while (kbase(l + 1) <= e) ++l;
+    const int r = (int)(e - kbase(l));
+    const int m1 = r / (2 * l + 1) - l, m2 = r % (2 * l + 1) - l;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k = 0; k < n; ++k)
+      acc = cadd(acc, cscale(S2[((long long)(m1 + L) * n + k) * W + (m2 + L)],
+                             w[k] * tab64[(long long)k * Kt + e]));
+    f[e] = acc;
+  }
+}
+
+void launch_soft_forward(hipStream_t s, int n, long long Kt, const double* tab64,
+                         const double2* tw64, const double2* g, double* w, double2* S2,
+                         double2* f) {
+  const int L = n / 2 - 1, W = 2 * L + 1;
+  hipLaunchKernelGGL(k_soft_weights, dim3(1), dim3(256), 0, s, n / 2, w);
+  hipLaunchKernelGGL(k_soft_fwd_plane, dim3(n), dim3(256), sizeof(double2) * (size_t)n * W, s,
+                     n, g, tw64, S2);
+  const long long K = kcount(L);
+  hipLaunchKernelGGL(k_soft_fwd_beta, dim3((unsigned)std::min<long long>((K + 255) / 256, 4096)),
+                     dim3(256), 0, s, n, Kt, tab64, w, S2, f);
+}
+
+size_t soft_forward_smem(int n) { return sizeof(double2) * (size_t)n * (n - 1); }
 
 void launch_coef_to_plane(hipStream_t s, int P, int n, int Lmax, const SphPair* pd,
                           const double2* a, float2* b) {

@@ -2,17 +2,20 @@
 running on the MI355X through ``libfastoverlap_hip``.
 
 Same constructor / call signatures and return tuples.  Structure factors
-(FP64), the |3D DFT| grid (FP32, pruned: only the (2n+1)^3 non-zero corner
-is transformed), the exact argmax (FP64 re-evaluation of every candidate
-within the FP32 error bound) and the LAP + mean-shift refinement (FP64) all
-run in HIP kernels.
+(FP64), the 3D transform of the overlap product (FP32, pruned: only the
+(2n+1)^3 non-zero corner is transformed), the exact argmax (FP64
+re-evaluation of every candidate within the FP32 error bound), the
+Gaussian-fitted peaks of ``npeaks > 1`` and the LAP + mean-shift refinement
+(FP64) all run in HIP kernels.
 
 Differences from the reference, by design:
  * the LAP uses the intended semantics (cost rows = X1, ``X2[perm]`` ~ X1);
    the reference's munkres path applies the inverse permutation
    (``periodicAlignment.py:94-107``);
- * ``f`` holds |f| (the phase of the overlap transform is not kept);
- * ``npeaks > 1`` (multi-peak Gaussian fits) is not built.
+ * ``f`` (the complex transform) is held in FP32 precision;
+ * the diagnostic attributes ``C1, C2, C, f, fabs, Csum`` are recomputed on
+   first access after an alignment (the reference recomputes them on every
+   call), so plain calls do not pay for them.
 """
 
 from __future__ import annotations
@@ -61,6 +64,8 @@ class BasePeriodicAlignment:
 class PeriodicAlign(BasePeriodicAlignment):
     """``PeriodicAlign(Natoms, boxvec, permlist=None, dim=3, scale=None, n=None)``."""
 
+    _LAZY = ("C1", "C2", "C", "f", "fabs", "Csum")
+
     def __init__(self, Natoms, boxvec, permlist=None, dim=3, scale=None, n=None):
         if dim != 3:
             raise ValueError("only dim=3 is supported")
@@ -78,7 +83,18 @@ class PeriodicAlign(BasePeriodicAlignment):
         self.n = int(np.ceil(1.3 * Natoms ** (1.0 / 3.0))) if n is None else n
         self.setScale(scale)
         self.setks()
-        self.setPos(self.pos1, self.pos2)
+        # the reference's setPos(pos1, pos2) of the zero configuration, on
+        # first access of its attributes
+        self._attrs = None
+        self._Cs = None
+
+    def __getattr__(self, name):
+        # C1, C2, C, f, fabs, Csum of the last configuration, computed on demand
+        if name in PeriodicAlign._LAZY:
+            if self.__dict__.get("_attrs") is None:
+                self.setPos(self.pos1, self.pos2, self.__dict__.get("_Cs"))
+            return self.__dict__["_attrs"][name]
+        raise AttributeError(name)
 
     def setks(self):
         """k-cube and FFT shape (``periodicAlignment.py:384-394``)."""
@@ -99,9 +115,9 @@ class PeriodicAlign(BasePeriodicAlignment):
             raise ValueError("fshape must be cubic")
         return F
 
-    def _batch(self, flags):
-        return PeriodicBatch([self.Natoms], self.boxvec, [self.scale], self.n, self._F(),
-                             [list(self.perm)], 10, flags)
+    def _batch(self, flags, P=1):
+        return PeriodicBatch([self.Natoms] * P, self.boxvec, [self.scale] * P, self.n,
+                             self._F(), [list(self.perm)] * P, 10, flags)
 
     def calcFourierCoeff(self, pos, out=None):
         """C[p][k] = sum_{j in p} exp(-i k.r_j) (``:400-406``), on the device."""
@@ -115,46 +131,67 @@ class PeriodicAlign(BasePeriodicAlignment):
             return out
         return C1
 
+    def _grid_batch(self, Cs, extra=0):
+        flags = _lib.PER_KEEP_GRID | extra | (_lib.PER_INPUT_COEFFS if Cs is not None
+                                              else _lib.PER_KEEP_COEFFS)
+        b = self._batch(flags)
+        b.set_positions(self.pos1, self.pos2)
+        if Cs is not None:
+            b.set_coeffs(Cs[0], Cs[1])
+        b.run(_lib.PSTAGE_SF | _lib.PSTAGE_GRID)
+        return b
+
     def setPos(self, pos1=None, pos2=None, Cs=None):
-        """Coefficients, overlap product and |f| (``:408-440``)."""
+        """Coefficients, overlap product, f and |f| (``:408-440``)."""
         if pos1 is not None:
-            self.pos1[:] = np.asanyarray(pos1)
+            self.pos1[:] = np.asanyarray(pos1).reshape(-1, 3)
         if pos2 is not None:
-            self.pos2[:] = np.asanyarray(pos2)
-        flags = _lib.PER_KEEP_GRID | (_lib.PER_INPUT_COEFFS if Cs is not None
-                                      else _lib.PER_KEEP_COEFFS)
-        with self._batch(flags) as b:
-            b.set_positions(self.pos1, self.pos2)
-            if Cs is not None:
-                b.set_coeffs(Cs[0], Cs[1])
-            b.run(_lib.PSTAGE_SF | _lib.PSTAGE_GRID)
+            self.pos2[:] = np.asanyarray(pos2).reshape(-1, 3)
+        with self._grid_batch(Cs, _lib.PER_KEEP_FFT) as b:
             if Cs is None:
-                self.C1, self.C2 = b.coeffs()
+                C1, C2 = b.coeffs()
             else:
-                self.C1, self.C2 = np.asarray(Cs[0]), np.asarray(Cs[1])
-            self.C = b.product()[0]
-            self.fabs = b.grid(0).astype(np.float64)
-            self.f = self.fabs
+                C1, C2 = np.asarray(Cs[0]), np.asarray(Cs[1])
+            a = dict(C1=C1, C2=C2, C=b.product()[0], fabs=b.grid(0).astype(np.float64),
+                     f=b.fft(0))
             pk = b.peaks()
-        self.Csum = (((self.C1 * self.C1.conj()).real + (self.C2 * self.C2.conj()).real)
+        a["Csum"] = (((C1 * C1.conj()).real + (C2 * C2.conj()).real)
                      * np.exp(-self.absks[None, :] ** 2 * (self.scale ** 2))).real.sum() \
             * 0.5 * self.factor
+        self._attrs = a
+        self._Cs = Cs
         self._peak = pk
         return pk
 
     def findDisps(self, pos1, pos2, Cs=None, npeaks=1, width=2):
-        """Displacement of the overlap maximum (``:442-456``)."""
-        if npeaks > 1:
-            raise NotImplementedError("npeaks > 1 (multi-peak Gaussian fits) is not built")
-        pk = self.setPos(pos1, pos2, Cs)
-        return pk["disp"][:1].copy()
+        """Displacement(s) of the overlap maximum (``:442-456``): the
+        interpolated maximum, or with ``npeaks > 1`` the centres of up to
+        npeaks Gaussian-fitted peaks of |f| (``utils.py:367-396``), times
+        box / F."""
+        self.pos1[:] = np.asanyarray(pos1).reshape(-1, 3)
+        self.pos2[:] = np.asanyarray(pos2).reshape(-1, 3)
+        if npeaks <= 1:
+            pk = self.setPos(None, None, Cs)
+            return pk["disp"][:1].copy()
+        with self._grid_batch(Cs) as b:
+            fit, cnt, _ = b.find_peaks(npeaks, width)
+            pk = b.peaks()
+        self._attrs = None
+        self._Cs = Cs
+        k = int(cnt[0])
+        if k == 0:
+            return pk["disp"][:1].copy()
+        return fit[0, :k, 8:11] * self.boxvec / np.array(self.fshape, float)
 
     def align(self, pos1, pos2, Cs=None, npeaks=1, width=2):
         """``:458-460``: returns (dist, X1, X2, perm, disp)."""
-        if npeaks > 1:
-            raise NotImplementedError("npeaks > 1 (multi-peak Gaussian fits) is not built")
         self.pos1[:] = np.asanyarray(pos1).reshape(-1, 3)
         self.pos2[:] = np.asanyarray(pos2).reshape(-1, 3)
+        self._attrs = None
+        self._Cs = Cs
+        if npeaks > 1:
+            disps = self.findDisps(self.pos1, self.pos2, Cs, npeaks, width)
+            return self.refine(self.pos1, self.pos2, disps)
         if Cs is not None:
             with self._batch(_lib.PER_INPUT_COEFFS) as b:
                 b.set_positions(self.pos1, self.pos2)
@@ -178,7 +215,7 @@ class PeriodicAlign(BasePeriodicAlignment):
         instances, an instance is not safe for concurrent calls."""
         key = (self.Natoms, tuple(np.asarray(self.boxvec, float).ravel()), float(self.scale),
                int(self.n), self._F(), tuple(tuple(int(i) for i in g) for g in self.perm))
-        cached = getattr(self, "_plain", None)
+        cached = self.__dict__.get("_plain")
         if cached is not None and cached[0] == key:
             return cached[1], True
         if cached is not None:
@@ -205,22 +242,39 @@ class PeriodicAlign(BasePeriodicAlignment):
         return (float(r["dist"][0]), r["X1"][0], r["X2"][0], r["perm"][0], r["disp"][0])
 
     def alignGroup(self, coords, keepCoords=False, npeaks=1, width=2):
-        """All-pairs alignment (``:462-479``) as one device batch."""
-        if npeaks > 1:
-            raise NotImplementedError("npeaks > 1 (multi-peak Gaussian fits) is not built")
+        """All-pairs alignment (``:462-479``).  The structure factors are
+        computed once per structure (one batch over the structures) and
+        shared by all n^2 pairs, which then run as one device batch."""
         coords = [np.asarray(c, float).reshape(-1, 3) for c in coords]
         n = len(coords)
+        with self._batch(_lib.PER_KEEP_COEFFS, n) as cb:
+            cb.set_positions(coords, coords)
+            cb.run(_lib.PSTAGE_SF)
+            C1, _ = cb.coeffs()
+        ng = len(self.perm)
+        Cs = C1.reshape((n, ng) + C1.shape[1:])
+        dists = np.zeros((n, n))
+        if keepCoords:
+            aligned = np.empty((2, n, n, self.Natoms, self.dim))
+        if npeaks > 1:
+            for i in range(n):
+                for j in range(n):
+                    d, x1, x2 = self.align(coords[i], coords[j], [Cs[i], Cs[j]], npeaks,
+                                           width)[:3]
+                    dists[i, j] = d
+                    if keepCoords:
+                        aligned[0, i, j], aligned[1, i, j] = x1, x2
+            return (dists, aligned) if keepCoords else dists
         P = n * n
-        p1 = [coords[i] for i in range(n) for j in range(n)]
-        p2 = [coords[j] for i in range(n) for j in range(n)]
-        with PeriodicBatch([self.Natoms] * P, self.boxvec, [self.scale] * P, self.n, self._F(),
-                           [list(self.perm)] * P, 10, 0) as b:
-            b.set_positions(p1, p2)
+        i1 = np.repeat(np.arange(n), n)
+        i2 = np.tile(np.arange(n), n)
+        with self._batch(_lib.PER_INPUT_COEFFS, P) as b:
+            b.set_positions([coords[i] for i in i1], [coords[j] for j in i2])
+            b.set_coeffs_indexed(Cs, i1, i2)
             b.run(_lib.PSTAGE_ALL)
             r = b.results()
         dists = r["dist"].reshape(n, n)
         if keepCoords:
-            aligned = np.empty((2, n, n, self.Natoms, self.dim))
             for k in range(P):
                 aligned[0, k // n, k % n] = r["X1"][k]
                 aligned[1, k // n, k % n] = r["X2"][k]

@@ -14,7 +14,7 @@ import numpy as np
 from numpy import pi
 
 from . import _lib
-from .batch import SphericalBatch, dense_to_compact
+from .batch import SphericalBatch, compact_to_dense, dense_to_compact, kcount
 
 
 def hermitian_parts(flmm):
@@ -86,9 +86,17 @@ class SOFT:
         return g[0] + 1j * g[1]
 
     def SOFT(self, data):
-        """Forward SOFT is not part of the alignment hot path (the reference
-        uses it only as the inverse's counterpart, ``fastoverlap/soft.py:98-113``)."""
-        raise NotImplementedError("forward SOFT is not provided by the GPU build")
+        """Forward SOFT (``fastoverlap/soft.py:98-113``) on the device, FP64:
+        2D DFT over alpha and gamma per beta plane, then the quadrature-
+        weighted Wigner-d contraction over beta.  Returns (bw, 2bw-1, 2bw-1)."""
+        bw = self.bw
+        data = np.ascontiguousarray(np.asarray(data, np.complex128))
+        assert all(n == self.n for n in data.shape)
+        out = np.zeros(kcount(bw - 1), np.complex128)
+        ctx = _lib.default_context()
+        _lib.call("fo_soft_forward", ctx.handle, bw, data.ctypes.data_as(_lib._pd),
+                  out.ctypes.data_as(_lib._pd))
+        return compact_to_dense(out, bw - 1)
 
     def indtoEuler(self, ind):
         """Grid index -> Euler angles (``fastoverlap/soft.py:127-130``)."""

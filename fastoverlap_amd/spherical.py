@@ -12,10 +12,8 @@ Differences from the reference, by design:
  * the rotation refinement is a damped Newton iteration with the exact
    Hessian instead of scipy L-BFGS-B; both converge to the same local
    optimum, and only the permutation / RMSD downstream of it is contracted;
- * ``malign``/``findRotations`` (multi-peak Gaussian fits) are not built:
-   ``__call__`` invokes ``malign`` inside the same bare ``try/except`` as the
-   reference, so a triggered gate returns the ``align`` result, exactly as
-   the reference does when ``malign`` raises;
+ * ``malign`` refines the inverted orientation's candidates from the
+   inverted structure (the reference passes the non-inverted X2 there);
  * an atom at the centroid follows the i_0(0) = 1 limit instead of NaN.
 """
 
@@ -26,8 +24,8 @@ from numpy.linalg import norm
 
 from . import _lib
 from .batch import (SphericalBatch, compact_to_dense, dense_to_compact, kcount, next_pow2)
-from .soft import SOFT
-from .utils import EulerM
+from .soft import SOFT, hermitian_parts
+from .utils import EulerM, indtoEuler as _ind_to_euler
 
 
 class _OptResult(dict):
@@ -236,10 +234,92 @@ class BaseSphericalAlignment:
         return o["R"][0, 0], float(o["fun"][0, 0])
 
     def findRotations(self, Ilmm, nrot=10, width=2):
-        raise NotImplementedError("multi-peak Gaussian fitting (findPeaks) is not built")
+        """Euler angles of up to ``nrot`` Gaussian-fitted peaks of the real
+        overlap grid of ``Ilmm`` (``sphericalAlignment.py:196-205``,
+        ``utils.py:367-396``).  Returns (Rs, amplitude, mean, sigma, f) like
+        the reference: the fitted peaks' heights, backgrounds and widths
+        (2 s)^-1/2 of the six quadratic-form entries, and the residual grid."""
+        Ilmm = np.asarray(Ilmm, np.complex128)
+        L = Ilmm.shape[0] - 1
+        n = max(self._grid_n(), 2 * (L + 1))
+        H1, H2 = hermitian_parts(Ilmm)
+        with SphericalBatch([1], [L], [1.0], n, None,
+                            _lib.SPH_GENERAL | _lib.SPH_KEEP_GRID) as b:
+            # the real part of iSOFT(f) is the grid of the Hermitian part H1
+            b.set_coeffs(dense_to_compact(H1), dense_to_compact(H2))
+            b.run(_lib.STAGE_GRID)
+            fit, cnt, res = b.find_peaks(nrot, width, residual=True)
+            pk = b.peaks()
+            R0 = b.start()
+        k = int(cnt[0, 0])
+        if k == 0:
+            # no fit succeeded: the interpolated maximum (utils.py:389-394)
+            R = np.atleast_2d(R0[0, 0])
+            return R, [float(pk["value"][0, 0])], [0.0], [np.nan], res[0, 0]
+        f = fit[0, 0, :k]
+        Rs = np.atleast_2d(_ind_to_euler(f[:, 8:11], n))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            sig = list((2 * f[:, 2:8]) ** -0.5)
+        return Rs, list(f[:, 0]), list(f[:, 1]), sig, res[0, 0]
 
     def malign(self, pos1, pos2, perm=None, invert=True, calcCoeffs=None, nrot=10):
-        raise NotImplementedError("multi-peak alignment (malign) is not built")
+        """Multi-peak alignment (``sphericalAlignment.py:206-235``): refine
+        from every Gaussian-fitted peak of each orientation's grid (LAP +
+        quaternion fit, no local maximisation) and keep the smallest
+        distance.  Grids, peak fits and refinements run on the device.  The
+        inverted orientation refines the inverted structure (the reference
+        passes the non-inverted X2 to its inverted candidates)."""
+        pos1 = np.asanyarray(pos1, float).reshape(-1, 3)
+        pos2 = np.asanyarray(pos2, float).reshape(-1, 3)
+        if perm is None:
+            perm = self.perm
+        groups = [_groups_from_perm(perm, len(pos1))]
+        L = self.Jmax
+        n = self._grid_n()
+        if self.calcScale:
+            self.scale = (self.averageSeparation(pos1) + self.averageSeparation(pos2)) / 6
+        scale = float(self.scale)
+        if calcCoeffs is None:
+            flags = _lib.SPH_KEEP_GRID | (0 if invert else _lib.SPH_NO_INVERT)
+            with SphericalBatch([len(pos1)], [L], [scale], n, groups, flags) as b:
+                b.set_positions(pos1, pos2)
+                b.run(_lib.STAGE_PREP | _lib.STAGE_COEFF | _lib.STAGE_GRID)
+                fit, cnt, _ = b.find_peaks(nrot, 2)
+                R0 = b.start()
+        else:
+            A = dense_to_compact(np.asarray(calcCoeffs(False)))
+            Bc = dense_to_compact(np.asarray(calcCoeffs(True))) if invert else None
+            flags = _lib.SPH_KEEP_GRID | (_lib.SPH_GENERAL if invert else _lib.SPH_NO_INVERT)
+            with SphericalBatch([len(pos1)], [L], [scale], n, groups, flags) as b:
+                b.set_positions(pos1, pos2)
+                b.set_coeffs(A, Bc)
+                b.run(_lib.STAGE_GRID)
+                fit, cnt, _ = b.find_peaks(nrot, 2)
+                R0 = b.start()
+        # candidate rotations per orientation (the interpolated maximum when
+        # no fit succeeded, as findPeaks falls back to it)
+        cands = []
+        for o in range(fit.shape[1]):
+            k = int(cnt[0, o])
+            Rs = _ind_to_euler(fit[0, o, :k, 8:11], n) if k else R0[0, o][None, :]
+            cands += [(o, np.asarray(R, float)) for R in np.atleast_2d(Rs)]
+        P = len(cands)
+        p2 = [(-pos2 if o == 1 else pos2) for o, _ in cands]
+        with SphericalBatch([len(pos1)] * P, [0] * P, [scale] * P, 2, groups * P,
+                            _lib.SPH_NO_INVERT) as fb:
+            fb.set_positions([pos1] * P, p2)
+            fb.set_rotation(np.array([R for _, R in cands]))
+            fb.run(_lib.STAGE_PREP | _lib.STAGE_FINISH)
+            r = fb.results()
+        # first smallest within an orientation; the inverted orientation
+        # replaces only on a strictly smaller distance
+        best = 0
+        for i in range(1, P):
+            if r["dist"][i] < r["dist"][best]:
+                best = i
+        self.last_malign = dict(candidates=cands, dist=r["dist"].copy(), chosen=best,
+                                inverted=cands[best][0] == 1, perm=r["perm"][best])
+        return float(r["dist"][best]), r["X1"][best], r["X2"][best]
 
     def __call__(self, pos1, pos2, perm=None, invert=True, calcCoeffs=None, nrot=10, niter=None):
         """``sphericalAlignment.py:237-247``."""

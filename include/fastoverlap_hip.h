@@ -70,6 +70,7 @@ int fo_context_memory(fo_context* ctx, unsigned long long* used, unsigned long l
 #define FO_SPH_NO_CENTER 4     /* positions are used as given */
 #define FO_SPH_GENERAL 8       /* two independent Hermitian coefficient sets */
 #define FO_SPH_DFT 16          /* direct DFT planes even when grid_n is a power of two */
+#define FO_SPH_AUTOTUNE 32     /* choose launch shapes by timing on the first full run */
 
 /* stages for fo_sph_batch_run (bit mask) */
 #define FO_STAGE_PREP 1        /* centroids, kernel width */
@@ -135,6 +136,27 @@ int fo_sph_batch_norient(const fo_sph_batch* b);
 /* kernel times (ms) of the last run: [prep, ylm, radial, contract, plane,
  * select, rotopt, finish]; requires timing enabled */
 int fo_sph_batch_timing(fo_sph_batch* b, int enable, float* ms8);
+/* Launch shapes of the batch (no counterpart in the reference: the
+ * project's autotuner): params[0] = pairs per XCD chunk of the plane kernel,
+ * params[1] = highest degree contracted with 16-row tiles (-1: none),
+ * params[2] = radial workgroups per pair (0: default).  set = 0 reads them
+ * (after an FO_SPH_AUTOTUNE batch's first full run: the tuned ones), set = 1
+ * imposes them.  Every shape gives identical results. */
+int fo_sph_batch_tuning(fo_sph_batch* b, int* params, int set);
+/* Grid peak start angles per (pair, orientation): the interpolated maximum
+ * as Euler angles, R0[npairs*norient*3] (fastoverlap/utils.py:319-338 +
+ * fastoverlap/soft.py:127-130). */
+int fo_sph_batch_start(fo_sph_batch* b, double* R0);
+/* Multi-peak search on the kept grids (FO_SPH_KEEP_GRID, after
+ * FO_STAGE_GRID): findPeaks of fastoverlap/utils.py:367-396 as used by
+ * findRotations / malign (fastoverlap/sphericalAlignment.py:196-235).  Per
+ * (pair, orientation) up to npeaks (<= 64) subtractive Gaussian fits on
+ * (2*width+1)^3 windows (width <= 3).  fit[npairs*norient*npeaks*11]: A,
+ * mu, s00, s01, s02, s11, s12, s22, centre (3 fractional grid indices);
+ * count[npairs*norient] successful fits (0: fall back to the interpolated
+ * maximum); residual (may be NULL): the grids after the subtractions. */
+int fo_sph_batch_find_peaks(fo_sph_batch* b, int npeaks, int width, double* fit, int* count,
+                            float* residual);
 
 /* Build and keep resident the tables of grid side grid_n (Wigner-d table,
  * twiddles, recurrence coefficients) -- the tables fastoverlap/soft.py:48-96
@@ -172,6 +194,11 @@ int fo_sph_wigner_D(fo_context* ctx, int L, const double* rot, double* D, double
 /* Wigner table of SOFT(bw) in the dense layout Ds[l, m1, m2, k] (FP64) */
 int fo_sph_wigner_table(fo_context* ctx, int bw, double* Ds);
 
+/* SOFT.SOFT(data): forward SO(3) transform of a complex (2bw)^3 grid
+ * (axes alpha, beta, gamma; interleaved re, im) into compact coefficients
+ * l <= bw-1 (fastoverlap/soft.py:98-113), FP64. */
+int fo_soft_forward(fo_context* ctx, int bw, const double* data, double* flmm);
+
 /* ---- linear assignment -------------------------------------------------
  * metric 0: squared Euclidean; 1: Euclidean with minimum image in box[3].
  * perm[i] = row i's column (per group); dist = |X1 - X2[perm]| (periodic for
@@ -187,6 +214,7 @@ int fo_lap_align(fo_context* ctx, int natoms, const double* X1, const double* X2
 #define FO_PER_KEEP_GRID 2      /* keep |f| (FP32) */
 #define FO_PER_INPUT_COEFFS 4   /* coefficients supplied (quickAlign path) */
 #define FO_PER_KEEP_COEFFS 8    /* keep per-group structure factors */
+#define FO_PER_KEEP_FFT 16      /* keep the complex transform f (FP32) */
 
 #define FO_PSTAGE_SF 1          /* structure factors + weighted product */
 #define FO_PSTAGE_GRID 2        /* 3D DFT planes + exact FP64 argmax */
@@ -227,6 +255,22 @@ int fo_per_batch_coeffs(fo_per_batch* b, double* C1, double* C2);
 int fo_per_batch_product(fo_per_batch* b, double* C);
 int fo_per_batch_grid(fo_per_batch* b, int pair, float* fabs_grid);
 int fo_per_batch_timing(fo_per_batch* b, int enable, float* ms4);
+/* complex f = fftn(C, (F,F,F)) of one pair, FP32 interleaved (re, im)
+ * (FO_PER_KEEP_FFT; fastoverlap/periodicAlignment.py:439) */
+int fo_per_batch_fft(fo_per_batch* b, int pair, float* f);
+/* Per-structure coefficients shared by the pairs (FO_PER_INPUT_COEFFS):
+ * Cs[nstruct][ngroups][2nk+1]^3 complex; pair p uses structure idx1[p] as
+ * its first and idx2[p] as its second structure (every pair has the same
+ * groups) -- the coefficient reuse of alignGroup,
+ * fastoverlap/periodicAlignment.py:462-473. */
+int fo_per_batch_set_coeffs_indexed(fo_per_batch* b, const double* Cs, int nstruct,
+                                    const int* idx1, const int* idx2);
+/* Multi-peak search on the kept |f| grids (FO_PER_KEEP_GRID, after
+ * FO_PSTAGE_GRID): findDisps with npeaks > 1
+ * (fastoverlap/periodicAlignment.py:442-456); layout as
+ * fo_sph_batch_find_peaks with one grid per pair. */
+int fo_per_batch_find_peaks(fo_per_batch* b, int npeaks, int width, double* fit, int* count,
+                            float* residual);
 
 /* ---- serving --------------------------------------------------------------
  * A continuous-batching server: requests are queued by shape class, admitted

@@ -681,6 +681,126 @@ def spherical_align(pos1, pos2, Jmax, scale=None, perm=None, invert=True,
 
 
 # ---------------------------------------------------------------------------
+# Multi-peak search   (fastoverlap/utils.py:347-396,
+#                      fastoverlap/sphericalAlignment.py:196-235)
+# ---------------------------------------------------------------------------
+
+
+def gaussian_nd(x, A, mu, *alphax0):
+    """A exp(-sum_{i<=j} s_ij d_i d_j) + mu with d = x - x0
+    (``fastoverlap/utils.py:347-354``): ``alphax0`` holds the upper triangle
+    of s (row-major) followed by the centre x0; ``x`` is (dim, K)."""
+    x = np.atleast_2d(x)
+    dim = len(x)
+    sig = np.zeros((dim, dim))
+    sig[np.triu_indices(dim)] = alphax0[:-dim]
+    d = x - np.asarray(alphax0[-dim:], float)[:, None]
+    return A * np.exp(-np.einsum("ik,jk,ij->k", d, d, sig)) + mu
+
+
+def fit_peak(f, ind, n=2):
+    """Least-squares Gaussian on the (2n+1)^dim block around ``ind``, indices
+    wrapped periodically (``fastoverlap/utils.py:356-365``); scipy curve_fit
+    from A = peak value, mu = 0, s = identity, x0 = 0."""
+    from scipy.optimize import curve_fit
+    dim = f.ndim
+    block = f[np.ix_(*[np.arange(i - n, i + n + 1) % sz for i, sz in zip(ind, f.shape)])]
+    pts = np.indices((2 * n + 1,) * dim).reshape((dim, -1)) - n
+    p0 = ([block[(n,) * dim], 0.0]
+          + [1.0 if i == j else 0.0 for i in range(dim) for j in range(i, dim)]
+          + [0.0] * dim)
+    popt, pcov = curve_fit(gaussian_nd, pts, block.ravel(), p0=p0)
+    return popt, pcov
+
+
+def find_peaks(a, npeaks=10, width=2):
+    """Subtractive Gaussian peak search (``fastoverlap/utils.py:367-396``):
+    from f = a - min(a), repeat npeaks times: argmax (first occurrence), fit
+    a Gaussian around it, record its centre, subtract the fitted Gaussian
+    (background mu included, no wrap) from the whole array; stop at the
+    first failed fit.  Without any peak, the interpolated maximum.
+    Returns (peaks (k, dim) fractional indices, amplitudes, backgrounds,
+    widths, residual array)."""
+    f = np.array(a, dtype=float)
+    f -= f.min()
+    dim = f.ndim
+    idx = np.indices(f.shape).reshape((dim, -1))
+    peaks, amp, mean, sigma = [], [], [], []
+    for _ in range(npeaks):
+        ind = np.unravel_index(f.argmax(), f.shape)
+        try:
+            popt = fit_peak(f, ind, width)[0]
+        except (RuntimeError, ValueError):
+            break
+        peaks.append(popt[-dim:] + np.asarray(ind))
+        amp.append(popt[0])
+        mean.append(popt[1])
+        with np.errstate(divide="ignore", invalid="ignore"):
+            sigma.append((2 * popt[2:-dim]) ** -0.5)
+        popt[-dim:] += np.asarray(ind)
+        f[tuple(idx)] -= gaussian_nd(idx, *popt)
+    peaks = np.array(peaks)
+    if len(peaks) == 0:
+        peaks = find_max(a)[1][None, :]
+        amp.append(np.max(a))
+        mean.append(0.0)
+        sigma.append(np.nan)
+    return peaks, amp, mean, sigma, f
+
+
+def find_rotations(soft, Ilmm, nrot=10, width=2):
+    """``fastoverlap/sphericalAlignment.py:196-205``: Euler angles of the
+    peaks of the real overlap grid (window widened until a fit succeeds)."""
+    overlap = soft.iSOFT(Ilmm).real
+    peaks = []
+    while len(peaks) == 0:
+        peaks, amp, mean, sigma, f = find_peaks(overlap, nrot, width)
+        width += 1
+    return np.atleast_2d(soft.indtoEuler(peaks)), peaks
+
+
+def refine_rotation(X1, X2, R, perm):
+    """``fastoverlap/sphericalAlignment.py:118-127``: rotate X2 by
+    EulerM(R), per-group LAP, quaternion fit.  Returns (dist, X2 aligned,
+    permutation)."""
+    X2R = X2.dot(euler_matrix(*R))
+    _, p = find_best_permutation(X1, X2R, perm)
+    dist, MR = kabsch_quaternion(X1, X2R[p])
+    return dist, X2R[p].dot(MR.T), p
+
+
+def spherical_malign(pos1, pos2, Jmax, scale, perm=None, invert=True, bw=None, nrot=10,
+                     soft=None, fast=True):
+    """``BaseSphericalAlignment.malign`` (``sphericalAlignment.py:206-235``):
+    refine from every peak of each orientation's grid, keep the smallest
+    distance.  The inverted branch refines -X2 (the intended semantics; the
+    reference passes X2 there).  Returns (dist, X1, X2, perm, inverted,
+    peaks per orientation)."""
+    pos1 = np.asanyarray(pos1, float).reshape(-1, 3)
+    pos2 = np.asanyarray(pos2, float).reshape(-1, 3)
+    if perm is None:
+        perm = [np.arange(len(pos1))]
+    bw = Jmax + 1 if bw is None else bw
+    soft = SOFTOracle(bw) if soft is None else soft
+    X1 = pos1 - pos1.mean(0)
+    X2 = pos2 - pos2.mean(0)
+    best, allpeaks = None, []
+    for inv in ([False, True] if invert else [False]):
+        X2o = -X2 if inv else X2
+        I = so3_coeffs_species(X1, X2o, perm, Jmax, scale, fast)
+        Ip = I if bw == Jmax + 1 else pad_coeffs(I, bw)
+        Rs, peaks = find_rotations(soft, Ip, nrot)
+        allpeaks.append(peaks)
+        for R in Rs:
+            d, X2a, p = refine_rotation(X1, X2o, R, perm)
+            # the inverted result replaces only on a strictly smaller distance
+            if best is None or d < best[0]:
+                best = (d, X2a, p, inv)
+    d, X2a, p, inv = best
+    return d, X1, X2a, p, inv, allpeaks
+
+
+# ---------------------------------------------------------------------------
 # Periodic alignment                (fastoverlap/periodicAlignment.py)
 # ---------------------------------------------------------------------------
 
@@ -786,14 +906,23 @@ class PeriodicOracle:
         dist = np.linalg.norm(self.periodic(pos1 - pos2))
         return dist, pos1, pos2, perm, disp
 
-    def align(self, pos1, pos2, Cs=None):
-        """``periodicAlignment.py:442-460`` (npeaks=1)."""
+    def find_disps(self, fabs, npeaks=1, width=2):
+        """``periodicAlignment.py:442-456``: the interpolated maximum
+        (npeaks = 1) or the Gaussian-fitted peaks of |f|, times box/F."""
+        shape = np.array(fabs.shape)
+        if npeaks > 1:
+            disps = find_peaks(fabs, npeaks, width)[0]
+            return np.atleast_2d(disps) * self.boxvec / shape
+        return (find_max(fabs)[1] * self.boxvec / shape)[None, :]
+
+    def align(self, pos1, pos2, Cs=None, npeaks=1, width=2):
+        """``periodicAlignment.py:442-460``."""
         pos1 = np.asanyarray(pos1, float).reshape(-1, 3)
         pos2 = np.asanyarray(pos2, float).reshape(-1, 3)
         C, fabs = self.overlap_grid(pos1, pos2, Cs)
-        ind, find = find_max(fabs)
-        disp = find * self.boxvec / np.array(fabs.shape)
-        out = self.refine(pos1, pos2, disp[None, :])
+        ind, _ = find_max(fabs)
+        disps = self.find_disps(fabs, npeaks, width)
+        out = self.refine(pos1, pos2, disps)
         return out, tuple(int(i) for i in ind), fabs
 
 
@@ -818,4 +947,9 @@ PINNED = {
     "PeriodicOracle.overlap_grid": "peak at the known translation; direct DFT",
     "PeriodicOracle.hungarian": "brute force (two groups); exact shifted copy; BLJ256 1.559",
     "PeriodicOracle.refine": "exact translation+permutation -> 0; BLJ256 1.559",
+    "gaussian_nd": "closed form at known points (upper-triangle quadratic form)",
+    "find_peaks": "sums of known Gaussians: every centre, amplitude and the "
+                  "argmax sequence recovered; brute-force argmax on small grids",
+    "spherical_malign": "exact rotation -> 0; agrees with align on easy pairs",
+    "PeriodicOracle.find_disps": "known translation recovered with npeaks > 1",
 }

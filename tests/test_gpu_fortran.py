@@ -1,5 +1,7 @@
 """The Fortran caller (iso_c_binding modules, built in-tree by
-fastoverlap_amd/build.py) reproduces the Python drop-in results."""
+fastoverlap_amd/build.py) against the FP64 oracle: distance within 1e-8 and
+the 1-based BESTPERM equal to the oracle's permutation, for the cluster
+(LJ38) and the bulk (BLJ256) ALIGN."""
 
 import os
 import subprocess
@@ -38,5 +40,10 @@ def test_fortran_align_matches_dropin(gpu, lj38, blj256):
     q1, q2 = blj256
     box = np.ones(3) * 5.975206329
     groups = [np.arange(204), np.arange(204, 256)]
-    dist = fo.PeriodicAlign(256, box, groups)(q1, q2)[0]
-    assert abs(bdist - dist) <= 1e-8
+    (odist, _, _, operm, _), _, _ = O.PeriodicOracle(256, box, groups).align(q1, q2)
+    assert bdist == pytest.approx(1.559, abs=1e-3)
+    assert abs(bdist - odist) <= 1e-8
+    bperm = np.array([int(v) for v in out["BPERM"]]) - 1
+    np.testing.assert_array_equal(bperm, operm)
+    # and the Python drop-in agrees with both
+    assert abs(fo.PeriodicAlign(256, box, groups)(q1, q2)[0] - bdist) <= 1e-8

@@ -49,9 +49,14 @@ def test_blj256_known_answer_dropin(gpu, blj256):
     np.testing.assert_array_equal(perm, operm)
     assert tuple(pa.last["peaks"]["ind"][0]) == oind
     np.testing.assert_allclose(X2, oX2, atol=1e-9)
-    # |f| grid of the drop-in's setPos
-    pa.setPos(p1, p2)
+    # the attributes after a plain call describe that call's configuration:
+    # |f|, the complex transform f (phase included) and the product C
     assert np.abs(pa.fabs - ofabs).max() <= GRID_TOL * ofabs.max()
+    C, _ = P.overlap_grid(p1, p2)
+    f = np.fft.fftn(C, P.fshape, axes=(0, 1, 2))
+    assert np.abs(pa.f - f).max() <= GRID_TOL * np.abs(f).max()
+    np.testing.assert_allclose(pa.C, C, atol=1e-11 * np.abs(C).max())
+    np.testing.assert_allclose(np.abs(pa.f), pa.fabs, rtol=1e-5, atol=1e-6 * ofabs.max())
 
 
 def test_c1_parity_64_grid(gpu):

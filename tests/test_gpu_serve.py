@@ -86,3 +86,47 @@ def test_server_rejects_bad_request_without_stopping(gpu):
         good = srv.align(dict(kind=0, pos1=c["pos1"], pos2=c["pos2"], groups=c["groups"],
                               L=c["L"], grid_n=c["grid_n"], scale=0.3))
         assert good["status"] == 0
+
+
+def test_server_admits_by_hbm_footprint(gpu):
+    """Admission by device footprint.  The budget holds about three of the
+    burst's largest requests, far below the burst's total: the bytes the
+    server's arenas hold never exceed it, requests wait for memory (several
+    smaller batches instead of one), every result equals the request served
+    alone, and a request larger than the whole budget fails with
+    FO_ERR_NOMEM while the server keeps serving."""
+    from fastoverlap_amd.batch import SphericalBatch
+    pairs = [W.config2(p) for p in range(60)]
+    sel = [c for c in pairs if c["grid_n"] == 64][:24]
+    reqs = [dict(kind=0, pos1=c["pos1"], pos2=c["pos2"], groups=c["groups"], L=c["L"],
+                 grid_n=64, scale=0.3) for c in sel]
+    foot = []
+    for c in sel:
+        with SphericalBatch([len(c["pos1"])], [c["L"]], 0.3, 64, [c["groups"]]) as b:
+            foot.append(b.footprint())
+    budget = 3 * max(foot)
+    assert budget < sum(foot) / 2
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(1000, 3)) * 3.0
+    huge = dict(kind=0, pos1=X, pos2=X, L=30, grid_n=64, scale=0.3)
+    say(f"case: server budget {budget} B ({len(reqs)} requests, footprints "
+        f"{min(foot)}..{max(foot)} B, total {sum(foot)} B) + one 1000-atom request")
+    with Server(gpu, max_batch=64, streams=3, max_wait_ms=2.0, hbm_budget=budget) as srv:
+        ids = [srv.submit(r) for r in reqs[:12]]
+        big = srv.submit(huge)
+        ids += [srv.submit(r) for r in reqs[12:]]
+        mixed = [srv.wait(i, timeout_ms=120000.0) for i in ids]
+        hb = srv.wait(big, timeout_ms=120000.0)
+        after = srv.align(reqs[0])
+        st = srv.stats()
+        alone = [srv.align(r) for r in reqs]
+    assert hb["status"] == 3, hb["status"]          # FO_ERR_NOMEM
+    assert after["status"] == 0
+    assert st["peak_bytes"] <= budget, (st["peak_bytes"], budget)
+    assert max(m["batch_size"] for m in mixed) < len(reqs)
+    assert st["batches"] >= 3
+    for k, (m, a) in enumerate(zip(mixed, alone)):
+        assert m["status"] == 0 and a["status"] == 0, k
+        assert m["dist"] == a["dist"], k
+        np.testing.assert_array_equal(m["perm"], a["perm"])
+        np.testing.assert_array_equal(m["ind"], a["ind"])

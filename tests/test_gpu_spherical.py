@@ -73,6 +73,26 @@ def test_isoft_general_matches_oracle(gpu):
     assert np.abs(g - ref).max() <= GRID_TOL * np.abs(ref).max()
 
 
+def test_soft_forward_matches_oracle_and_round_trips(gpu):
+    say("case: SOFT(8).SOFT random grid; SOFT(iSOFT(f)) round trip")
+    rng = np.random.default_rng(4)
+    bw = 8
+    n = 2 * bw
+    data = rng.normal(size=(n, n, n)) + 1j * rng.normal(size=(n, n, n))
+    got = fo.SOFT(bw).SOFT(data)
+    ref = O.SOFTOracle(bw).SOFT(data)
+    np.testing.assert_allclose(got, ref, atol=1e-12 * np.abs(ref).max())
+    f = np.zeros((bw, 2 * bw - 1, 2 * bw - 1), complex)
+    m = 2 * bw - 1
+    for l in range(bw):
+        for m1 in range(-l, l + 1):
+            for m2 in range(-l, l + 1):
+                f[l, m1 % m, m2 % m] = rng.normal() + 1j * rng.normal()
+    soft = fo.SOFT(bw)
+    back = soft.SOFT(soft.iSOFT(f))   # the inverse runs in FP32
+    assert np.abs(back - f).max() <= 1e-5 * np.abs(f).max()
+
+
 def _oracle_pair(c, keep_grids=False):
     return O.spherical_align(c["pos1"], c["pos2"], c["L"], c["scale"], perm=c["groups"],
                              bw=c["grid_n"] // 2, fast=True, keep_grids=keep_grids)
@@ -421,3 +441,35 @@ def test_invalid_input_is_an_error_not_a_crash(gpu):
         SphericalBatch([10], [70], 0.3, 256)     # Jmax beyond 63
     with pytest.raises(fo.FastOverlapError):
         SphericalBatch([5], [5], 0.3, 16, [[np.array([0, 1, 7])]])  # atom out of range
+
+
+def test_autotuner_shapes_give_identical_results(gpu):
+    """The autotuner (FO_SPH_AUTOTUNE) picks launch shapes by timing on the
+    batch's first full run; every shape it can pick computes the same
+    arithmetic, so the tuned batch and batches forced to the extreme shapes
+    give bit-identical coefficients, argmax, permutation and distance."""
+    pairs = [W.config2(p) for p in range(400, 440)]
+    sel = [c for c in pairs if c["grid_n"] == 64][:24]
+
+    def run(flags, shape=None):
+        with SphericalBatch([len(c["pos1"]) for c in sel], [c["L"] for c in sel], 0.3, 64,
+                            [c["groups"] for c in sel], flags) as b:
+            if shape is not None:
+                b.set_tuning(shape)
+            b.set_positions([c["pos1"] for c in sel], [c["pos2"] for c in sel])
+            b.run(graph=True)
+            b.run(graph=True)
+            return b.results(), b.peaks(), b.coeffs(), b.tuning()
+
+    say(f"case: autotuned batch of {len(sel)} C2 pairs n=64")
+    ref = run(0)
+    tuned = run(_lib.SPH_AUTOTUNE)
+    say(f"tuned shapes: {tuned[3]}")
+    outs = [tuned] + [run(0, sh) for sh in ((1, -1, 64), (32, 7, 4096), (2, 15, 256))]
+    for got in outs:
+        np.testing.assert_array_equal(got[0]["dist"], ref[0]["dist"])
+        np.testing.assert_array_equal(got[1]["ind"], ref[1]["ind"])
+        for a, e in zip(got[0]["perm"], ref[0]["perm"]):
+            np.testing.assert_array_equal(a, e)
+        for a, e in zip(got[2], ref[2]):
+            np.testing.assert_array_equal(a, e)

@@ -477,3 +477,86 @@ def test_periodic_hungarian_brute_force_and_semantics():
     Y = X1[sh] + box * np.array([1, -2, 0])
     _, p2 = P.hungarian(X1, Y)
     np.testing.assert_array_equal(sh[p2], np.arange(7))
+
+
+# ---- multi-peak search ----------------------------------------------------------------
+
+def test_gaussian_nd_closed_form():
+    A, mu = 2.5, 0.3
+    s = np.array([1.2, 0.1, -0.2, 0.8, 0.05, 1.5])      # s00 s01 s02 s11 s12 s22
+    c = np.array([0.4, -0.3, 0.2])
+    x = np.array([[0.0, 1.0, -1.0], [0.5, 0.0, 2.0], [1.0, -1.0, 0.0]])
+    d = x - c[:, None]
+    q = (s[0] * d[0] ** 2 + s[1] * d[0] * d[1] + s[2] * d[0] * d[2] + s[3] * d[1] ** 2
+         + s[4] * d[1] * d[2] + s[5] * d[2] ** 2)
+    np.testing.assert_allclose(O.gaussian_nd(x, A, mu, *s, *c), A * np.exp(-q) + mu,
+                               rtol=1e-15)
+
+
+def _gauss_grid(shape, comps, bg=0.0):
+    idx = np.indices(shape).reshape(3, -1).astype(float)
+    g = np.full(idx.shape[1], bg)
+    for A, s, c in comps:
+        g += O.gaussian_nd(idx, A, 0.0, *s, *c)
+    return g.reshape(shape)
+
+
+def test_find_peaks_recovers_known_gaussians():
+    # three well-separated anisotropic Gaussians: every centre and amplitude
+    # is recovered, in order of height (the argmax sequence of the residual)
+    comps = [(5.0, [0.9, 0.2, 0.0, 0.7, -0.1, 1.1], [6.3, 7.8, 5.2]),
+             (3.0, [0.6, 0.0, 0.1, 0.8, 0.0, 0.5], [17.6, 6.1, 16.4]),
+             (1.5, [1.0, -0.2, 0.0, 1.0, 0.1, 0.9], [8.2, 18.7, 17.9])]
+    a = _gauss_grid((24, 24, 24), comps, bg=0.25)
+    peaks, amp, mean, sigma, f = O.find_peaks(a, npeaks=3, width=2)
+    for k, (A, s, c) in enumerate(comps):
+        np.testing.assert_allclose(peaks[k], c, atol=1e-6)
+        assert amp[k] == pytest.approx(A, rel=1e-6)
+    # brute force: the first index of each fit is the argmax of what remains
+    assert np.unravel_index(a.argmax(), a.shape) == tuple(np.round(comps[0][2]).astype(int))
+    # the residual is flat once every component is removed
+    assert np.abs(f).max() < 1e-5 * a.max()
+
+
+def test_find_peaks_window_wraps_periodically():
+    comps = [(4.0, [0.8, 0.0, 0.0, 0.8, 0.0, 0.8], [0.3, 9.6, 12.0])]
+    shape = (16, 16, 16)
+    idx = np.indices(shape).reshape(3, -1).astype(float)
+    # periodic images so the grid itself is periodic around index 0
+    g = np.zeros(idx.shape[1])
+    for sh in (-16.0, 0.0, 16.0):
+        A, s, c = comps[0]
+        g += O.gaussian_nd(idx, A, 0.0, *s, *(np.array(c) + [sh, 0, 0]))
+    a = g.reshape(shape)
+    peaks = O.find_peaks(a, npeaks=1, width=2)[0]
+    np.testing.assert_allclose(peaks[0], comps[0][2], atol=1e-6)
+
+
+def test_malign_exact_rotation_and_agreement_with_align():
+    rng = np.random.default_rng(11)
+    X = rng.normal(size=(30, 3)) * 1.2
+    X -= X.mean(0)
+    Y = X.dot(O.euler_matrix(1.0, 0.7, 2.5).T)[rng.permutation(30)]
+    assert O.spherical_malign(X, Y, 12, 0.3, nrot=3)[0] < 1e-6
+    # an easy pair: the distance depends only on the permutation, which the
+    # peak fits and the refined grid maximum agree on
+    from fastoverlap_amd import workloads as W
+    c = W.config0()
+    m = O.spherical_malign(c["pos1"], c["pos2"], 15, 0.3, c["groups"], nrot=4)
+    a = O.spherical_align(c["pos1"], c["pos2"], 15, 0.3, perm=c["groups"], fast=True)
+    assert m[0] == pytest.approx(a.dist, abs=1e-10)
+    np.testing.assert_array_equal(m[3], a.perm)
+
+
+def test_periodic_find_disps_multi_peak_recovers_translation():
+    rng = np.random.default_rng(6)
+    box = np.full(3, 6.0)
+    X = rng.random((40, 3)) * box
+    shift = np.array([1.2, -2.1, 0.45])
+    perm = rng.permutation(40)
+    Y = (X + shift)[perm]
+    P = O.PeriodicOracle(40, box)
+    (d1, _, _, p1, disp1), _, _ = P.align(X, Y)
+    (d3, _, _, p3, disp3), _, _ = P.align(X, Y, npeaks=3)
+    assert d1 < 1e-9 and d3 < 1e-9
+    np.testing.assert_array_equal(perm[p3], np.arange(40))
