@@ -1,14 +1,15 @@
 """Benchmark: aligned cluster pairs/s of the batched GPU path on the
 4096-pair mixed workload, beside the plain per-call GPU drop-in, the FP64 CPU
-oracle, the periodic path on C1 pairs and, with --serve, a served Poisson
-stream (p50/p99 latency).
+oracle, the periodic path on C1 pairs, the large-degree C3 batch and a
+served Poisson stream through the continuous-batching server under a 32 GiB
+HBM admission budget (pairs/s, p50/p99 latency).
 
 One "step" = one pass of the full alignment (coefficients, inverse SOFT,
 exact argmax, rotation refinement, LAP, quaternion fit, both orientations)
 over the 4096 pairs, inputs resident in HBM (one hipGraph replay per grid
 class).  Prints ONE JSON line.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--serve]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--no-serve]
 """
 
 from __future__ import annotations
@@ -90,6 +91,110 @@ def make_batches(pairs, ctx, streams=1, flags=0):
                                [c["scale"] for c in sel], n, [c["groups"] for c in sel], flags,
                                ctx)
             out.append((n, sel, b))
+    return out
+
+
+def precision_report(batches):
+    """Per (pair, orientation): |largest reduced-precision grid value - FP64
+    value at the exact argmax| / |FP64 value| (the error stated per result),
+    and the bound the selection used, relative the same way."""
+    errs, bounds = [], []
+    for _, _, b in batches:
+        pr = b.precision()
+        v64 = np.abs(pr["val64"])
+        errs.append((np.abs(pr["val32"] - pr["val64"]) / v64).ravel())
+        bounds.append((pr["delta"] / v64).ravel())
+    e, d = np.concatenate(errs), np.concatenate(bounds)
+    return {"max_rel_err_of_grid_max": float(e.max()),
+            "median_rel_err_of_grid_max": float(np.median(e)),
+            "max_rel_bound": float(d.max()), "median_rel_bound": float(np.median(d))}
+
+
+def timed_steps(batches, steps, warmup):
+    def step():
+        for _, _, b in batches:
+            b.run(graph=True, sync=False)
+        for _, _, b in batches:
+            b.sync()
+    for _ in range(warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def kernel_times(batches, steps):
+    kt = {}
+    for _, _, b in batches:
+        b.enable_timing(True)
+        for _ in range(steps):
+            b.run(graph=False)
+            for k, v in b.last_timing().items():
+                kt[k] = kt.get(k, 0.0) + v / steps
+        b.enable_timing(False)
+    return {k: round(v, 4) for k, v in kt.items()}
+
+
+def agreement(batches_a, batches_b):
+    """Pairs whose argmax (both orientations), permutation and orientation
+    are identical in two runs of the same pairs, and the largest distance
+    difference."""
+    same, tot, worst = 0, 0, 0.0
+    for (_, _, a), (_, _, b) in zip(batches_a, batches_b):
+        ra, rb, pa, pb = a.results(), b.results(), a.peaks(), b.peaks()
+        for i in range(a.P):
+            tot += 1
+            same += int(np.array_equal(pa["ind"][i], pb["ind"][i])
+                        and np.array_equal(ra["perm"][i], rb["perm"][i])
+                        and ra["inverted"][i] == rb["inverted"][i])
+            worst = max(worst, abs(ra["dist"][i] - rb["dist"][i]))
+    return f"{same}/{tot}", worst
+
+
+def beta_gemm_arm(pairs, ctx, steps, warmup, ref_batches):
+    """The split-BF16x3 matrix-core arm of the beta contraction on the same
+    pairs: step time, kernel times, per-result error and agreement with the
+    FP32 arm (argmax, permutation and distance are exact in both: the FP64
+    re-evaluation widens its window by the arm's error bound)."""
+    from fastoverlap_amd import _lib
+    batches = make_batches(pairs, ctx, 1, _lib.SPH_BETA_GEMM)
+    for n, sel, b in batches:
+        b.set_positions([c["pos1"] for c in sel], [c["pos2"] for c in sel])
+    ms = timed_steps(batches, steps, warmup)
+    kt = kernel_times(batches, steps)
+    same, worst = agreement(ref_batches, batches)
+    out = {"ms_per_step": ms, "pairs_per_s": len(pairs) / ms * 1e3, "kernel_ms": kt,
+           "identical_argmax_perm_vs_fp32_arm": same, "max_abs_dist_diff_vs_fp32_arm": worst,
+           "precision": precision_report(batches)}
+    for _, _, b in batches:
+        b.close()
+    return out
+
+
+def c3_supplement(c3pairs, ctx, steps):
+    """The large-degree workload (1000-atom pairs, Jmax 40, grid 128, width
+    0.2): both arms of the beta contraction, step time and per-result error,
+    and their agreement."""
+    from fastoverlap_amd import _lib
+    out, runs = {}, []
+    for arm, flags in (("fp32", 0), ("bf16x3", _lib.SPH_BETA_GEMM)):
+        batches = make_batches(c3pairs, ctx, 1, flags)
+        for n, sel, b in batches:
+            b.set_positions([c["pos1"] for c in sel], [c["pos2"] for c in sel])
+        ms = timed_steps(batches, steps, 1)
+        out[arm] = {"ms_per_step": ms, "pairs_per_s": len(c3pairs) / ms * 1e3,
+                    "kernel_ms": kernel_times(batches, 1),
+                    "precision": precision_report(batches)}
+        runs.append(batches)
+    same, worst = agreement(runs[0], runs[1])
+    out["identical_argmax_perm_between_arms"] = same
+    out["max_abs_dist_diff_between_arms"] = worst
+    for batches in runs:
+        for _, _, b in batches:
+            b.close()
+    out["workload"] = (f"C3: {len(c3pairs)} pairs, N=1000 (600:400), Jmax 40, grid 128, "
+                       f"sigma 0.2, radius 6.5")
     return out
 
 
@@ -263,8 +368,25 @@ def periodic_batched(ctx, pairs, steps, warmup, plain_sample):
         agree += int(np.array_equal(pa.last["peaks"]["ind"][0], pk["ind"][i])
                      and np.array_equal(perm, r["perm"][i]))
     plain = plain_sample / (time.perf_counter() - t0)
+    # algorithmic work: structure factors as separable phase products, 8 flops
+    # per complex multiply-add, (2n+1)^3 k per atom, both structures (FP64);
+    # the pruned transform: F planes of (2n+1)^3 + (2n+1)^2 F + (2n+1) F^2
+    # complex multiply-adds (FP32)
+    NB = 2 * c0["nk"] + 1
+    F = c0["F"]
+    f_sf = sum(2 * 8 * NB ** 3 * len(c["pos1"]) for c in pairs)
+    f_plane = P * 8 * F * (NB ** 3 + NB * NB * F + NB * F * F)
+    roof = {}
+    for k, fl, peak, unit in (("sf", f_sf, FP64_PEAK, "FP64 vector ALU"),
+                              ("plane", f_plane, FP32_PEAK / 2,
+                               "FP32 vector ALU, single-issue (packed FP32 disabled)")):
+        if kt.get(k, 0) > 0:
+            ach = fl / (kt[k] * 1e-3)
+            roof[k] = {"achieved_tflops": ach / 1e12, "peak_tflops": peak / 1e12,
+                       "frac": ach / peak, "engine": unit}
     return {"workload": f"C1: {P} periodic pairs, N=256 (204:52), box 6.0, k-cube n=8, "
                         f"64^3 grid, sigma 0.3", "pairs": P, "ms_per_step": dt * 1e3,
+            "roofline": roof,
             "pairs_per_s": P / dt, "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
             "plain_pairs_per_s": plain, "speedup_vs_plain": (P / dt) / plain,
             "plain_identical_ind_perm": f"{agree}/{plain_sample}"}
@@ -284,15 +406,17 @@ def main():
     ap.add_argument("--gen-workers", type=int, default=None,
                     help="processes drawing the synthetic pairs (default: up to 16)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve", action="store_true",
-                    help="also run the served mix (off by default: a run of the full "
-                         "command that included it ended in an unexplained GPU fault)")
-    ap.add_argument("--no-serve", action="store_true", help="(default) skip the served mix")
+    ap.add_argument("--serve", action="store_true", help="(default) run the served mix")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served mix")
     ap.add_argument("--periodic-pairs", type=int, default=128,
                     help="C1 pairs of the periodic supplement (0: skip it)")
     ap.add_argument("--no-periodic", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="run the default launch shapes instead of the autotuned ones")
+    ap.add_argument("--c3-pairs", type=int, default=256,
+                    help="pairs of the large-degree supplement (0: skip it)")
+    ap.add_argument("--no-bf16", action="store_true",
+                    help="skip the split-BF16x3 arm of the beta contraction")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -305,11 +429,13 @@ def main():
     t = time.perf_counter()
     pairs = W.generate(W.config2, range(args.pairs), workers=args.gen_workers)
     # every workload is drawn (with forked workers) before the GPU is touched
-    run_serve = args.serve and not args.no_serve
+    run_serve = not args.no_serve and args.serve_requests > 0
     serve_reqs = ([] if not run_serve else
                   W.generate(W.config4, range(args.serve_requests), workers=args.gen_workers))
     nper = 0 if args.no_periodic else args.periodic_pairs
     per_pairs = W.generate(W.config1, range(nper), workers=args.gen_workers) if nper else []
+    c3_pairs = (W.generate(W.config3, range(args.c3_pairs), workers=args.gen_workers)
+                if args.c3_pairs else [])
     log(f"generated {len(pairs)} pairs, {len(per_pairs)} periodic pairs and "
         f"{len(serve_reqs)} served requests in "
         f"{time.perf_counter() - t:.1f}s")
@@ -454,11 +580,19 @@ def main():
         "plain_identical_argmax_perm": f"{agree}/{len(sample)}",
         "roofline": roof,
         "tuning": tuning,
+        "precision": {"fp32_arm": precision_report(batches)},
     }
+    if not args.no_bf16:
+        out["precision"]["bf16x3_arm"] = beta_gemm_arm(pairs, ctx, args.steps, args.warmup,
+                                                       batches)
+        log(f"bf16x3 arm: {out['precision']['bf16x3_arm']}")
     if per_pairs:
         out["periodic"] = periodic_batched(ctx, per_pairs, args.steps, args.warmup,
                                            min(4, len(per_pairs)))
         log(f"periodic: {out['periodic']}")
+    if c3_pairs:
+        out["c3"] = c3_supplement(c3_pairs, ctx, 2)
+        log(f"c3: {out['c3']}")
     if run_serve:
         out["serve"] = served(ctx, serve_reqs)
         log(f"served: {out['serve']}")

@@ -23,6 +23,7 @@ ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "unsuppo
 # spherical flags / stages
 SPH_NO_INVERT, SPH_KEEP_GRID, SPH_NO_CENTER, SPH_GENERAL, SPH_DFT, SPH_AUTOTUNE = \
     1, 2, 4, 8, 16, 32
+SPH_BETA_GEMM = 64
 STAGE_PREP, STAGE_COEFF, STAGE_GRID, STAGE_ROTOPT, STAGE_FINISH, STAGE_ALL = 1, 2, 4, 8, 16, 31
 # periodic flags / stages
 PER_KEEP_GRID, PER_INPUT_COEFFS, PER_KEEP_COEFFS, PER_KEEP_FFT = 2, 4, 8, 16
@@ -102,6 +103,7 @@ _PROTOS = {
     "fo_sph_batch_find_peaks": (C.c_int, [_vp, C.c_int, C.c_int, _pd, _pi, _pf]),
     "fo_sph_batch_start": (C.c_int, [_vp, _pd]),
     "fo_sph_batch_tuning": (C.c_int, [_vp, _pi, C.c_int]),
+    "fo_sph_batch_precision": (C.c_int, [_vp, _pd, _pd, _pd]),
     "fo_sph_harm": (C.c_int, [_vp, C.c_int, C.c_int, _pd, _pd, _pd]),
     "fo_sph_energy": (C.c_int, [_vp, C.c_int, _pd, _pd, _pd, _pd]),
     "fo_sph_wigner_D": (C.c_int, [_vp, C.c_int, _pd, _pd, _pd]),

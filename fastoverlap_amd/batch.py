@@ -192,6 +192,15 @@ class SphericalBatch:
                     margin=mar.reshape(sh), delta=dl.reshape(sh), ncand=nc.reshape(sh),
                     noverflow=no.reshape(sh))
 
+    def precision(self):
+        """Per (pair, orientation): the largest value of the reduced-precision
+        grid, the FP64 value at the exact argmax and the error bound used."""
+        m = self.P * self.norient
+        v32, v64, dl = np.zeros(m), np.zeros(m), np.zeros(m)
+        call("fo_sph_batch_precision", self.handle, dptr(v32), dptr(v64), dptr(dl))
+        sh = (self.P, self.norient)
+        return dict(val32=v32.reshape(sh), val64=v64.reshape(sh), delta=dl.reshape(sh))
+
     def tuning(self):
         """Launch shapes in use: (plane pairs per XCD chunk, highest degree
         contracted with 16-row tiles, radial workgroups per pair)."""

@@ -41,10 +41,12 @@ size_t isoft_plane_smem(int, int, int);
 int isoft_plane_use_fft(int, int, size_t);
 void launch_isoft_plane(hipStream_t, int, int, int, int, long long, const SphPair*, const float*,
                         const float2*, const float2*, const float2*, int, int, PlaneCand*, float*,
-                        double*, int);
+                        double*, int, const float4*, int);
+void launch_beta_gemm(hipStream_t, int, int, int, long long, int, const SphPair*, const float*,
+                      const float2*, float4*, int, double*);
 void launch_sph_select(hipStream_t, int, int, long long, int, const SphPair*, const double*,
                        const double2*, const double2*, const double2*, int, const PlaneCand*,
-                       double2*, long long, const double*, SphPeak*);
+                       double2*, long long, const double*, SphPeak*, double);
 void launch_sph_rotopt(hipStream_t, int, int, const SphPair*, const double2*, const double2*, int,
                        const SphPeak*, const double*, const double*, int, SphOpt*);
 void launch_rec_table(hipStream_t, int, double*, double*, double2*, double*);
@@ -486,11 +488,14 @@ struct fo_sph_batch {
   // layout (byte offsets inside base) and ownership
   struct {
     size_t pd, grp, gath, pos1, pos2, X1c, X2c, X2R, X2out, scale, R, Y1, Y2, I64A, I32A, I64B,
-        I32B, cand, abound, peaks, opt, res, perm, rowsol, scr, grid;
+        I32B, cand, abound, peaks, opt, res, perm, rowsol, scr, grid, S;
   } off{};
   int gsum = 0;
   bool own_mem = false, own_stream = false;
   PeakScratch pk;  // multi-peak search buffers (allocated on first use)
+  // FO_SPH_BETA_GEMM: even / odd-degree beta sums from k_beta_gemm
+  // (split-BF16x3 on the matrix cores), [pair][beta][m1 <= Lmax][entry]
+  float4* S = nullptr;
   // launch shapes (arithmetic-neutral): plane pairs per XCD chunk, highest
   // degree contracted with 16-row tiles (-1: none), radial workgroups per
   // pair; 0 = default.  autotune: pick them on the first full run.
@@ -535,13 +540,17 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
   mark(3);
   mark(4);
   if (stages & FO_STAGE_GRID) {
+    if (b->S)
+      launch_beta_gemm(s, P, L.n, L.Lmax, b->tab->Kp, L.Lt, b->pd, b->tab->tab32, b->I32A, b->S,
+                       L.norient, b->abound);
     launch_isoft_plane(s, P, L.n, L.Lmax, b->fft_chunk, b->tab->Kp, b->pd, b->tab->tab32,
                        b->tab->tw32, b->I32A, b->I32B, L.mode, L.norient, b->cand, b->grid,
-                       b->abound, b->tune[0]);
+                       b->abound, b->tune[0], b->S, L.Lmax + 1);
     mark(5);
+    // split-BF16x3 products carry up to 3.1 * 2^-16 of each |d I| beyond FP32
     launch_sph_select(s, P, L.n, b->tab->Kt, L.norient, b->pd, b->tab->tab64, b->tab->tw64,
                       b->I64A, b->I64B, L.mode, b->cand, b->scr, b->scr_stride, b->abound,
-                      b->peaks);
+                      b->peaks, b->S ? 3.1 / 65536.0 : 0.0);
   } else {
     mark(5);
   }
@@ -673,6 +682,10 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
                                            std::to_string(isoft_plane_smem(n, Lmax, 0)) + " > " +
                                            std::to_string(c->max_lds) + " bytes)");
   b->fft_chunk = (d->flags & FO_SPH_DFT) ? 0 : isoft_plane_use_fft(n, Lmax, c->max_lds);
+  const bool beta_gemm = (d->flags & FO_SPH_BETA_GEMM) != 0;
+  if (beta_gemm && (general || !b->fft_chunk))
+    return set_err(FO_ERR_UNSUPPORTED, "FO_SPH_BETA_GEMM needs the alignment mode and a "
+                                       "power-of-two grid side in [16, 256]");
   if (lap_lds_bytes(b->maxgroup) > c->max_lds)
     return set_err(FO_ERR_UNSUPPORTED, "permutation group too large for the LDS assignment solver");
   int rc = get_sph_table(c, n, &b->tab, build_tables);
@@ -728,6 +741,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   o.rowsol = lay.add<int>(Ntot + 1);
   o.scr = lay.add<double2>((size_t)P * no * b->scr_stride);
   o.grid = lc.keep_grid ? lay.add<float>((size_t)P * no * n * n * n) : 0;
+  o.S = beta_gemm ? lay.add<float4>((size_t)P * n * (Lmax + 1) * n) : 0;
   b->bytes = lay.off;
   *out = b.release();
   return FO_OK;
@@ -782,6 +796,7 @@ int sph_batch_bind(fo_sph_batch* b, void* mem, hipStream_t stream) {
   b->rowsol = (int*)(B + o.rowsol);
   b->scr = (double2*)(B + o.scr);
   b->grid = b->lc.keep_grid ? (float*)(B + o.grid) : nullptr;
+  b->S = o.S ? (float4*)(B + o.S) : nullptr;
   if (stream) {
     b->stream = stream;
     b->own_stream = false;
@@ -947,7 +962,7 @@ static int sph_autotune(fo_sph_batch* b) {
       if (knob == 0) {
         launch_isoft_plane(s, L.P, L.n, L.Lmax, b->fft_chunk, b->tab->Kp, b->pd, b->tab->tab32,
                            b->tab->tw32, b->I32A, b->I32B, L.mode, L.norient, b->cand, b->grid,
-                           b->abound, b->tune[0]);
+                           b->abound, b->tune[0], b->S, L.Lmax + 1);
       } else {
         for (const CoefChunk& C : b->chunks) {
           const SphPair* pdc = b->pd + C.p0;
@@ -1149,6 +1164,20 @@ int fo_sph_batch_tuning(fo_sph_batch* b, int* params, int set) {
     b->graphs.clear();
   } else {
     for (int k = 0; k < 3; ++k) params[k] = b->tune[k];
+  }
+  return FO_OK;
+}
+
+int fo_sph_batch_precision(fo_sph_batch* b, double* val32, double* val64, double* delta) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  const size_t m = (size_t)b->lc.P * b->lc.norient;
+  std::vector<SphPeak> pk(m);
+  FO_HIP(hipMemcpyAsync(pk.data(), b->peaks, sizeof(SphPeak) * m, hipMemcpyDeviceToHost, b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < m; ++i) {
+    if (val32) val32[i] = pk[i].val32;
+    if (val64) val64[i] = pk[i].value;
+    if (delta) delta[i] = pk[i].delta;
   }
   return FO_OK;
 }

@@ -55,6 +55,7 @@ struct SphPeak {
   double R0[3];        // interpolated Euler angles (fastoverlap/soft.py:127-130)
   double margin;       // FP64 value gap to the runner-up candidate (or +inf)
   double delta;        // FP32 error bound used to collect candidates
+  double val32;        // largest FP32 grid value (plane kernels)
 };
 
 struct SphOpt {

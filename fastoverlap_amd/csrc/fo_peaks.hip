@@ -12,8 +12,10 @@
 //   3. record the peak at argmax + x0, subtract the fitted function (with its
 //      background mu, unwrapped indices) from the whole grid;
 // stopping at the first fit that fails.  The fit is a Levenberg-Marquardt
-// iteration in FP64 with the analytic Jacobian and Marquardt's diagonal
-// damping; the grid itself is the FP32 grid of the plane kernels.
+// iteration in FP64 with the analytic Jacobian, Marquardt's diagonal scaling
+// and Nielsen's gain-ratio damping update (damping starts at 1e-3 of the
+// largest diagonal entry); the grid itself is the FP32 grid of the plane
+// kernels.
 #include "fo_common.h"
 #include "fo_internal.h"
 
@@ -98,7 +100,8 @@ __global__ void __launch_bounds__(256) k_find_peaks(
   __shared__ int s_ri[16];
   __shared__ int s_state;      // 0 iterate, 1 converged, 2 failed
   __shared__ int s_use_trial;  // evaluate at the trial parameters
-  __shared__ double s_lam, s_cost;
+  __shared__ double s_lam, s_nu, s_cost, s_pred;
+  __shared__ int s_fev;  // function evaluations, counted as a difference-Jacobian solver would
   __shared__ int s_ind[3];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const long long g = blockIdx.x;
@@ -149,7 +152,9 @@ __global__ void __launch_bounds__(256) k_find_peaks(
       for (int i = 0; i < kNPar; ++i) s_p[i] = s_pt[i] = p0[i];
       s_state = 0;
       s_use_trial = 0;
-      s_lam = 1e-3;
+      s_lam = -1.0;  // set from the first normal equations
+      s_nu = 2.0;
+      s_fev = 0;
     }
     __syncthreads();
     // Levenberg-Marquardt: each pass evaluates the residuals at s_p (full
@@ -182,6 +187,11 @@ __global__ void __launch_bounds__(256) k_find_peaks(
       __syncthreads();
       if (t == 0) {
         const double cost = s_sum[kNSum - 1];
+        // the reference's solver evaluates the residuals once per trial and
+        // kNPar times per forward-difference Jacobian, and gives up at
+        // 200 (kNPar + 1) evaluations
+        s_fev += trial ? 1 : (s_fev == 0 ? 1 + kNPar : kNPar);
+        double hmax = 1e-300;
         if (!trial) {
           // fresh normal equations at s_p: accepted point
           s_cost = cost;
@@ -192,24 +202,36 @@ __global__ void __launch_bounds__(256) k_find_peaks(
               s_H[j * kNPar + i] = s_sum[h];
               ++h;
             }
+          for (int i = 0; i < kNPar; ++i) hmax = fmax(hmax, s_H[i * kNPar + i]);
+          if (s_lam < 0.0) s_lam = 1e-3 * hmax;  // first point of this fit
           if (!isfinite(cost)) s_state = 2;
         } else {
-          if (isfinite(cost) && cost < s_cost) {
+          for (int i = 0; i < kNPar; ++i) hmax = fmax(hmax, s_H[i * kNPar + i]);
+          // gain ratio: actual over predicted reduction of the cost
+          const double rho = s_pred > 0.0 ? (s_cost - cost) / s_pred : -1.0;
+          if (isfinite(cost) && rho > 0.0) {
             const double rel = (s_cost - cost) / fmax(s_cost, 1e-300);
+            const double prel = s_pred / fmax(s_cost, 1e-300);
             double dn = 0.0, pn = 0.0;
             for (int i = 0; i < kNPar; ++i) {
               dn += (s_pt[i] - s_p[i]) * (s_pt[i] - s_p[i]);
               pn += s_p[i] * s_p[i];
             }
             for (int i = 0; i < kNPar; ++i) s_p[i] = s_pt[i];
-            s_lam = fmax(s_lam * 0.1, 1e-15);
-            // tolerances of the reference's least-squares fit (1.49012e-8)
-            if (rel <= 1.49012e-8 || sqrt(dn) <= 1.49012e-8 * (sqrt(pn) + 1.49012e-8))
+            // Nielsen's damping update
+            const double t = 2.0 * rho - 1.0;
+            s_lam *= fmax(1.0 / 3.0, 1.0 - t * t * t);
+            s_nu = 2.0;
+            // tolerances of the reference's least-squares fit (1.49012e-8):
+            // actual and predicted relative reductions, or the relative step
+            if ((rel <= 1.49012e-8 && prel <= 1.49012e-8) ||
+                sqrt(dn) <= 1.49012e-8 * (sqrt(pn) + 1.49012e-8))
               s_state = 1;
             s_use_trial = 0;  // next pass: normal equations at the new point
           } else {
-            s_lam *= 10.0;
-            if (s_lam > 1e16) s_state = 1;  // no descent left: a stationary point
+            s_lam *= s_nu;
+            s_nu *= 2.0;
+            if (s_lam > 1e16 * fmax(1.0, hmax)) s_state = 1;  // no descent left
           }
         }
         if (s_state == 0 && (!trial || s_use_trial)) {
@@ -218,15 +240,23 @@ __global__ void __launch_bounds__(256) k_find_peaks(
           if (!trial)
             for (int i = 0; i < kNPar; ++i) s_g[i] = s_sum[kNPar * (kNPar + 1) / 2 + i];
           double x[kNPar];
-          while (!damped_solve(s_H, s_g, s_lam, x) && s_lam < 1e16) s_lam *= 10.0;
-          if (s_lam >= 1e16) {
+          const double lim = 1e16 * fmax(1.0, hmax);
+          while (!damped_solve(s_H, s_g, s_lam, x) && s_lam < lim) s_lam *= 10.0;
+          if (s_lam >= lim) {
             s_state = 1;
           } else {
-            for (int i = 0; i < kNPar; ++i) s_pt[i] = s_p[i] + x[i];
+            // predicted reduction of the cost: x . (lam D x - g), D = diag(H)
+            double pr = 0.0;
+            for (int i = 0; i < kNPar; ++i) {
+              s_pt[i] = s_p[i] + x[i];
+              pr += x[i] * (s_lam * fmax(s_H[i * kNPar + i], 1e-300) * x[i] - s_g[i]);
+            }
+            s_pred = pr;
             s_use_trial = 1;
           }
         }
-        if (pass == 4 * kMaxIter - 1 && s_state == 0) s_state = 2;
+        if (s_state == 0 && (s_fev >= 200 * (kNPar + 1) || pass == 4 * kMaxIter - 1))
+          s_state = 2;
       }
       __syncthreads();
       if (s_state != 0) break;

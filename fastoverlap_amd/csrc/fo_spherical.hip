@@ -964,12 +964,15 @@ FO_DEV void static_for(F&& f) {
   }
 }
 
-template <int N1, int N2, int MODE>
+// FROMS: the beta contraction was done by k_beta_gemm; its even / odd-degree
+// sums are read from S (rows m1 <= Srows - 1) instead of contracted here.
+template <int N1, int N2, int MODE, bool FROMS>
 __global__ void __launch_bounds__(256) k_isoft_fft(
     const SphPair* __restrict__ pd, int P, int C, long long Kp, const float* __restrict__ tab32,
     const float2* __restrict__ twg, const float2* __restrict__ IA,
     const float2* __restrict__ IB, int norient, PlaneCand* __restrict__ cand,
-    float* __restrict__ grid, double* __restrict__ abound) {
+    float* __restrict__ grid, double* __restrict__ abound, const float4* __restrict__ Sg,
+    int Srows) {
   constexpr int n = N1 * N2;
   constexpr int TPF = fft_tpf<N1, N2>();
   static_assert(TPF == N1 && N2 % 4 == 0, "lanes own N2 contiguous plane-layout entries");
@@ -1019,6 +1022,16 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
     // sums over degrees l = m1w, m1w + 2, ... (A) and m1w + 1, m1w + 3, ... (B):
     // the even / odd-l partial sums (alignment mode) in some order, or, in
     // general mode, the two coefficient sets (A, B) over all degrees
+    float2 e[N2], od[N2];
+    if constexpr (FROMS) {
+      const float4* src = Sg + (((long long)p * n + ib) * Srows + (row ? m1 : 0)) * n + lane * N2;
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) {
+        const float4 v = row ? src[j2] : make_float4(0.f, 0.f, 0.f, 0.f);
+        e[j2] = make_float2(v.x, v.y);
+        od[j2] = make_float2(v.z, v.w);
+      }
+    } else {
     float2 a[N2], bq[N2];
 #pragma unroll
     for (int j2 = 0; j2 < N2; ++j2) a[j2] = bq[j2] = make_float2(0.f, 0.f);
@@ -1128,13 +1141,13 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
       });
     }
     // alignment mode: e = even-l sums, od = odd-l sums
-    float2 e[N2], od[N2];
     const bool swap = MODE == 0 && (m1w & 1);
 #pragma unroll
     for (int j2 = 0; j2 < N2; ++j2) {
       e[j2] = swap ? bq[j2] : a[j2];
       od[j2] = swap ? a[j2] : bq[j2];
     }
+    }  // FROMS
     for (int o = 0; o < no; ++o) {
       float2 v[N2], u[N1];
 #pragma unroll
@@ -1156,7 +1169,8 @@ __global__ void __launch_bounds__(256) k_isoft_fft(
     }
   }
   __syncthreads();
-  if (wantA) plane_abound(asum, MODE != 0, no, abound + (long long)p * no);
+  // (FROMS: the error-bound sum comes from k_coef_abound)
+  if (!FROMS && wantA) plane_abound(asum, MODE != 0, no, abound + (long long)p * no);
 
   // ---- alpha pass (columns g in rounds of NG)
   CandTrack ct;
@@ -1530,7 +1544,7 @@ __global__ void __launch_bounds__(256) k_sph_select(
     const double2* __restrict__ tw64, const double2* __restrict__ IA,
     const double2* __restrict__ IB, int mode, int norient, const PlaneCand* __restrict__ cand,
     double2* __restrict__ scratch, long long scratch_stride, const double* __restrict__ abound,
-    SphPeak* __restrict__ peaks, int P) {
+    SphPeak* __restrict__ peaks, int P, double extra_rel) {
   __shared__ double red[16 * kEvalBatch];
   __shared__ int redi[16];
   __shared__ int s_cnt, s_ovf;
@@ -1566,7 +1580,9 @@ __global__ void __launch_bounds__(256) k_sph_select(
   // error bound: A = sum_l sqrt((2l+1)/2) sum |I^l| >= every |grid value|,
   // from the plane kernel (Hermitian sets: |I[-m1,-m2]| = |I[m1,m2]|)
   const double A = abound[(long long)p * norient + o];
-  const double delta = (8.0 * (L + 1) + 32.0) * 5.9604644775390625e-08 * A;
+  // extra_rel: error of the beta contraction beyond FP32 rounding (the
+  // split-BF16x3 products: 3.1 * 2^-16 of each |d I|)
+  const double delta = ((8.0 * (L + 1) + 32.0) * 5.9604644775390625e-08 + extra_rel) * A;
   // |g32 - g64| <= delta everywhere, so the FP64 maximum has an FP32 value
   // of at least (FP32 maximum) - 2 delta
   const double thr = (double)mv - 2.0 * delta;
@@ -1733,6 +1749,7 @@ __global__ void __launch_bounds__(256) k_sph_select(
     pk.noverflow = cand_overflow ? n : novf;
     pk.pad = 0;
     pk.value = best;
+    pk.val32 = mv;
     pk.R0[0] = 2.0 * kPi / n * (ind[0] - off[0]);
     pk.R0[1] = kPi / n * (ind[1] - off[1]) + 0.5 * kPi / n;
     pk.R0[2] = 2.0 * kPi / n * (ind[2] - off[2]);
@@ -2137,6 +2154,161 @@ __global__ void __launch_bounds__(256) k_sph_finish(
 // ---------------------------------------------------------------------------
 namespace fo {
 
+// ---------------------------------------------------------------------------
+// The beta contraction as a GEMM on the matrix cores, split-BF16x3 with FP32
+// accumulation.  For each row m1 >= 0 and plane entry (m2), over a tile of 16
+// pairs and every beta:
+//   S_par[beta][pair] = sum_{l = par (mod 2)} dn^l_{m1m2}(beta) C^l_{m1m2}[pair]
+// i.e. (beta x l) . (l x pairs) per (m1, m2) -- the Wigner-d x coefficient
+// product of fastoverlap/soft.py:119-123 batched over pairs.  par = even /
+// odd degrees: the two orientations' sums are E + O and E - O.  Each FP32
+// operand x is split x = xh + xl (xh = bf16(x), xl = bf16(x - xh)); a b is
+// taken as ah bh + ah bl + al bh (three v_mfma_f32_16x16x16bf16, products
+// exact in FP32), which leaves at most 3.1 * 2^-16 |a b| per term.
+// Output S[pair][beta][m1][pos] = (E.re, E.im, O.re, O.im), rows m1 <= Lmax
+// (the batch's largest degree), in the plane layout's entry order.
+// MFMA lane layout (16x16x16): A lane -> (row lane%16, k 4(lane/16)..+3),
+// B lane -> (k 4(lane/16)..+3, column lane%16), result register r of a lane
+// -> (row 4(lane/16) + r, column lane%16).
+// ---------------------------------------------------------------------------
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+FO_DEV unsigned short bf16_rne(float x) {
+  const unsigned u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+FO_DEV void split_bf16(float x, short& hi, short& lo) {
+  const unsigned short h = bf16_rne(x);
+  hi = (short)h;
+  lo = (short)bf16_rne(x - __uint_as_float((unsigned)h << 16));
+}
+FO_DEV v4f mfma_bf16(v4s a, v4s b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+constexpr int kGemmPos = 8;  // plane entries per wavefront
+
+// one wavefront per (tile of 16 pairs, row m1, kGemmPos plane entries)
+__global__ void __launch_bounds__(64) k_beta_gemm(const SphPair* __restrict__ pd, int P, int n,
+                                                  int Lmax, long long Kp, int Lt,
+                                                  const float* __restrict__ tab32,
+                                                  const float2* __restrict__ IA,
+                                                  float4* __restrict__ S) {
+  const int lane = threadIdx.x, col = lane & 15, kq = lane >> 4;
+  const int nblk = n / kGemmPos;
+  int w = blockIdx.x;
+  const int pb = w % nblk;
+  w /= nblk;
+  const int m1 = w % (Lmax + 1);
+  const int tile = w / (Lmax + 1);
+  // this lane's pair: B operand column and output column
+  const int pj = tile * 16 + col;
+  const bool pok = pj < P;
+  const SphPair d = pd[pok ? pj : 0];
+  const int Lp = pok ? d.L : -1;
+  const int n1 = plane_n1(n), n2 = n / n1;
+  const int nbt = n / 16;
+  const int R = Lmax + 1;
+  for (int q = 0; q < kGemmPos; ++q) {
+    const int pos = kGemmPos * pb + q;
+    const int k = (pos % n2) * n1 + pos / n2;  // inverse of plane_pos
+    const int m2 = k < n / 2 ? k : k - n;
+    const int l0 = max(m1, abs(m2));
+    // coefficients of this lane's pair (B), split once per (parity, chunk)
+    v4s bh[2][2][2], bl[2][2][2];  // [parity][chunk][re, im]
+    const int nch = l0 <= Lmax ? (Lmax - l0) / 32 + 1 : 0;  // <= 2 for Lmax <= 63
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+#pragma unroll
+      for (int kc = 0; kc < 2; ++kc) {
+        const int lf = l0 + ((l0 + par) & 1) + 32 * kc;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int l = lf + 2 * (4 * kq + e);
+          float2 c = make_float2(0.f, 0.f);
+          if (kc < nch && l <= Lp) c = IA[d.c32_off + (long long)qlm(l, m1) * n + pos];
+          short h, lo;
+          split_bf16(c.x, h, lo);
+          bh[par][kc][0][e] = h;
+          bl[par][kc][0][e] = lo;
+          split_bf16(c.y, h, lo);
+          bh[par][kc][1][e] = h;
+          bl[par][kc][1][e] = lo;
+        }
+      }
+    for (int bt = 0; bt < nbt; ++bt) {
+      v4f acc[2][2];
+#pragma unroll
+      for (int par = 0; par < 2; ++par) acc[par][0] = acc[par][1] = v4f{0.f, 0.f, 0.f, 0.f};
+      const int beta = bt * 16 + col;  // A operand row of this lane
+#pragma unroll
+      for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc) {
+          if (kc >= nch) continue;  // wavefront-uniform
+          const int lf = l0 + ((l0 + par) & 1) + 32 * kc;
+          v4s ah, al;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int l = lf + 2 * (4 * kq + e);
+            float a = 0.f;
+            if (l <= Lmax && l <= Lt)
+              a = tab32[(long long)beta * Kp + (long long)qlm(l, m1) * n + pos];
+            short h, lo;
+            split_bf16(a, h, lo);
+            ah[e] = h;
+            al[e] = lo;
+          }
+#pragma unroll
+          for (int ri = 0; ri < 2; ++ri) {
+            acc[par][ri] = mfma_bf16(ah, bh[par][kc][ri], acc[par][ri]);
+            acc[par][ri] = mfma_bf16(ah, bl[par][kc][ri], acc[par][ri]);
+            acc[par][ri] = mfma_bf16(al, bh[par][kc][ri], acc[par][ri]);
+          }
+        }
+      if (pok)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int b = bt * 16 + 4 * kq + r;
+          S[(((long long)pj * n + b) * R + m1) * n + pos] =
+              make_float4(acc[0][0][r], acc[0][1][r], acc[1][0][r], acc[1][1][r]);
+        }
+    }
+  }
+}
+
+// error bound input of the selection for the GEMM arm: A = sum_l
+// sqrt((2l+1)/2) sum |C^l| (rows m1 > 0 count twice), one workgroup per pair
+__global__ void __launch_bounds__(256) k_coef_abound(const SphPair* __restrict__ pd, int n,
+                                                     const float2* __restrict__ IA, int norient,
+                                                     double* __restrict__ abound) {
+  const SphPair d = pd[blockIdx.x];
+  const long long tot = plane_rows(d.L) * n;
+  double s = 0.0;
+  for (long long e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int q = (int)(e / n);
+    int l = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+    while (qlm(l + 1, 0) <= q) ++l;
+    while (qlm(l, 0) > q) --l;
+    const int m1 = q - qlm(l, 0);
+    const float2 c = IA[d.c32_off + e];
+    const float wl = sqrtf(l + 0.5f) * (m1 == 0 ? 1.f : 2.f);
+    s += wl * sqrtf(c.x * c.x + c.y * c.y);
+  }
+  __shared__ double sc[8];
+  double v[1] = {s};
+  block_sum_n<1>(v, sc);
+  if (threadIdx.x == 0)
+    for (int o = 0; o < norient; ++o)
+      abound[(long long)blockIdx.x * norient + o] = v[0] * (1.0 + 1.0 / 65536.0);
+}
+
+}  // namespace fo
+
+
+namespace fo {
+
 static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 void launch_wigner_table(hipStream_t s, int n, int Lt, double* tab64, float* tab32) {
@@ -2232,7 +2404,8 @@ int isoft_plane_use_fft(int n, int Lmax, size_t max_lds) {
 void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long long Kp,
                         const SphPair* pd, const float* tab32, const float2* tw32,
                         const float2* IA, const float2* IB, int mode, int norient,
-                        PlaneCand* cand, float* grid, double* abound, int chunk) {
+                        PlaneCand* cand, float* grid, double* abound, int chunk,
+                        const float4* S, int Srows) {
   const size_t sm = isoft_plane_smem(n, Lmax, use_fft);
   int n1, n2;
   if (!use_fft || !fft_shape(n, n1, n2)) {
@@ -2245,12 +2418,15 @@ void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long
   const int C = std::max(1, std::min(chunk > 0 ? chunk : 16, (P + 7) / 8));
   const dim3 g((unsigned)(8 * ((P + 8 * C - 1) / (8 * C)) * C * n)), b(256);
 #define FO_FFT(N1, N2)                                                                         \
-  if (mode == 0)                                                                                \
-    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 0>), g, b, sm, s, pd, P, C, Kp, tab32, tw32, IA,    \
-                       IB, norient, cand, grid, abound);                                        \
+  if (S)                                                                                        \
+    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 0, true>), g, b, sm, s, pd, P, C, Kp, tab32, tw32,  \
+                       IA, IB, norient, cand, grid, abound, S, Srows);                          \
+  else if (mode == 0)                                                                           \
+    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 0, false>), g, b, sm, s, pd, P, C, Kp, tab32, tw32, \
+                       IA, IB, norient, cand, grid, abound, S, Srows);                          \
   else                                                                                          \
-    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 1>), g, b, sm, s, pd, P, C, Kp, tab32, tw32, IA,    \
-                       IB, norient, cand, grid, abound)
+    hipLaunchKernelGGL((k_isoft_fft<N1, N2, 1, false>), g, b, sm, s, pd, P, C, Kp, tab32, tw32, \
+                       IA, IB, norient, cand, grid, abound, S, Srows)
   switch (n) {
     case 16: FO_FFT(4, 4); break;
     case 32: FO_FFT(8, 4); break;
@@ -2261,12 +2437,24 @@ void launch_isoft_plane(hipStream_t s, int P, int n, int Lmax, int use_fft, long
 #undef FO_FFT
 }
 
+void launch_beta_gemm(hipStream_t s, int P, int n, int Lmax, long long Kp, int Lt,
+                      const SphPair* pd, const float* tab32, const float2* IA, float4* S,
+                      int norient, double* abound) {
+  const int tiles = (P + 15) / 16;
+  const long long wgs = (long long)tiles * (Lmax + 1) * (n / kGemmPos);
+  hipLaunchKernelGGL(k_beta_gemm, dim3((unsigned)wgs), dim3(64), 0, s, pd, P, n, Lmax, Kp, Lt,
+                     tab32, IA, S);
+  hipLaunchKernelGGL(k_coef_abound, dim3(P), dim3(256), 0, s, pd, n, IA, norient, abound);
+}
+
 void launch_sph_select(hipStream_t s, int P, int n, long long Kt, int norient, const SphPair* pd,
                        const double* tab64, const double2* tw64, const double2* IA,
                        const double2* IB, int mode, const PlaneCand* cand, double2* scratch,
-                       long long scratch_stride, const double* abound, SphPeak* peaks) {
+                       long long scratch_stride, const double* abound, SphPeak* peaks,
+                       double extra_rel) {
   hipLaunchKernelGGL(k_sph_select, dim3(xcd_grid(P, norient)), dim3(256), 0, s, pd, n, Kt, tab64,
-                     tw64, IA, IB, mode, norient, cand, scratch, scratch_stride, abound, peaks, P);
+                     tw64, IA, IB, mode, norient, cand, scratch, scratch_stride, abound, peaks, P,
+                     extra_rel);
 }
 
 void launch_sph_rotopt(hipStream_t s, int P, int norient, const SphPair* pd, const double2* IA,

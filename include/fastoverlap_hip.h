@@ -71,6 +71,9 @@ int fo_context_memory(fo_context* ctx, unsigned long long* used, unsigned long l
 #define FO_SPH_GENERAL 8       /* two independent Hermitian coefficient sets */
 #define FO_SPH_DFT 16          /* direct DFT planes even when grid_n is a power of two */
 #define FO_SPH_AUTOTUNE 32     /* choose launch shapes by timing on the first full run */
+#define FO_SPH_BETA_GEMM 64    /* beta contraction as a split-BF16x3 GEMM on the matrix
+                                  cores (FP32 accumulation) instead of FP32 in the plane
+                                  kernel; alignment mode, power-of-two grid sides */
 
 /* stages for fo_sph_batch_run (bit mask) */
 #define FO_STAGE_PREP 1        /* centroids, kernel width */
@@ -143,6 +146,12 @@ int fo_sph_batch_timing(fo_sph_batch* b, int enable, float* ms8);
  * (after an FO_SPH_AUTOTUNE batch's first full run: the tuned ones), set = 1
  * imposes them.  Every shape gives identical results. */
 int fo_sph_batch_tuning(fo_sph_batch* b, int* params, int set);
+/* Precision of the grid stage per (pair, orientation): val32 = largest value
+ * of the FP32 (or split-BF16x3) grid, val64 = FP64 value at the exact
+ * argmax, delta = the bound on |grid - FP64 grid| used to collect the
+ * candidates re-evaluated in FP64 (no counterpart in the reference, whose
+ * grid is FP64). */
+int fo_sph_batch_precision(fo_sph_batch* b, double* val32, double* val64, double* delta);
 /* Grid peak start angles per (pair, orientation): the interpolated maximum
  * as Euler angles, R0[npairs*norient*3] (fastoverlap/utils.py:319-338 +
  * fastoverlap/soft.py:127-130). */

@@ -28,13 +28,47 @@ def say(*a):
     sys.stdout.flush()
 
 
+def _check_fits_are_minima(tag, grid, fit, k, width):
+    """Solver-independent check of the kernel's fits: on the residual grid
+    (the grid less the min and the earlier fits), each window is centred on
+    the residual's argmax, and scipy's least-squares fit started from the
+    kernel's parameters cannot lower the residual by more than 1e-4 nor move
+    the centre by more than 0.01 cells (a local minimum of the same
+    problem)."""
+    from scipy.optimize import curve_fit
+    f = grid.astype(np.float64) - grid.min()
+    idx = np.indices(f.shape).reshape(3, -1)
+    pts = np.indices((2 * width + 1,) * 3).reshape(3, -1) - width
+    for i in range(k):
+        ind = np.array(np.unravel_index(f.argmax(), f.shape))
+        p = fit[i].copy()
+        p[8:11] -= ind
+        assert np.all(np.abs(p[8:11]) <= width + 1), (tag, i, p[8:11])
+        win = f[np.ix_(*[np.arange(c - width, c + width + 1) % s for c, s in zip(ind, f.shape)])]
+        q, _ = curve_fit(O.gaussian_nd, pts, win.ravel(), p0=p)
+        cost = lambda v: float(((O.gaussian_nd(pts, *v) - win.ravel()) ** 2).sum())
+        # scipy cannot lower the residual meaningfully (flat directions, such
+        # as height against background on a broad peak, may still move)
+        assert cost(p) <= cost(q) * (1 + 1e-4) + 1e-12 * float((win ** 2).sum()), \
+            (tag, i, cost(p), cost(q))
+        np.testing.assert_allclose(q[8:11], p[8:11], atol=1e-2, err_msg=f"{tag} fit {i}")
+        f[tuple(idx)] -= O.gaussian_nd(idx, *fit[i])
+
+
 def _compare_peaks(tag, grid, fit, cnt, npeaks, width):
     peaks, amp, mean, sigma, f = O.find_peaks(grid.astype(np.float64), npeaks, width)
     k = int(cnt)
-    # no successful fit: the oracle returns the interpolated maximum (with a
-    # scalar NaN width), the kernel a count of 0
+    # no successful fit in the oracle: it returns the interpolated maximum
+    # (with a scalar NaN width).  Its scipy solver gives up after 2400
+    # function evaluations where the kernel's can still converge (e.g. on the
+    # broad |f| peak of BLJ256); the kernel's fits are then checked on their
+    # own: each must be a least-squares minimum of its window.
     fallback = len(sigma) == 1 and np.ndim(sigma[0]) == 0
-    assert k == (0 if fallback else len(peaks)), (tag, k, len(peaks))
+    if fallback:
+        if k:
+            _check_fits_are_minima(tag, grid, fit, k, width)
+        return
+    assert k == len(peaks), (tag, k, len(peaks))
     for i in range(k):
         np.testing.assert_allclose(fit[i, 8:11], peaks[i], atol=1e-4, err_msg=f"{tag} peak {i}")
         assert abs(fit[i, 0] - amp[i]) <= 1e-3 * abs(amp[0]), (tag, i, fit[i, 0], amp[i])
