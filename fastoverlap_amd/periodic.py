@@ -280,3 +280,83 @@ class PeriodicAlign(BasePeriodicAlignment):
                 aligned[1, k // n, k % n] = r["X2"][k]
             return dists, aligned
         return dists
+
+
+class PeriodicAlignFortran(BasePeriodicAlignment):
+    """``PeriodicAlignFortran(Natoms, boxVec=None, scale=0, perm=None)``
+    (``fastoverlap/periodicAlignment.py:482-605``): the calls of the
+    reference's f2py wrapper of ``BULKFASTOVERLAP``, answered by the
+    library's Fortran-facing entry points (the same ones the
+    iso_c_binding module binds).  ``align`` follows ``ALIGN`` /
+    ``ALIGNCOEFFS`` (every candidate displacement refined, the best kept;
+    ``ohcell`` tries the 48 octahedral images, ``ohcell=2`` with each
+    image's own overlap); ``alignGroup`` follows ``ALIGNGROUP`` at the
+    CALCDEFAULTS width, as the reference's wrapper does.  Returned
+    permutations are 1-based, as the reference's ``commons.bestperm``."""
+
+    def __init__(self, Natoms, boxVec=None, scale=0, perm=None):
+        self.Natoms = 1 if Natoms is None else Natoms
+        self.boxvec = np.array(boxVec, dtype=float)
+        self.scale = scale
+        self.setPerm([np.arange(self.Natoms)] if perm is None else perm)
+
+    def setPerm(self, perm):
+        if len(perm):
+            self.perm = [np.asarray(p) for p in perm]
+            self.npermsize = [len(p) for p in self.perm]
+            self.permgroup = np.concatenate([p + 1 for p in self.perm]).astype(np.int32)
+            self.Natoms = len(self.permgroup)
+        else:
+            self.npermsize = [self.Natoms]
+            self.permgroup = np.arange(1, self.Natoms + 1, dtype=np.int32)
+            self.perm = [self.permgroup - 1]
+
+    def _groups(self):
+        return (len(self.npermsize), np.asarray(self.npermsize, np.int32),
+                np.ascontiguousarray(self.permgroup, np.int32))
+
+    def align(self, pos1, pos2, ndisps=10, perm=None, ohcell=False, debug=False):
+        """Returns (distance, pos1 (N, 3), aligned pos2 (N, 3), 1-based
+        permutation)."""
+        coordsb = _lib.as_f64(np.asanyarray(pos1, float).flatten())
+        coordsa = _lib.as_f64(np.asanyarray(pos2, float).flatten())
+        if perm is not None:
+            self.setPerm(perm)
+        if coordsa.size != 3 * self.Natoms:
+            self.setPerm([np.arange(coordsa.size // 3)])
+        ng, sizes, groups = self._groups()
+        dist, dist2 = np.zeros(1), np.zeros(1)
+        best = np.zeros(self.Natoms, np.int32)
+        disp = np.zeros(3)
+        L = _lib.load()
+        _lib.check(L.fo_f90_bulk_align(
+            _lib.dptr(coordsb), _lib.dptr(coordsa), self.Natoms, *map(float, self.boxvec),
+            float(self.scale), 10 if ndisps == 0 else int(ndisps), int(ohcell), ng,
+            _lib.iptr(sizes), _lib.iptr(groups), _lib.dptr(dist), _lib.dptr(dist2),
+            _lib.iptr(best), _lib.dptr(disp)))
+        self.disp = disp
+        return (float(dist[0]), coordsb.reshape(self.Natoms, 3), coordsa.reshape(self.Natoms, 3),
+                best)
+
+    def alignGroup(self, coords, ndisps=1):
+        """``(dists, aligned)``: ``dists[i, j]`` (i >= j; zero above the
+        diagonal) and ``aligned`` of shape (Natoms, 3, nlist, nlist) holding
+        structure j aligned onto structure i."""
+        coordslist = np.asanyarray(coords, float)
+        nlist, natoms, dim = coordslist.shape
+        assert dim == 3
+        assert natoms == self.Natoms
+        L = _lib.load()
+        kw, nw, nf = np.zeros(1), np.zeros(1, np.int32), np.zeros(1, np.int32)
+        _lib.check(L.fo_f90_bulk_defaults(natoms, *map(float, self.boxvec), _lib.dptr(kw),
+                                          _lib.iptr(nw), _lib.iptr(nf)))
+        flat = _lib.as_f64(coordslist.reshape(nlist, -1))
+        ng, sizes, groups = self._groups()
+        dists = np.zeros((nlist, nlist))   # [j][i] = Fortran DISTMAT(i, j)
+        aligned = np.zeros((nlist, nlist, 3 * natoms))
+        _lib.check(L.fo_f90_bulk_align_group(
+            _lib.dptr(flat), nlist, _lib.dptr(flat), nlist, natoms, *map(float, self.boxvec),
+            float(kw[0]), int(ndisps), int(nw[0]), int(nf[0]), 1, ng, _lib.iptr(sizes),
+            _lib.iptr(groups), _lib.dptr(dists), _lib.dptr(aligned)))
+        # Fortran order: DISTMAT(i, j) at [j][i], ALIGNEDCOORDS2(:, i, j) at [j][i]
+        return dists.T.copy(), aligned.transpose(2, 1, 0).reshape(natoms, 3, nlist, nlist)

@@ -349,10 +349,33 @@ int fo_f90_cluster_align(double* coordsb, double* coordsa, int natoms, int L, do
                          int invert, int ngroups, const int* npermsize, const int* permgroup,
                          double* distance, double* dist2, double* rmatbest, int* bestperm,
                          int* inverted);
+/* ndisps = NDISPLACEMENTS (<= 1: the interpolated overlap maximum; above:
+ * that many Gaussian-fitted peaks, each refined, as ALIGNCOEFFS,
+ * f90/fastbulk.f90:501-565); ohcell 1 = the COMMONS flag OHCELLT, which also
+ * tries the 48 octahedral images of coordsa, each from the displacements of
+ * the untransformed pair as f90/fastbulk.f90:458-481 does; ohcell 2 takes
+ * each image's displacements from its own overlap. */
 int fo_f90_bulk_align(double* coordsb, double* coordsa, int natoms, double boxlx, double boxly,
-                      double boxlz, double kernelwidth, int ngroups, const int* npermsize,
-                      const int* permgroup, double* distance, double* dist2, int* bestperm,
-                      double* disp);
+                      double boxlz, double kernelwidth, int ndisps, int ohcell, int ngroups,
+                      const int* npermsize, const int* permgroup, double* distance,
+                      double* dist2, int* bestperm, double* disp);
+/* CALCDEFAULTS (f90/fastbulk.f90:253-277): default kernel width, NWAVE and
+ * NFSPACE of the group alignment. */
+int fo_f90_bulk_defaults(int natoms, double boxlx, double boxly, double boxlz, double* kwidth,
+                         int* nwave, int* nfspace);
+/* ALIGNGROUP (f90/fastbulk.f90:336-412): every coords2 structure j aligned
+ * onto every coords1 structure i (i >= j only when sym; coords2 may then be
+ * NULL).  distmat is column-major [n1][n2] (distmat[i + n1 j]); aligned is
+ * [n2][n1][3 natoms] (ALIGNEDCOORDS2(3*NATOMS, N1LIST, N2LIST)); cells sym
+ * leaves out are zero.  ndisps = 0 means 10. */
+int fo_f90_bulk_align_group(const double* coords1, int n1, const double* coords2, int n2,
+                            int natoms, double boxlx, double boxly, double boxlz, double kwidth,
+                            int ndisps, int nwave, int nfspace, int sym, int ngroups,
+                            const int* npermsize, const int* permgroup, double* distmat,
+                            double* aligned);
+/* Row-major 3x3 matrix of octahedral operation op (0-based) in the order of
+ * OHOPS (f90/alignutils.f90:830-987): y = rmat x for each atom. */
+int fo_oh_operation(int op, double* rmat);
 
 #ifdef __cplusplus
 }

@@ -1,7 +1,8 @@
 """The Fortran caller (iso_c_binding modules, built in-tree by
 fastoverlap_amd/build.py) against the FP64 oracle: distance within 1e-8 and
 the 1-based BESTPERM equal to the oracle's permutation, for the cluster
-(LJ38) and the bulk (BLJ256) ALIGN."""
+(LJ38) and the bulk (BLJ256) ALIGN, the bulk ALIGN with OHCELLT and the group
+alignment ALIGNGROUP (with CALCDEFAULTS)."""
 
 import os
 import subprocess
@@ -47,3 +48,17 @@ def test_fortran_align_matches_dropin(gpu, lj38, blj256):
     np.testing.assert_array_equal(bperm, operm)
     # and the Python drop-in agrees with both
     assert abs(fo.PeriodicAlign(256, box, groups)(q1, q2)[0] - bdist) <= 1e-8
+    # OHCELLT: 48 images from the displacement of the untransformed pair
+    assert int(out["BULKOH"][1]) == 0
+    (odist, _, _, operm, _), _ = O.bulk_align(O.PeriodicOracle(256, box, groups), q1, q2,
+                                              ohcell=1)
+    assert abs(float(out["BULKOH"][0]) - odist) <= 1e-8
+    np.testing.assert_array_equal(np.array([int(v) for v in out["OHPERM"]]) - 1, operm)
+    # ALIGNGROUP over (coords, finish, coords mirrored in z), symmetric
+    kw, nwave, nf = O.bulk_defaults(256, box)
+    assert float(out["DEFAULTS"][0]) == pytest.approx(kw, rel=1e-14)
+    assert [int(v) for v in out["DEFAULTS"][1:]] == [nwave, nf, 0]
+    frames = np.stack([q1, q2, q1 * np.array([1.0, 1.0, -1.0])])
+    odm, _ = O.bulk_align_group(frames, boxvec=box, perm=groups)
+    dm = np.array([float(v) for v in out["GROUP"]]).reshape(3, 3, order="F")
+    np.testing.assert_allclose(dm, odm, rtol=0, atol=1e-8)

@@ -4,7 +4,9 @@ fails it: known answers from the reference's own example data, closed forms,
 library routines, invariants and brute force on tiny inputs."""
 
 import itertools
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -12,6 +14,8 @@ from scipy.fft import next_fast_len as scipy_next_fast_len
 from scipy.special import spherical_in
 
 from oracle import fo_oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 # ---- Wigner-d -----------------------------------------------------------------
@@ -560,3 +564,83 @@ def test_periodic_find_disps_multi_peak_recovers_translation():
     (d3, _, _, p3, disp3), _, _ = P.align(X, Y, npeaks=3)
     assert d1 < 1e-9 and d3 < 1e-9
     np.testing.assert_array_equal(perm[p3], np.arange(40))
+
+
+def test_oh_operations_match_the_reference_table():
+    """The 48 octahedral operations: the reference's OHOPS table, in its
+    order (fixture read from its DATA statement), and a group of distinct
+    signed permutation matrices closed under products."""
+    ref = json.load(open(os.path.join(GOLDEN, "oh_ops.json")))["ops"]
+    M = O.oh_matrices()
+    np.testing.assert_array_equal(M, np.array(ref))
+    keys = {m.tobytes() for m in M}
+    assert len(keys) == 48
+    for a in M:
+        assert sorted(np.abs(a).sum(0)) == [1, 1, 1] and sorted(np.abs(a).sum(1)) == [1, 1, 1]
+        for b in M:
+            assert (a @ b).tobytes() in keys
+    np.testing.assert_array_equal(M[0], np.eye(3))
+
+
+def _box_pair(seed, n=32, side=5.0):
+    rng = np.random.default_rng(seed)
+    box = np.full(3, side)
+    X = rng.random((n, 3)) * box
+    return rng, box, X
+
+
+def test_bulk_align_reduces_to_align_and_finds_images():
+    rng, box, X = _box_pair(21)
+    P = O.PeriodicOracle(32, box)
+    Y = X + rng.normal(scale=0.05, size=X.shape)
+    # one displacement, no Oh search: the Python drop-in's align
+    (d, _, X2, p, _), o = O.bulk_align(P, X, Y)
+    (da, _, X2a, pa, _), _, _ = P.align(X, Y)
+    assert o == 0 and d == da
+    np.testing.assert_array_equal(p, pa)
+    # a translated, permuted copy: operation 0 (identity) wins at distance 0
+    perm = rng.permutation(32)
+    Z = (X + np.array([0.9, -1.7, 2.2]))[perm]
+    (d1, _, _, p1, _), o1 = O.bulk_align(P, X, Z, ohcell=1)
+    assert d1 < 1e-9 and o1 == 0
+    np.testing.assert_array_equal(perm[p1], np.arange(32))
+    # an octahedral image: with per-image overlaps the inverse operation
+    # brings it back to a translated copy (distance 0), and no earlier
+    # operation does
+    M = O.oh_matrices()
+    k = 13
+    W = (X @ M[k].T + np.array([0.4, 1.3, -0.8]))[perm]
+    (d2, _, _, p2, _), o2 = O.bulk_align(P, X, W, ohcell=2)
+    np.testing.assert_array_equal(M[o2] @ M[k], np.eye(3))
+    assert d2 < 1e-9
+    np.testing.assert_array_equal(perm[p2], np.arange(32))
+    # several displacements: each refined, the best kept (never worse than
+    # the single interpolated maximum)
+    (d3, _, _, _, _), _ = O.bulk_align(P, X, Y, ndisps=4)
+    assert d3 <= d + 1e-12
+
+
+def test_bulk_defaults_closed_form():
+    kw, nwave, nf = O.bulk_defaults(256, np.full(3, 5.975206329))
+    assert kw == pytest.approx(5.975206329 / 256 ** (1 / 3) / 3, rel=1e-15)
+    # 2 pi / L * 1.5 / kw = 2 pi * 4.5 * 256^(1/3) / L^2 = 5.03  -> 6; FFT
+    # side: the 2,3,5-smooth length >= 4 * 6 + 3 = 27
+    assert 2 * math.pi * 4.5 * 256 ** (1 / 3) / 5.975206329 ** 2 == pytest.approx(5.03, abs=0.01)
+    assert nwave == 6 and nf == 27
+
+
+def test_bulk_align_group_cells_are_pair_alignments():
+    rng, box, X = _box_pair(22, n=24, side=4.5)
+    coords = np.stack([X, (X + 0.3)[rng.permutation(24)], rng.random((24, 3)) * box])
+    dist, aligned = O.bulk_align_group(coords, boxvec=box)
+    kw, nwave, nf = O.bulk_defaults(24, box)
+    P = O.PeriodicOracle(24, box, scale=kw, n=nwave, fshape=(nf,) * 3)
+    for j in range(3):
+        for i in range(3):
+            if i < j:
+                assert dist[i, j] == 0 and not aligned[i, j].any()
+                continue
+            (d, _, X2, _, _), _ = O.bulk_align(P, coords[i], coords[j])
+            assert dist[i, j] == d
+            np.testing.assert_array_equal(aligned[i, j], X2)
+    assert dist[1, 0] < 1e-9 and dist[0, 0] < 1e-9
