@@ -10,10 +10,11 @@ from .batch import (SphericalBatch, PeriodicBatch, align_spherical_batch,
                     align_periodic_batch, compact_to_dense, dense_to_compact, kcount,
                     next_fast_len, periodic_defaults)
 from .soft import SOFT
-from .spherical import SphericalAlign, BaseSphericalAlignment
+from .spherical import SphericalAlign, SphericalHarmonicAlign, BaseSphericalAlignment
 from .periodic import PeriodicAlign, PeriodicAlignFortran, BasePeriodicAlignment
 
-__all__ = ["SphericalAlign", "PeriodicAlign", "PeriodicAlignFortran", "SOFT",
+__all__ = ["SphericalAlign", "SphericalHarmonicAlign", "PeriodicAlign", "PeriodicAlignFortran",
+           "SOFT",
            "SphericalBatch", "PeriodicBatch", "align_spherical_batch", "align_periodic_batch", "FastOverlapError", "Context",
            "default_context", "compact_to_dense", "dense_to_compact", "kcount",
            "next_fast_len", "periodic_defaults", "BaseSphericalAlignment",

@@ -77,6 +77,20 @@ def next_pow2(x: int) -> int:
 # spherical
 # ---------------------------------------------------------------------------
 
+def _harmonic_dense(c, L):
+    """[groups][q(l,m), m >= 0][n] -> [groups][n][l][m % (2L+1)], negative m
+    from c[l,-m] = (-1)^m conj(c[l,m])."""
+    ng, _, nn = c.shape
+    out = np.zeros((ng, nn, L + 1, 2 * L + 1), np.complex128)
+    for l in range(L + 1):
+        for m in range(l + 1):
+            v = c[:, l * (l + 1) // 2 + m, :]
+            out[:, :, l, m] = v
+            if m:
+                out[:, :, l, -m] = (-1) ** m * v.conj()
+    return out
+
+
 class SphericalBatch:
     """``npairs`` cluster pairs on one SO(3) grid of side ``grid_n``.
 
@@ -85,7 +99,8 @@ class SphericalBatch:
     atoms.  flags: ``_lib.SPH_*``.
     """
 
-    def __init__(self, natoms, L, scale=None, grid_n=None, groups=None, flags=0, ctx=None):
+    def __init__(self, natoms, L, scale=None, grid_n=None, groups=None, flags=0, ctx=None,
+                 harmonic=None):
         natoms = np.atleast_1d(np.asarray(natoms, dtype=np.int64))
         P = len(natoms)
         L = np.broadcast_to(np.asarray(L, dtype=np.int64), (P,))
@@ -109,7 +124,14 @@ class SphericalBatch:
         d = _lib.SphBatchDesc(P, self.grid_n, iptr(self._nat), iptr(self._L), dptr(self._scale),
                               iptr(self._ng), iptr(self._gs), iptr(self._ga), flags)
         h = C.c_void_p()
-        call("fo_sph_batch_create", self.ctx.handle, C.byref(d), C.byref(h))
+        # harmonic = (nmax, harmscale): SphericalHarmonicAlign's coefficients
+        self.harmonic = None if harmonic is None else (int(harmonic[0]), float(harmonic[1]))
+        self._groups = [len(g) for g in groups]
+        if self.harmonic is None:
+            call("fo_sph_batch_create", self.ctx.handle, C.byref(d), C.byref(h))
+        else:
+            call("fo_sph_batch_create_harmonic", self.ctx.handle, C.byref(d),
+                 self.harmonic[0], self.harmonic[1], C.byref(h))
         self.handle = h
 
     # -- inputs ------------------------------------------------------------
@@ -135,6 +157,22 @@ class SphericalBatch:
         Bv = None if B is None else np.ascontiguousarray(B, np.complex128).ravel()
         call("fo_sph_batch_set_coeffs", self.handle, A.ctypes.data_as(_lib._pd),
              None if Bv is None else Bv.ctypes.data_as(_lib._pd))
+
+    def harmonic_coeffs(self, pair=0):
+        """Harmonic coefficients of one pair after the coefficient stage:
+        (c1, c2), each [ngroups][nmax+1][L+1][2L+1] complex with m at
+        index m % (2L+1) (the reference's calcHarmCoeff layout)."""
+        if self.harmonic is None:
+            raise ValueError("not a harmonic-basis batch")
+        nmax = self.harmonic[0]
+        L = int(self.L[pair])
+        ng = self._groups[pair]
+        nq = (L + 1) * (L + 2) // 2
+        c1 = np.zeros((ng, nq, nmax + 1), np.complex128)
+        c2 = np.zeros_like(c1)
+        call("fo_sph_batch_harmonic_coeffs", self.handle, int(pair),
+             c1.ctypes.data_as(_lib._pd), c2.ctypes.data_as(_lib._pd))
+        return _harmonic_dense(c1, L), _harmonic_dense(c2, L)
 
     def set_start(self, R0):
         R0 = as_f64(R0).ravel()

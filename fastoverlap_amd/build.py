@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FO_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["fo_spherical.hip", "fo_periodic.hip", "fo_api.cpp", "fo_server.cpp", "fo_f90.cpp",
-           "fo_noalloc.hip", "fo_peaks.hip"]
+           "fo_noalloc.hip", "fo_peaks.hip", "fo_harmonic.hip"]
 FORTRAN = os.path.join(HERE, "fortran")
 FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 

@@ -34,6 +34,9 @@ void launch_sph_ylm(hipStream_t, int, const SphPair*, const int*, const double*,
                     const double2*, const double*, double2*, double2*);
 void launch_sph_radial(hipStream_t, int, int, long long, const SphPair*, const SphGroup*, const int*,
                        const double*, const double*, const double*, double*, int);
+void launch_harm_coeffs(hipStream_t, int, int, int, const SphPair*, const SphGroup*, const int*,
+                        const double*, const double*, const double*, int, double, const double*,
+                        const double2*, const double2*, double*, int, double2*, float2*);
 void launch_sph_contract(hipStream_t, int, int, int, const SphPair*, const SphGroup*,
                          const double2*, const double2*, const double*, const double*, double2*,
                          float2*, int);
@@ -117,6 +120,7 @@ static bool all_finite(const double* p, size_t n) {
 }
 static constexpr int kMaxL = 63;
 static constexpr int kMaxGroup = 4096;
+static constexpr int kHarmMaxN = 40;  // harmonic basis radial degree
 
 // ---------------------------------------------------------------------------
 // context
@@ -488,7 +492,7 @@ struct fo_sph_batch {
   // layout (byte offsets inside base) and ownership
   struct {
     size_t pd, grp, gath, pos1, pos2, X1c, X2c, X2R, X2out, scale, R, Y1, Y2, I64A, I32A, I64B,
-        I32B, cand, abound, peaks, opt, res, perm, rowsol, scr, grid, S;
+        I32B, cand, abound, peaks, opt, res, perm, rowsol, scr, grid, S, gtab;
   } off{};
   int gsum = 0;
   bool own_mem = false, own_stream = false;
@@ -501,6 +505,13 @@ struct fo_sph_batch {
   // pair; 0 = default.  autotune: pick them on the first full run.
   std::array<int, 3> tune{{0, 15, 0}};
   bool autotune = false, tuned = false;
+  // harmonic-oscillator basis (harm_nmax >= 0): radial degree, length scale,
+  // basis polynomial table g[l][n][k] on the device, most groups of a pair
+  int harm_nmax = -1;
+  double harm_r0 = 1.0;
+  double* gtab = nullptr;
+  std::vector<double> hgtab;  // host copy (kept until the upload has run)
+  int maxgroups = 1;
 };
 
 static int sph_enqueue(fo_sph_batch* b, int stages) {
@@ -527,6 +538,13 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
       cmark(3 * c);
       launch_sph_ylm(s, Pc, pdc, b->gath, b->X1c, b->X2c, b->rect->leg, b->rect->cm, b->Y1, b->Y2);
       cmark(3 * c + 1);
+      if (b->harm_nmax >= 0) {
+        launch_harm_coeffs(s, Pc, C.Lmax, b->maxgroups, pdc, b->grp, b->gath, b->X1c, b->X2c,
+                           scc, b->harm_nmax, b->harm_r0, b->gtab, b->Y1, b->Y2, b->R, L.n,
+                           b->I64A, b->I32A);
+        cmark(3 * c + 2);
+        continue;
+      }
       launch_sph_radial(s, Pc, C.Lmax, C.maxjk, pdc, b->grp, b->gath, b->X1c, b->X2c, scc, b->R,
                         b->tune[2]);
       cmark(3 * c + 2);
@@ -569,6 +587,29 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
 
 namespace fo {
 
+// Coefficients of the harmonic-oscillator basis functions in powers of r
+// (fastoverlap/utils.py:408-427): g[l][n][k] multiplies r^{l+2k} in
+// N_nl r^l L_n^{l+1/2}(r^2), N_nl = (2 n! r0^{-2l-3} / Gamma(n+l+3/2))^{1/2},
+// L_n^a(x) = sum_k (-1)^k [prod_{i=k+1..n} (a+i)] / ((n-k)! k!) x^k.
+std::vector<double> harmonic_basis_table(int L, int nmax, double r0) {
+  const int NN = nmax + 1;
+  std::vector<double> g((size_t)(L + 1) * NN * NN, 0.0);
+  for (int l = 0; l <= L; ++l)
+    for (int n = 0; n <= nmax; ++n) {
+      const double a = l + 0.5;
+      const double norm = std::exp(0.5 * (std::log(2.0) + std::lgamma(n + 1.0) -
+                                          (2.0 * l + 3.0) * std::log(r0) - std::lgamma(n + l + 1.5)));
+      for (int k = 0; k <= n; ++k) {
+        double v = 1.0;
+        for (int i = k + 1; i <= n; ++i) v *= a + i;
+        for (int i = 2; i <= n - k; ++i) v /= i;
+        for (int i = 2; i <= k; ++i) v /= i;
+        g[((size_t)l * NN + n) * NN + k] = (k & 1 ? -v : v) * norm;
+      }
+    }
+  return g;
+}
+
 // Byte budget of one coefficient chunk (harmonics of both structures plus
 // the radial kernels).  Unlimited by default (one chunk: the kernels need
 // many pairs per launch to fill the device); FO_COEF_CHUNK_MB caps the
@@ -582,7 +623,7 @@ static size_t coef_chunk_bytes() {
 }
 
 int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out,
-                   bool build_tables) {
+                   bool build_tables, int harm_nmax, double harm_r0) {
   if (!c || !d || !out) return set_err(FO_ERR_INVALID, "NULL argument");
   if (d->npairs < 1) return set_err(FO_ERR_INVALID, "npairs must be >= 1");
   if (!d->natoms || !d->L || !d->ngroups)
@@ -590,9 +631,18 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   const int P = d->npairs, n = d->grid_n;
   if (n < 2 || n > 256) return set_err(FO_ERR_UNSUPPORTED, "grid_n must be in [2, 256]");
   const bool general = d->flags & FO_SPH_GENERAL;
+  if (harm_nmax > kHarmMaxN)
+    return set_err(FO_ERR_UNSUPPORTED, "harmonic radial degree nmax must be <= " +
+                                           std::to_string(kHarmMaxN));
+  if (harm_nmax >= 0 && !(harm_r0 > 0))
+    return set_err(FO_ERR_INVALID, "harmonic length scale must be > 0");
+  if (harm_nmax >= 0 && general)
+    return set_err(FO_ERR_UNSUPPORTED, "the harmonic basis builds its own coefficients");
   auto b = std::make_unique<fo_sph_batch>();
   b->ctx = c;
   b->flags = d->flags;
+  b->harm_nmax = harm_nmax;
+  b->harm_r0 = harm_r0;
   int Lmax = 0, maxatoms = 0, maxgath = 0;
   long long maxjk = 0, Ntot = 0, Ktot = 0, C32tot = 0, radtot = 0, ylmtot = 0;
   long long radmax = 0, ylmmax = 0;  // largest chunk
@@ -653,7 +703,14 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
     Ktot += kcount(Lp);
     C32tot += plane_rows(Lp) * n;
     // coefficient chunks: K and Y offsets are relative to the pair's chunk
-    const long long rs = (long long)(Lp + 1) * jk, ys = (long long)(Lp + 1) * (Lp + 2) / 2 * ng;
+    // radial block: K_l[j][k] of every group, or for the harmonic basis
+    // D[j][l][n] of both structures then c[group][q(l,m)][n] of both
+    const long long nn = harm_nmax + 1;
+    const long long rs = harm_nmax >= 0
+                             ? 2 * ng * (Lp + 1) * nn + 4 * (long long)sp.ngroups * plane_rows(Lp) * nn
+                             : (long long)(Lp + 1) * jk;
+    const long long ys = (long long)(Lp + 1) * (Lp + 2) / 2 * ng;
+    b->maxgroups = std::max(b->maxgroups, sp.ngroups);
     const size_t cb = sizeof(double) * (size_t)(radtot + rs) + 2 * sizeof(double2) * (size_t)(ylmtot + ys);
     if (b->chunks.empty() || (cb > chunk_budget && radtot + ylmtot > 0)) {
       b->chunks.push_back(CoefChunk{p, p, 0, 0});
@@ -705,7 +762,8 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   lc.mode = general ? 1 : 0;
   if (general) lc.norient = 2;
   lc.keep_grid = (d->flags & FO_SPH_KEEP_GRID) ? 1 : 0;
-  b->autotune = (d->flags & FO_SPH_AUTOTUNE) != 0;
+  // the tuned shapes belong to the Gaussian-overlap coefficient kernels
+  b->autotune = (d->flags & FO_SPH_AUTOTUNE) != 0 && harm_nmax < 0;
   lc.center = (d->flags & FO_SPH_NO_CENTER) ? 0 : 1;
   b->Ntot = Ntot;
   b->Ktot = Ktot;
@@ -742,6 +800,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   o.scr = lay.add<double2>((size_t)P * no * b->scr_stride);
   o.grid = lc.keep_grid ? lay.add<float>((size_t)P * no * n * n * n) : 0;
   o.S = beta_gemm ? lay.add<float4>((size_t)P * n * (Lmax + 1) * n) : 0;
+  o.gtab = harm_nmax >= 0 ? lay.add<double>((size_t)(Lmax + 1) * (harm_nmax + 1) * (harm_nmax + 1)) : 0;
   b->bytes = lay.off;
   *out = b.release();
   return FO_OK;
@@ -797,6 +856,7 @@ int sph_batch_bind(fo_sph_batch* b, void* mem, hipStream_t stream) {
   b->scr = (double2*)(B + o.scr);
   b->grid = b->lc.keep_grid ? (float*)(B + o.grid) : nullptr;
   b->S = o.S ? (float4*)(B + o.S) : nullptr;
+  b->gtab = b->harm_nmax >= 0 ? (double*)(B + o.gtab) : nullptr;
   if (stream) {
     b->stream = stream;
     b->own_stream = false;
@@ -816,6 +876,11 @@ int sph_batch_bind(fo_sph_batch* b, void* mem, hipStream_t stream) {
                           b->stream));
   FO_HIP(hipMemcpyAsync(b->gath, b->hgath.data(), sizeof(int) * b->hgath.size(),
                         hipMemcpyHostToDevice, b->stream));
+  if (b->gtab) {
+    b->hgtab = harmonic_basis_table(b->lc.Lmax, b->harm_nmax, b->harm_r0);
+    FO_HIP(hipMemcpyAsync(b->gtab, b->hgtab.data(), sizeof(double) * b->hgtab.size(),
+                          hipMemcpyHostToDevice, b->stream));
+  }
   return FO_OK;
 }
 
@@ -836,6 +901,43 @@ int fo_sph_batch_create(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch*
     return rc;
   }
   *out = b;
+  return FO_OK;
+}
+
+int fo_sph_batch_create_harmonic(fo_context* c, const fo_sph_batch_desc* d, int nmax,
+                                 double harmscale, fo_sph_batch** out) {
+  if (!out) return set_err(FO_ERR_INVALID, "out is NULL");
+  if (nmax < 0) return set_err(FO_ERR_INVALID, "nmax must be >= 0");
+  fo_sph_batch* b = nullptr;
+  int rc = sph_batch_plan(c, d, &b, true, nmax, harmscale);
+  if (rc) return rc;
+  rc = sph_batch_bind(b, nullptr, nullptr);
+  if (rc == FO_OK && hipStreamSynchronize(b->stream) != hipSuccess)
+    rc = set_err(FO_ERR_HIP, "descriptor upload failed");
+  if (rc) {
+    fo_sph_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return FO_OK;
+}
+
+int fo_sph_batch_harmonic_coeffs(fo_sph_batch* b, int pair, double* c1, double* c2) {
+  if (!b) return set_err(FO_ERR_INVALID, "NULL batch");
+  if (b->harm_nmax < 0) return set_err(FO_ERR_STATE, "not a harmonic-basis batch");
+  if (pair < 0 || pair >= b->lc.P) return set_err(FO_ERR_INVALID, "pair out of range");
+  if (b->chunks.size() != 1)
+    return set_err(FO_ERR_STATE, "coefficients are kept only for a single-chunk batch");
+  const SphPair& d = b->hp[pair];
+  const size_t nn = b->harm_nmax + 1;
+  const size_t one = (size_t)d.ngroups * plane_rows(d.L) * nn;
+  const double* base = b->R + d.rad_off + 2 * (size_t)d.ngath * (d.L + 1) * nn;
+  FO_HIP(hipSetDevice(b->ctx->device));
+  if (c1) FO_HIP(hipMemcpyAsync(c1, base, sizeof(double2) * one, hipMemcpyDeviceToHost, b->stream));
+  if (c2)
+    FO_HIP(hipMemcpyAsync(c2, base + 2 * one, sizeof(double2) * one, hipMemcpyDeviceToHost,
+                          b->stream));
+  FO_HIP(hipStreamSynchronize(b->stream));
   return FO_OK;
 }
 

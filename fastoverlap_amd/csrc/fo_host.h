@@ -12,8 +12,11 @@ namespace fo {
 // memory layout, and resolve device tables; no batch memory is allocated.
 // build_tables = false: the grid's tables must already be resident (the
 // allocation-free entry point plans this way).
+// harm_nmax >= 0: the coefficient stage projects on the harmonic-oscillator
+// basis of radial degree <= harm_nmax and length scale harm_r0
+// (SphericalHarmonicAlign) instead of contracting the Gaussian overlap.
 int sph_batch_plan(fo_context* ctx, const fo_sph_batch_desc* d, fo_sph_batch** out,
-                   bool build_tables = true);
+                   bool build_tables = true, int harm_nmax = -1, double harm_r0 = 1.0);
 int per_batch_plan(fo_context* ctx, const fo_per_batch_desc* d, fo_per_batch** out);
 // Device bytes the planned batch needs.
 size_t sph_batch_bytes(const fo_sph_batch* b);

@@ -1,5 +1,6 @@
-"""Drop-in ``SphericalAlign`` (``fastoverlap/sphericalAlignment.py:29-273``)
-running on the MI355X through ``libfastoverlap_hip``.
+"""Drop-in ``SphericalAlign`` and ``SphericalHarmonicAlign``
+(``fastoverlap/sphericalAlignment.py:29-414``) running on the MI355X through
+``libfastoverlap_hip``.
 
 Same constructor and call signatures and return tuples as the reference.
 Every numerical stage -- coefficients C^l (FP64), inverse SOFT (FP32 grid),
@@ -14,10 +15,18 @@ Differences from the reference, by design:
    optimum, and only the permutation / RMSD downstream of it is contracted;
  * ``malign`` refines the inverted orientation's candidates from the
    inverted structure (the reference passes the non-inverted X2 there);
- * an atom at the centroid follows the i_0(0) = 1 limit instead of NaN.
+ * an atom at the centroid follows the i_0(0) = 1 limit instead of NaN;
+ * ``SphericalHarmonicAlign.alignGroup`` aligns every pair exactly as
+   ``align`` does (centred structures, each pair's own width, permutation
+   groups); the reference builds the coefficients from the uncentred
+   structures with the width of the previous call and one group
+   (``sphericalAlignment.py:397-414``), which agrees when the structures are
+   centred and the width is fixed.
 """
 
 from __future__ import annotations
+
+import math
 
 import numpy as np
 from numpy.linalg import norm
@@ -162,6 +171,10 @@ class BaseSphericalAlignment:
     def _grid_n(self):
         return self.soft.n
 
+    def _batch_kw(self):
+        """Extra SphericalBatch arguments of the coefficient method."""
+        return {}
+
     def align(self, pos1, pos2, perm=None, invert=True, calcCoeffs=None):
         """``sphericalAlignment.py:160-188``: returns (dist, X1, X2)."""
         return self._align(pos1, pos2, perm, invert, calcCoeffs)[:3]
@@ -209,7 +222,8 @@ class BaseSphericalAlignment:
         system reuse its buffers and its captured launch graph.  Like the
         reference's instances (mutable scratch), an instance is not safe for
         concurrent calls."""
-        key = (n, L, self._grid_n(), scale, flags,
+        kw = self._batch_kw()
+        key = (n, L, self._grid_n(), scale, flags, tuple(sorted(kw.items())),
                tuple(tuple(int(i) for i in g) for g in groups[0]))
         cached = getattr(self, "_plain", None)
         if cached is not None and cached[0] == key:
@@ -217,7 +231,7 @@ class BaseSphericalAlignment:
         if cached is not None:
             cached[1].close()
             self._plain = None
-        b = SphericalBatch([n], [L], [scale], self._grid_n(), groups, flags)
+        b = SphericalBatch([n], [L], [scale], self._grid_n(), groups, flags, **kw)
         self._plain = (key, b)
         return b, False
 
@@ -281,7 +295,8 @@ class BaseSphericalAlignment:
         scale = float(self.scale)
         if calcCoeffs is None:
             flags = _lib.SPH_KEEP_GRID | (0 if invert else _lib.SPH_NO_INVERT)
-            with SphericalBatch([len(pos1)], [L], [scale], n, groups, flags) as b:
+            with SphericalBatch([len(pos1)], [L], [scale], n, groups, flags,
+                                **self._batch_kw()) as b:
                 b.set_positions(pos1, pos2)
                 b.run(_lib.STAGE_PREP | _lib.STAGE_COEFF | _lib.STAGE_GRID)
                 fit, cnt, _ = b.find_peaks(nrot, 2)
@@ -375,3 +390,119 @@ def _lap(X1, X2, groups, metric, box):
               _lib.iptr(sizes), _lib.iptr(atoms), metric, _lib.dptr(boxa), _lib.iptr(perm),
               _lib.dptr(d))
     return float(d[0]), perm.astype(np.int64)
+
+
+class SphericalHarmonicAlign(BaseSphericalAlignment):
+    """``SphericalHarmonicAlign(scale=None, harmscale=1.0, nmax=15, Jmax=15,
+    perm=None)`` (``sphericalAlignment.py:276-414``): the overlap coefficients
+    come from each structure's projection on the harmonic-oscillator basis of
+    radial degree <= nmax and length scale ``harmscale``,
+    C^l = sum_n conj(c1) c2 (``:340-368``), built on the device by the
+    batch's harmonic coefficient stage; the grid, argmax, rotation
+    refinement, LAP and fit are the spherical pipeline's.  ``bw`` is an
+    extension: the SOFT bandwidth (default Jmax+1)."""
+
+    def __init__(self, scale=None, harmscale=1.0, nmax=15, Jmax=15, perm=None, bw=None):
+        if scale is not None:
+            self.scale = scale
+            self.calcScale = False
+        else:
+            self.calcScale = True
+        self.harmscale = harmscale
+        self._bw = bw
+        self.setCoeffs(nmax, Jmax, harmscale)
+        self.perm = perm
+
+    @classmethod
+    def HarmCoeffs(cls, nmax, lmax, r0):
+        """Basis polynomial coefficients [n][l][s] (``:320-326``): N_nl times
+        the coefficient of r^s in r^l L_n^{l+1/2}(r^2) (``utils.py:414-427``),
+        from the Laguerre closed form."""
+        c = np.zeros((nmax + 1, lmax + 1, 2 * nmax + lmax + 1))
+        for n in range(nmax + 1):
+            for l in range(lmax + 1):
+                a = l + 0.5
+                norm = math.exp(0.5 * (math.log(2.0) + math.lgamma(n + 1) -
+                                       (2 * l + 3) * math.log(r0) - math.lgamma(n + l + 1.5)))
+                for k in range(n + 1):
+                    v = 1.0
+                    for i in range(k + 1, n + 1):
+                        v *= a + i
+                    v /= math.factorial(n - k) * math.factorial(k)
+                    c[n, l, l + 2 * k] = (-v if k & 1 else v) * norm
+        return c
+
+    def setCoeffs(self, nmax=None, Jmax=None, harmscale=None):
+        """``:328-338``."""
+        if nmax is None:
+            nmax = self.nmax
+        else:
+            self.nmax = nmax
+        if Jmax is None:
+            Jmax = self.Jmax
+        else:
+            self.setJ(Jmax, self._bw)
+        if harmscale is None:
+            harmscale = self.harmscale
+        else:
+            self.harmscale = harmscale
+        self.coeffs = self.HarmCoeffs(nmax, Jmax, harmscale)
+
+    def _batch_kw(self):
+        return {"harmonic": (int(self.nmax), float(self.harmscale))}
+
+    def calcHarmCoeff(self, pos):
+        """c[n, l, m % (2J+1)] of Gaussians (width ``self.scale``) at ``pos``,
+        positions used as given (``:340-356``), on the device."""
+        pos = np.atleast_2d(np.asarray(pos, float))
+        L = self.Jmax
+        with SphericalBatch([len(pos)], [L], [float(self.scale)], 2 * (L + 1), None,
+                            _lib.SPH_NO_INVERT | _lib.SPH_NO_CENTER, **self._batch_kw()) as b:
+            b.set_positions(pos, pos)
+            b.run(_lib.STAGE_PREP | _lib.STAGE_COEFF)
+            c1, _ = b.harmonic_coeffs(0)
+        return c1[0]
+
+    def calcSO3Coeffs(self, pos1, pos2):
+        """sum_n conj(c1) c2 of two point sets (``:358-361``), dense
+        (J+1, 2J+1, 2J+1), on the device."""
+        pos1 = np.atleast_2d(np.asarray(pos1, float))
+        pos2 = np.atleast_2d(np.asarray(pos2, float))
+        assert pos1.shape == pos2.shape
+        L = self.Jmax
+        with SphericalBatch([len(pos1)], [L], [float(self.scale)], 2 * (L + 1), None,
+                            _lib.SPH_NO_INVERT | _lib.SPH_NO_CENTER, **self._batch_kw()) as b:
+            b.set_positions(pos1, pos2)
+            b.run(_lib.STAGE_PREP | _lib.STAGE_COEFF)
+            c = b.coeffs()[0]
+        return compact_to_dense(c, L)
+
+    def alignGroup(self, coords, keepCoords=False):
+        """Every ordered pair of ``coords`` aligned (``:397-414``), all n^2
+        pairs as one device batch; returns dists (and the aligned pairs
+        [2, n, n, N, 3] with keepCoords)."""
+        coords = [np.asarray(c, float).reshape(-1, 3) for c in coords]
+        n = len(coords)
+        natoms = len(coords[0])
+        if self.calcScale:
+            # the reference computes the width of each pair from that pair
+            seps = [self.averageSeparation(c) for c in coords]
+        groups = [_groups_from_perm(self.perm, natoms)]
+        P = n * n
+        i1 = np.repeat(np.arange(n), n)
+        i2 = np.tile(np.arange(n), n)
+        scales = ([(seps[i] + seps[j]) / 6 for i, j in zip(i1, i2)] if self.calcScale
+                  else [float(self.scale)] * P)
+        with SphericalBatch([natoms] * P, [self.Jmax] * P, scales, self._grid_n(), groups * P,
+                            0, **self._batch_kw()) as b:
+            b.set_positions([coords[i] for i in i1], [coords[j] for j in i2])
+            b.run(_lib.STAGE_ALL)
+            r = b.results()
+        dists = r["dist"].reshape(n, n)
+        if keepCoords:
+            aligned = np.empty((2, n, n, natoms, 3))
+            for k in range(P):
+                aligned[0, k // n, k % n] = r["X1"][k]
+                aligned[1, k // n, k % n] = r["X2"][k]
+            return dists, aligned
+        return dists

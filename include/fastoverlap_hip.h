@@ -96,6 +96,18 @@ typedef struct {
 } fo_sph_batch_desc;
 
 int fo_sph_batch_create(fo_context* ctx, const fo_sph_batch_desc* desc, fo_sph_batch** out);
+/* SphericalHarmonicAlign (fastoverlap/sphericalAlignment.py:276-391): the
+ * coefficient stage projects each structure's Gaussian density (width =
+ * the pair's scale) on the harmonic-oscillator basis of radial degree
+ * <= nmax and length scale harmscale (:340-356, utils.py:408-427) and forms
+ * C^l = sum_groups sum_n conj(c1) c2 (:363-368); every other stage is the
+ * batch's own. */
+int fo_sph_batch_create_harmonic(fo_context* ctx, const fo_sph_batch_desc* desc, int nmax,
+                                 double harmscale, fo_sph_batch** out);
+/* harmonic coefficients of one pair after FO_STAGE_COEFF (single-chunk
+ * batches): c1, c2 [ngroups][q(l,m)][nmax+1] complex, q(l,m) = l(l+1)/2 + m,
+ * 0 <= m <= l <= L (c[l,-m] = (-1)^m conj(c[l,m])) */
+int fo_sph_batch_harmonic_coeffs(fo_sph_batch* b, int pair, double* c1, double* c2);
 int fo_sph_batch_destroy(fo_sph_batch* b);
 /* bytes of device memory the batch holds */
 int fo_sph_batch_footprint(const fo_sph_batch* b, unsigned long long* bytes);

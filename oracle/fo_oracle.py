@@ -681,6 +681,105 @@ def spherical_align(pos1, pos2, Jmax, scale=None, perm=None, invert=True,
 
 
 # ---------------------------------------------------------------------------
+# Harmonic-oscillator basis   (SphericalHarmonicAlign,
+#                              fastoverlap/sphericalAlignment.py:276-391,
+#                              fastoverlap/utils.py:408-427)
+# ---------------------------------------------------------------------------
+
+
+def harmonic_norm(n, l, r0):
+    """``norm_harmonicBasis`` (``fastoverlap/utils.py:408-412``)."""
+    from scipy.special import gamma
+    return (2 * math.factorial(n) * r0 ** (-2 * l - 3) / gamma(1.5 + n + l)) ** 0.5
+
+
+def harmonic_coeffs(n, l, r0):
+    """``coeffs_harmonicBasis`` (``fastoverlap/utils.py:414-427``): the
+    coefficients of r^l L_n^{l+1/2}(r^2) in powers of r (scipy's genlaguerre
+    coefficients at s = l, l+2, ...), times the normalisation."""
+    from scipy.special import genlaguerre
+    out = np.zeros(2 * n + l + 1)
+    out[l::2] = genlaguerre(n, l + 0.5).coeffs[::-1]
+    return out * harmonic_norm(n, l, r0)
+
+
+def harmonic_coeff_table(nmax, lmax, r0):
+    """``SphericalHarmonicAlign.HarmCoeffs`` (``sphericalAlignment.py:320-326``)."""
+    c = np.zeros((nmax + 1, lmax + 1, 2 * nmax + lmax + 1))
+    for n in range(nmax + 1):
+        for l in range(lmax + 1):
+            c[n, l, :2 * n + l + 1] = harmonic_coeffs(n, l, r0)
+    return c
+
+
+def radial_integral_harmonic(s, l, rj, sigma, r0):
+    """4 pi int_0^inf exp(-(r^2 + rj^2)/2 sigma^2) exp(-r^2/2 r0^2) i_l(r rj/sigma^2)
+    r^s r^2 dr in the reference's closed form (``sphericalAlignment.py:287-300``;
+    its docstring omits the 4 pi of the Gaussian's multipole expansion)."""
+    from scipy.special import gamma, hyp1f1
+    return ((2.0 ** (3 + s - l) * pi ** 3) ** 0.5 * rj ** l
+            * (sigma ** -2 + r0 ** -2) ** (-0.5 * (3 + s + l)) * sigma ** (-2 * l)
+            * hyp1f1(0.5 * (3 + s + l), 1.5 + l,
+                     0.5 * rj ** 2 * r0 ** 2 / (r0 ** 2 * sigma ** 2 + sigma ** 4))
+            * gamma(0.5 * (3 + l + s)) / gamma(1.5 + l)) * np.exp(-0.5 * rj ** 2 * sigma ** -2)
+
+
+def harmonic_coeff(pos, Jmax, nmax, scale, r0):
+    """c[n, l, m % (2J+1)] of a set of Gaussians of width ``scale``
+    (``SphericalHarmonicAlign.calcHarmCoeff``, ``sphericalAlignment.py:340-356``)."""
+    pos = np.atleast_2d(np.asarray(pos, float))
+    theta, phi, r = calc_theta_phi_r(pos)
+    Y = sph_harm_table(Jmax, theta, phi)
+    J = np.arange(Jmax + 1)
+    dslj = radial_integral_harmonic(np.arange(2 * nmax + Jmax + 1)[:, None, None],
+                                    J[None, :, None], r[None, None, :], scale, r0)
+    dnlj = np.einsum("nls,slj->nlj", harmonic_coeff_table(nmax, Jmax, r0), dslj)
+    return np.einsum("nlj,lmj->nlm", dnlj, Y.conj())
+
+
+def harmonic_so3_coeffs(X1, X2, perm, Jmax, nmax, scale, r0):
+    """``calcSO3Coeffs`` summed over groups (``sphericalAlignment.py:358-361``,
+    ``:176``): I[l, m1, m2] = sum_groups sum_n conj(c1[n,l,m1]) c2[n,l,m2]."""
+    return sum(np.einsum("nlm,nlo->lmo", harmonic_coeff(X1[p], Jmax, nmax, scale, r0).conj(),
+                         harmonic_coeff(X2[p], Jmax, nmax, scale, r0)) for p in perm)
+
+
+def harmonic_align(pos1, pos2, Jmax, nmax, scale=None, r0=1.0, perm=None, invert=True,
+                   soft=None):
+    """``SphericalHarmonicAlign.align`` = ``BaseSphericalAlignment.align``
+    (``sphericalAlignment.py:160-188``) with the harmonic-basis coefficients,
+    then ``refine`` (``:118-127``); the inverted orientation's coefficients
+    are those of -X2 and replace on a strictly smaller objective."""
+    pos1 = np.asanyarray(pos1, float).reshape(-1, 3)
+    pos2 = np.asanyarray(pos2, float).reshape(-1, 3)
+    if scale is None:
+        scale = (average_separation(pos1) + average_separation(pos2)) / 6
+    if perm is None:
+        perm = [np.arange(len(pos1))]
+    soft = SOFTOracle(Jmax + 1) if soft is None else soft
+    X1 = pos1 - pos1.mean(0)
+    X2 = pos2 - pos2.mean(0)
+    res = SphericalResult(0.0, X1, X2, None, False, None)
+    best = None
+    for inv in ([False, True] if invert else [False]):
+        X2o = -X2 if inv else X2
+        I = harmonic_so3_coeffs(X1, X2o, perm, Jmax, nmax, scale, r0)
+        ind, find = find_max(soft.iSOFT(I))
+        R, opt = max_overlap(soft.indtoEuler(find), I.conj(), Jmax)
+        res.ind.append(tuple(int(i) for i in ind))
+        res.fun.append(float(opt.fun))
+        if best is None or opt.fun < best[1]:
+            best = (R, opt.fun, inv)
+    R, _, inv = best
+    X2o = -X2 if inv else X2
+    X2R = X2o.dot(euler_matrix(*R))
+    _, p = find_best_permutation(X1, X2R, perm)
+    dist, MR = kabsch_quaternion(X1, X2R[p])
+    res.dist, res.X2, res.perm, res.inverted, res.R = dist, X2R[p].dot(MR.T), p, inv, R
+    return res
+
+
+# ---------------------------------------------------------------------------
 # Multi-peak search   (fastoverlap/utils.py:347-396,
 #                      fastoverlap/sphericalAlignment.py:196-235)
 # ---------------------------------------------------------------------------
@@ -1047,4 +1146,12 @@ PINNED = {
                   "at operation 0; octahedral image -> 0 at the inverse operation (ohcell=2)",
     "bulk_defaults": "closed form for BLJ256",
     "bulk_align_group": "every cell equals bulk_align of that pair; sym leaves i < j zero",
+    "harmonic_norm": "basis orthonormality by quadrature at r0 = 1",
+    "harmonic_coeffs": "basis orthonormality by quadrature; Laguerre closed form",
+    "harmonic_coeff_table": "rows equal harmonic_coeffs",
+    "radial_integral_harmonic": "numerical quadrature of the defining integral",
+    "harmonic_coeff": "single atom on the z axis: only m = 0, equal to the radial integrals "
+                      "times Y_l0; rotation covariance of |c| per (n, l)",
+    "harmonic_so3_coeffs": "Hermitian identity of a pair with itself; inversion (-1)^l",
+    "harmonic_align": "exact rotated, permuted copy -> 0; inverted copy -> 0 and inverted",
 }

@@ -644,3 +644,83 @@ def test_bulk_align_group_cells_are_pair_alignments():
             assert dist[i, j] == d
             np.testing.assert_array_equal(aligned[i, j], X2)
     assert dist[1, 0] < 1e-9 and dist[0, 0] < 1e-9
+
+
+def test_harmonic_basis_is_orthonormal_and_laguerre():
+    from scipy.integrate import quad
+    from scipy.special import gamma
+    for l in (0, 3):
+        phis = []
+        for n in range(5):
+            g = O.harmonic_coeffs(n, l, 1.0)
+            phis.append(lambda r, g=g: np.polyval(g[::-1], r) * np.exp(-0.5 * r * r))
+            # Laguerre closed form of the coefficients at s = l + 2k
+            a = l + 0.5
+            for k in range(n + 1):
+                v = (-1) ** k * gamma(n + a + 1) / (gamma(n - k + 1) * gamma(a + k + 1)
+                                                    * math.factorial(k))
+                assert g[l + 2 * k] == pytest.approx(v * O.harmonic_norm(n, l, 1.0), rel=1e-10)
+        for i in range(5):
+            for j in range(5):
+                v = quad(lambda r: phis[i](r) * phis[j](r) * r * r, 0, 20, limit=200)[0]
+                assert v == pytest.approx(1.0 if i == j else 0.0, abs=1e-9)
+    np.testing.assert_array_equal(O.harmonic_coeff_table(3, 2, 1.3)[2, 1, :6],
+                                  O.harmonic_coeffs(2, 1, 1.3))
+
+
+def test_harmonic_radial_integral_matches_quadrature():
+    from scipy.integrate import quad
+    from scipy.special import spherical_in
+    for (s, l, rj, sig, r0) in [(0, 0, 1.3, 0.4, 1.1), (4, 2, 0.7, 0.3, 0.8),
+                                (9, 5, 2.2, 0.35, 1.0), (6, 6, 0.0, 0.3, 1.2)]:
+        f = (lambda r: np.exp(-(r * r + rj * rj) / (2 * sig ** 2)) * np.exp(-r * r / (2 * r0 ** 2))
+             * spherical_in(l, r * rj / sig ** 2) * r ** s * r * r)
+        v = quad(f, 0, rj + 12 * sig, limit=400, epsabs=0, epsrel=1e-12)[0]
+        # the closed form carries the 4 pi of the Gaussian's multipole
+        # expansion exp(-|x - r_j|^2 / 2 s^2) = 4 pi e^{..} sum_l i_l Y Y*
+        w = O.radial_integral_harmonic(s, l, rj, sig, r0) / (4 * math.pi)
+        if l > 0 and rj == 0:
+            assert w == 0 and v == 0
+        else:
+            assert w == pytest.approx(v, rel=1e-9)
+
+
+def test_harmonic_coefficients_layout_and_rotation_invariance():
+    rng = np.random.default_rng(41)
+    J, nmax, sig, r0 = 6, 4, 0.35, 1.2
+    # one atom on the z axis: only m = 0 survives, Y_l0 = sqrt((2l+1)/4pi)
+    c = O.harmonic_coeff(np.array([[0.0, 0.0, 0.9]]), J, nmax, sig, r0)
+    tab = O.harmonic_coeff_table(nmax, J, r0)
+    for l in range(J + 1):
+        for n in range(nmax + 1):
+            d = sum(tab[n, l, s] * O.radial_integral_harmonic(s, l, 0.9, sig, r0)
+                    for s in range(2 * nmax + J + 1))
+            assert c[n, l, 0] == pytest.approx(d * math.sqrt((2 * l + 1) / (4 * math.pi)), rel=1e-12)
+        m = np.arange(2 * J + 1) != 0
+        assert np.abs(c[:, l, m]).max() <= 1e-15 * np.abs(c).max()
+    # the power per (n, l) is invariant under rotations
+    X = rng.normal(size=(9, 3))
+    Rm = O.euler_matrix(0.4, 1.1, 2.0)
+    p1 = (np.abs(O.harmonic_coeff(X, J, nmax, sig, r0)) ** 2).sum(2)
+    p2 = (np.abs(O.harmonic_coeff(X @ Rm, J, nmax, sig, r0)) ** 2).sum(2)
+    np.testing.assert_allclose(p1, p2, rtol=1e-10, atol=1e-14 * p1.max())
+    # a structure with itself: Hermitian per degree; -X2 flips odd degrees
+    I = O.harmonic_so3_coeffs(X, X, [np.arange(9)], J, nmax, sig, r0)
+    np.testing.assert_allclose(I, I.transpose(0, 2, 1).conj(), rtol=0, atol=1e-14 * np.abs(I).max())
+    Y = rng.normal(size=(9, 3))
+    Ia = O.harmonic_so3_coeffs(X, Y, [np.arange(9)], J, nmax, sig, r0)
+    Ib = O.harmonic_so3_coeffs(X, -Y, [np.arange(9)], J, nmax, sig, r0)
+    sg = (-1.0) ** np.arange(J + 1)
+    np.testing.assert_allclose(Ib, Ia * sg[:, None, None], rtol=0, atol=1e-13 * np.abs(Ia).max())
+
+
+def test_harmonic_align_exact_copies():
+    rng = np.random.default_rng(42)
+    X = rng.normal(size=(14, 3)) * 1.1
+    perm = rng.permutation(14)
+    Rm = O.euler_matrix(2.1, 0.9, 0.3)
+    r = O.harmonic_align(X, (X @ Rm.T)[perm], 8, 5, scale=0.3, r0=1.0)
+    assert r.dist < 1e-6
+    np.testing.assert_array_equal(perm[r.perm], np.arange(14))
+    r = O.harmonic_align(X, -(X @ Rm.T)[perm], 8, 5, scale=0.3, r0=1.0)
+    assert r.dist < 1e-6 and r.inverted
