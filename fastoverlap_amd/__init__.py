@@ -10,10 +10,12 @@ from .batch import (SphericalBatch, PeriodicBatch, align_spherical_batch,
                     align_periodic_batch, compact_to_dense, dense_to_compact, kcount,
                     next_fast_len, periodic_defaults)
 from .soft import SOFT
-from .spherical import SphericalAlign, SphericalHarmonicAlign, BaseSphericalAlignment
+from .spherical import (SphericalAlign, SphericalHarmonicAlign, SphericalHarmonicAlignFortran,
+                        BaseSphericalAlignment)
 from .periodic import PeriodicAlign, PeriodicAlignFortran, BasePeriodicAlignment
 
-__all__ = ["SphericalAlign", "SphericalHarmonicAlign", "PeriodicAlign", "PeriodicAlignFortran",
+__all__ = ["SphericalAlign", "SphericalHarmonicAlign", "SphericalHarmonicAlignFortran",
+           "PeriodicAlign", "PeriodicAlignFortran",
            "SOFT",
            "SphericalBatch", "PeriodicBatch", "align_spherical_batch", "align_periodic_batch", "FastOverlapError", "Context",
            "default_context", "compact_to_dense", "dense_to_compact", "kcount",

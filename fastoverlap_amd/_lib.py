@@ -81,7 +81,8 @@ _PROTOS = {
     "fo_context_memory": (C.c_int, [_vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
     "fo_sph_batch_create": (C.c_int, [_vp, C.POINTER(SphBatchDesc), C.POINTER(_vp)]),
     "fo_sph_batch_create_harmonic": (C.c_int, [_vp, C.POINTER(SphBatchDesc), C.c_int,
-                                               C.c_double, C.POINTER(_vp)]),
+                                               C.c_double, C.c_int, C.POINTER(_vp)]),
+    "fo_sph_batch_coef_norm2": (C.c_int, [_vp, _pd]),
     "fo_sph_batch_harmonic_coeffs": (C.c_int, [_vp, C.c_int, _pd, _pd]),
     "fo_sph_batch_destroy": (C.c_int, [_vp]),
     "fo_sph_batch_footprint": (C.c_int, [_vp, C.POINTER(C.c_ulonglong)]),

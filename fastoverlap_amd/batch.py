@@ -124,14 +124,17 @@ class SphericalBatch:
         d = _lib.SphBatchDesc(P, self.grid_n, iptr(self._nat), iptr(self._L), dptr(self._scale),
                               iptr(self._ng), iptr(self._gs), iptr(self._ga), flags)
         h = C.c_void_p()
-        # harmonic = (nmax, harmscale): SphericalHarmonicAlign's coefficients
-        self.harmonic = None if harmonic is None else (int(harmonic[0]), float(harmonic[1]))
+        # harmonic = (nmax, harmscale[, basis]): harmonic-oscillator basis
+        # coefficients (basis 0: SphericalHarmonicAlign's, 1: the Fortran
+        # module's HARMONICCOEFFS)
+        self.harmonic = None if harmonic is None else (
+            int(harmonic[0]), float(harmonic[1]), int(harmonic[2]) if len(harmonic) > 2 else 0)
         self._groups = [len(g) for g in groups]
         if self.harmonic is None:
             call("fo_sph_batch_create", self.ctx.handle, C.byref(d), C.byref(h))
         else:
             call("fo_sph_batch_create_harmonic", self.ctx.handle, C.byref(d),
-                 self.harmonic[0], self.harmonic[1], C.byref(h))
+                 self.harmonic[0], self.harmonic[1], self.harmonic[2], C.byref(h))
         self.handle = h
 
     # -- inputs ------------------------------------------------------------
@@ -173,6 +176,12 @@ class SphericalBatch:
         call("fo_sph_batch_harmonic_coeffs", self.handle, int(pair),
              c1.ctypes.data_as(_lib._pd), c2.ctypes.data_as(_lib._pd))
         return _harmonic_dense(c1, L), _harmonic_dense(c2, L)
+
+    def coef_norm2(self):
+        """Per pair: sum of |C^l_{m1 m2}|^2 over the full coefficient set."""
+        out = np.zeros(self.P)
+        call("fo_sph_batch_coef_norm2", self.handle, dptr(out))
+        return out
 
     def set_start(self, R0):
         R0 = as_f64(R0).ravel()

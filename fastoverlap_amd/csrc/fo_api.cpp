@@ -34,9 +34,12 @@ void launch_sph_ylm(hipStream_t, int, const SphPair*, const int*, const double*,
                     const double2*, const double*, double2*, double2*);
 void launch_sph_radial(hipStream_t, int, int, long long, const SphPair*, const SphGroup*, const int*,
                        const double*, const double*, const double*, double*, int);
-void launch_harm_coeffs(hipStream_t, int, int, int, const SphPair*, const SphGroup*, const int*,
-                        const double*, const double*, const double*, int, double, const double*,
-                        const double2*, const double2*, double*, int, double2*, float2*);
+void launch_harm_coeffs(hipStream_t, int, int, int, int, const SphPair*, const SphGroup*,
+                        const int*, const double*, const double*, const double*, int, double, int,
+                        const double*, const double2*, const double2*, double*, int, double2*,
+                        float2*);
+void launch_coef_norm2(hipStream_t, int, const SphPair*, const double2*, double*);
+int harm_block_lx(int L, int nmax, int basis);
 void launch_sph_contract(hipStream_t, int, int, int, const SphPair*, const SphGroup*,
                          const double2*, const double2*, const double*, const double*, double2*,
                          float2*, int);
@@ -508,6 +511,7 @@ struct fo_sph_batch {
   // harmonic-oscillator basis (harm_nmax >= 0): radial degree, length scale,
   // basis polynomial table g[l][n][k] on the device, most groups of a pair
   int harm_nmax = -1;
+  int harm_basis = 0;  // 0: SphericalHarmonicAlign's basis, 1: HARMONICCOEFFS'
   double harm_r0 = 1.0;
   double* gtab = nullptr;
   std::vector<double> hgtab;  // host copy (kept until the upload has run)
@@ -539,9 +543,9 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
       launch_sph_ylm(s, Pc, pdc, b->gath, b->X1c, b->X2c, b->rect->leg, b->rect->cm, b->Y1, b->Y2);
       cmark(3 * c + 1);
       if (b->harm_nmax >= 0) {
-        launch_harm_coeffs(s, Pc, C.Lmax, b->maxgroups, pdc, b->grp, b->gath, b->X1c, b->X2c,
-                           scc, b->harm_nmax, b->harm_r0, b->gtab, b->Y1, b->Y2, b->R, L.n,
-                           b->I64A, b->I32A);
+        launch_harm_coeffs(s, Pc, C.Lmax, b->maxgroups, L.maxgath, pdc, b->grp, b->gath, b->X1c,
+                           b->X2c, scc, b->harm_nmax, b->harm_r0, b->harm_basis, b->gtab, b->Y1,
+                           b->Y2, b->R, L.n, b->I64A, b->I32A);
         cmark(3 * c + 2);
         continue;
       }
@@ -623,7 +627,7 @@ static size_t coef_chunk_bytes() {
 }
 
 int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out,
-                   bool build_tables, int harm_nmax, double harm_r0) {
+                   bool build_tables, int harm_nmax, double harm_r0, int harm_basis) {
   if (!c || !d || !out) return set_err(FO_ERR_INVALID, "NULL argument");
   if (d->npairs < 1) return set_err(FO_ERR_INVALID, "npairs must be >= 1");
   if (!d->natoms || !d->L || !d->ngroups)
@@ -643,6 +647,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   b->flags = d->flags;
   b->harm_nmax = harm_nmax;
   b->harm_r0 = harm_r0;
+  b->harm_basis = harm_basis ? 1 : 0;
   int Lmax = 0, maxatoms = 0, maxgath = 0;
   long long maxjk = 0, Ntot = 0, Ktot = 0, C32tot = 0, radtot = 0, ylmtot = 0;
   long long radmax = 0, ylmmax = 0;  // largest chunk
@@ -706,8 +711,9 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
     // radial block: K_l[j][k] of every group, or for the harmonic basis
     // D[j][l][n] of both structures then c[group][q(l,m)][n] of both
     const long long nn = harm_nmax + 1;
+    const long long lx = harm_nmax >= 0 ? harm_block_lx(Lp, harm_nmax, b->harm_basis) : 0;
     const long long rs = harm_nmax >= 0
-                             ? 2 * ng * (Lp + 1) * nn + 4 * (long long)sp.ngroups * plane_rows(Lp) * nn
+                             ? 2 * ng * (lx + 1) * nn + 4 * (long long)sp.ngroups * plane_rows(Lp) * nn
                              : (long long)(Lp + 1) * jk;
     const long long ys = (long long)(Lp + 1) * (Lp + 2) / 2 * ng;
     b->maxgroups = std::max(b->maxgroups, sp.ngroups);
@@ -800,7 +806,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   o.scr = lay.add<double2>((size_t)P * no * b->scr_stride);
   o.grid = lc.keep_grid ? lay.add<float>((size_t)P * no * n * n * n) : 0;
   o.S = beta_gemm ? lay.add<float4>((size_t)P * n * (Lmax + 1) * n) : 0;
-  o.gtab = harm_nmax >= 0 ? lay.add<double>((size_t)(Lmax + 1) * (harm_nmax + 1) * (harm_nmax + 1)) : 0;
+  o.gtab = harm_nmax >= 0 && !b->harm_basis ? lay.add<double>((size_t)(Lmax + 1) * (harm_nmax + 1) * (harm_nmax + 1)) : 0;
   b->bytes = lay.off;
   *out = b.release();
   return FO_OK;
@@ -856,7 +862,7 @@ int sph_batch_bind(fo_sph_batch* b, void* mem, hipStream_t stream) {
   b->scr = (double2*)(B + o.scr);
   b->grid = b->lc.keep_grid ? (float*)(B + o.grid) : nullptr;
   b->S = o.S ? (float4*)(B + o.S) : nullptr;
-  b->gtab = b->harm_nmax >= 0 ? (double*)(B + o.gtab) : nullptr;
+  b->gtab = b->harm_nmax >= 0 && !b->harm_basis ? (double*)(B + o.gtab) : nullptr;
   if (stream) {
     b->stream = stream;
     b->own_stream = false;
@@ -905,11 +911,12 @@ int fo_sph_batch_create(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch*
 }
 
 int fo_sph_batch_create_harmonic(fo_context* c, const fo_sph_batch_desc* d, int nmax,
-                                 double harmscale, fo_sph_batch** out) {
+                                 double harmscale, int basis, fo_sph_batch** out) {
   if (!out) return set_err(FO_ERR_INVALID, "out is NULL");
   if (nmax < 0) return set_err(FO_ERR_INVALID, "nmax must be >= 0");
+  if (basis != 0 && basis != 1) return set_err(FO_ERR_INVALID, "basis must be 0 or 1");
   fo_sph_batch* b = nullptr;
-  int rc = sph_batch_plan(c, d, &b, true, nmax, harmscale);
+  int rc = sph_batch_plan(c, d, &b, true, nmax, harmscale, basis);
   if (rc) return rc;
   rc = sph_batch_bind(b, nullptr, nullptr);
   if (rc == FO_OK && hipStreamSynchronize(b->stream) != hipSuccess)
@@ -931,13 +938,31 @@ int fo_sph_batch_harmonic_coeffs(fo_sph_batch* b, int pair, double* c1, double* 
   const SphPair& d = b->hp[pair];
   const size_t nn = b->harm_nmax + 1;
   const size_t one = (size_t)d.ngroups * plane_rows(d.L) * nn;
-  const double* base = b->R + d.rad_off + 2 * (size_t)d.ngath * (d.L + 1) * nn;
+  const size_t lx = harm_block_lx(d.L, b->harm_nmax, b->harm_basis);
+  const double* base = b->R + d.rad_off + 2 * (size_t)d.ngath * (lx + 1) * nn;
   FO_HIP(hipSetDevice(b->ctx->device));
   if (c1) FO_HIP(hipMemcpyAsync(c1, base, sizeof(double2) * one, hipMemcpyDeviceToHost, b->stream));
   if (c2)
     FO_HIP(hipMemcpyAsync(c2, base + 2 * one, sizeof(double2) * one, hipMemcpyDeviceToHost,
                           b->stream));
   FO_HIP(hipStreamSynchronize(b->stream));
+  return FO_OK;
+}
+
+int fo_sph_batch_coef_norm2(fo_sph_batch* b, double* norm2) {
+  if (!b || !norm2) return set_err(FO_ERR_INVALID, "NULL argument");
+  if (b->lc.mode != 0) return set_err(FO_ERR_UNSUPPORTED, "one coefficient set per pair only");
+  FO_HIP(hipSetDevice(b->ctx->device));
+  void* buf = nullptr;
+  int rc = ctx_alloc(b->ctx, sizeof(double) * b->lc.P, &buf);
+  if (rc) return rc;
+  launch_coef_norm2(b->stream, b->lc.P, b->pd, b->I64A, (double*)buf);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(norm2, buf, sizeof(double) * b->lc.P, hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  ctx_free(b->ctx, buf, sizeof(double) * b->lc.P);
+  if (e != hipSuccess) return set_err(FO_ERR_HIP, hipGetErrorString(e));
   return FO_OK;
 }
 

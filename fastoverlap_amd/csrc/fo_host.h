@@ -16,7 +16,8 @@ namespace fo {
 // basis of radial degree <= harm_nmax and length scale harm_r0
 // (SphericalHarmonicAlign) instead of contracting the Gaussian overlap.
 int sph_batch_plan(fo_context* ctx, const fo_sph_batch_desc* d, fo_sph_batch** out,
-                   bool build_tables = true, int harm_nmax = -1, double harm_r0 = 1.0);
+                   bool build_tables = true, int harm_nmax = -1, double harm_r0 = 1.0,
+                   int harm_basis = 0);
 int per_batch_plan(fo_context* ctx, const fo_per_batch_desc* d, fo_per_batch** out);
 // Device bytes the planned batch needs.
 size_t sph_batch_bytes(const fo_sph_batch* b);

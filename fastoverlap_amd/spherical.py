@@ -506,3 +506,88 @@ class SphericalHarmonicAlign(BaseSphericalAlignment):
                 aligned[1, k // n, k % n] = r["X2"][k]
             return dists, aligned
         return dists
+
+
+class SphericalHarmonicAlignFortran(BaseSphericalAlignment):
+    """``SphericalHarmonicAlignFortran(scale=0.3, Jmax=15, harmscale=1.0,
+    nmax=20, perm=None, Natoms=None)`` (``sphericalAlignment.py:535-663``):
+    the calls of the reference's f2py wrapper of ``ALIGNHARM`` and
+    ``CALCOVERLAPMATRICES``, with the Fortran module's harmonic basis
+    (``HARMONICCOEFFS``, ``f90/fastclusters.f90:492-687``: normalised
+    oscillator functions of r / harmscale) built on the device.  As the
+    library's Fortran cluster entry point, ``__call__`` refines the best
+    overlap peak (``nrot`` is accepted for signature compatibility)."""
+
+    def __init__(self, scale=0.3, Jmax=15, harmscale=1.0, nmax=20, perm=None, Natoms=None):
+        self.scale = scale
+        self.calcScale = False
+        self.harmscale = harmscale
+        self.nmax = nmax
+        self.Natoms = Natoms
+        self.setJ(Jmax)
+        if perm is not None:
+            self.setPerm(perm)
+        elif Natoms is not None:
+            self.setPerm([np.arange(Natoms)])
+        else:
+            self.perm = perm
+        self.malign = self.__call__
+
+    def setPerm(self, perm):
+        self.Natoms = sum(map(len, perm))
+        self.perm = perm
+
+    def _batch_kw(self):
+        return {"harmonic": (int(self.nmax), float(self.harmscale), 1)}
+
+    def align(self, pos1, pos2, perm=None, invert=True, debug=False):
+        return self(pos1, pos2, perm, invert, 1, debug)
+
+    def __call__(self, pos1, pos2, perm=None, invert=True, nrot=10, debug=False):
+        """(distance, pos1 centred, aligned pos2, rmatbest): rmatbest maps
+        the (inverted, when chosen) centred pos2 onto the result."""
+        if perm is not None:
+            self.setPerm(perm)
+        elif len(pos1) != self.Natoms:
+            self.setPerm([np.arange(len(pos1))])
+        dist, X1, X2 = BaseSphericalAlignment.align(self, pos1, pos2, self.perm, invert)
+        # the batch's rmat maps the centred pos2 (rows) onto X2, inversion
+        # included; RMATBEST acts on columns of the inverted input
+        sg = -1.0 if self.last["inverted"] else 1.0
+        rmat = np.asarray(self.last["rmat"], float).reshape(3, 3) / sg
+        return dist, X1, X2, rmat.T.copy()
+
+    def compareList(self, poslist, perm=None):
+        """(avgoverlap, maxoverlap, navgoverlap, nmaxoverlap) of every pair
+        of ``poslist`` (``sphericalAlignment.py:622-663``,
+        ``CALCOVERLAPMATRICES``): sum |C|^2 and the maximum of the real
+        overlap grid, all pairs i <= j as one device batch."""
+        coords = np.array(poslist, float)
+        nlist, Natoms, dim = coords.shape
+        assert dim == 3
+        if perm is None:
+            if Natoms != self.Natoms:
+                self.Natoms = Natoms
+                self.setPerm([np.arange(self.Natoms)])
+        else:
+            self.setPerm(perm)
+        coords -= coords.mean(1)[:, None, :]
+        groups = [_groups_from_perm(self.perm, Natoms)]
+        pairs = [(i, j) for i in range(nlist) for j in range(i, nlist)]
+        P = len(pairs)
+        L = self.Jmax
+        with SphericalBatch([Natoms] * P, [L] * P, [float(self.scale)] * P, 2 * (L + 1),
+                            groups * P, _lib.SPH_NO_INVERT | _lib.SPH_NO_CENTER,
+                            **self._batch_kw()) as b:
+            b.set_positions([coords[i] for i, _ in pairs], [coords[j] for _, j in pairs])
+            b.run(_lib.STAGE_PREP | _lib.STAGE_COEFF | _lib.STAGE_GRID)
+            norms = b.coef_norm2()
+            maxs = b.peaks()["value"][:, 0]
+        avg = np.zeros((nlist, nlist))
+        mx = np.zeros((nlist, nlist))
+        for k, (i, j) in enumerate(pairs):
+            avg[i, j] = avg[j, i] = norms[k]
+            mx[i, j] = mx[j, i] = maxs[k]
+        da, dm = avg.diagonal(), mx.diagonal()
+        return (avg, mx, avg / np.sqrt(da[:, None] * da[None, :]),
+                mx / np.sqrt(dm[:, None] * dm[None, :]))

@@ -99,11 +99,17 @@ int fo_sph_batch_create(fo_context* ctx, const fo_sph_batch_desc* desc, fo_sph_b
 /* SphericalHarmonicAlign (fastoverlap/sphericalAlignment.py:276-391): the
  * coefficient stage projects each structure's Gaussian density (width =
  * the pair's scale) on the harmonic-oscillator basis of radial degree
- * <= nmax and length scale harmscale (:340-356, utils.py:408-427) and forms
+ * <= nmax and length scale harmscale and forms
  * C^l = sum_groups sum_n conj(c1) c2 (:363-368); every other stage is the
- * batch's own. */
+ * batch's own.  basis 0: the Python class's basis (:340-356,
+ * utils.py:408-427); basis 1: the Fortran module's HARMONICCOEFFS
+ * (f90/fastclusters.f90:492-540, :654-687), normalised oscillator functions
+ * of r / harmscale. */
 int fo_sph_batch_create_harmonic(fo_context* ctx, const fo_sph_batch_desc* desc, int nmax,
-                                 double harmscale, fo_sph_batch** out);
+                                 double harmscale, int basis, fo_sph_batch** out);
+/* per pair, sum over (l, m1, m2) of |C^l_{m1 m2}|^2 of the current
+ * coefficients (CALCSIMILARITY's NORM, f90/fastclusters.f90:805-812) */
+int fo_sph_batch_coef_norm2(fo_sph_batch* b, double* norm2);
 /* harmonic coefficients of one pair after FO_STAGE_COEFF (single-chunk
  * batches): c1, c2 [ngroups][q(l,m)][nmax+1] complex, q(l,m) = l(l+1)/2 + m,
  * 0 <= m <= l <= L (c[l,-m] = (-1)^m conj(c[l,m])) */

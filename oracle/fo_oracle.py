@@ -737,19 +737,28 @@ def harmonic_coeff(pos, Jmax, nmax, scale, r0):
     return np.einsum("nlj,lmj->nlm", dnlj, Y.conj())
 
 
-def harmonic_so3_coeffs(X1, X2, perm, Jmax, nmax, scale, r0):
+def harmonic_so3_coeffs(X1, X2, perm, Jmax, nmax, scale, r0, basis="python"):
     """``calcSO3Coeffs`` summed over groups (``sphericalAlignment.py:358-361``,
-    ``:176``): I[l, m1, m2] = sum_groups sum_n conj(c1[n,l,m1]) c2[n,l,m2]."""
-    return sum(np.einsum("nlm,nlo->lmo", harmonic_coeff(X1[p], Jmax, nmax, scale, r0).conj(),
-                         harmonic_coeff(X2[p], Jmax, nmax, scale, r0)) for p in perm)
+    ``:176``): I[l, m1, m2] = sum_groups sum_n conj(c1[n,l,m1]) c2[n,l,m2];
+    ``basis="fortran"``: the coefficients of ``HARMONICCOEFFS`` as summed by
+    ``ALIGNHARM`` (``fastoverlap/f90/fastclusters.f90:313-331``)."""
+    if basis == "fortran":
+        def cf(X):
+            return harmonic_coeff_fortran(X, Jmax, nmax, r0, scale)
+    else:
+        def cf(X):
+            return harmonic_coeff(X, Jmax, nmax, scale, r0)
+    return sum(np.einsum("nlm,nlo->lmo", cf(X1[p]).conj(), cf(X2[p])) for p in perm)
 
 
 def harmonic_align(pos1, pos2, Jmax, nmax, scale=None, r0=1.0, perm=None, invert=True,
-                   soft=None):
+                   soft=None, basis="python"):
     """``SphericalHarmonicAlign.align`` = ``BaseSphericalAlignment.align``
     (``sphericalAlignment.py:160-188``) with the harmonic-basis coefficients,
     then ``refine`` (``:118-127``); the inverted orientation's coefficients
-    are those of -X2 and replace on a strictly smaller objective."""
+    are those of -X2 and replace on a strictly smaller objective.
+    ``basis="fortran"`` uses ``HARMONICCOEFFS`` (r0 = HWIDTH, scale =
+    KWIDTH) with the same search."""
     pos1 = np.asanyarray(pos1, float).reshape(-1, 3)
     pos2 = np.asanyarray(pos2, float).reshape(-1, 3)
     if scale is None:
@@ -763,7 +772,7 @@ def harmonic_align(pos1, pos2, Jmax, nmax, scale=None, r0=1.0, perm=None, invert
     best = None
     for inv in ([False, True] if invert else [False]):
         X2o = -X2 if inv else X2
-        I = harmonic_so3_coeffs(X1, X2o, perm, Jmax, nmax, scale, r0)
+        I = harmonic_so3_coeffs(X1, X2o, perm, Jmax, nmax, scale, r0, basis)
         ind, find = find_max(soft.iSOFT(I))
         R, opt = max_overlap(soft.indtoEuler(find), I.conj(), Jmax)
         res.ind.append(tuple(int(i) for i in ind))
@@ -777,6 +786,72 @@ def harmonic_align(pos1, pos2, Jmax, nmax, scale=None, r0=1.0, perm=None, invert
     dist, MR = kabsch_quaternion(X1, X2R[p])
     res.dist, res.X2, res.perm, res.inverted, res.R = dist, X2R[p].dot(MR.T), p, inv, R
     return res
+
+
+def harmonic_nl(nmax, L, rj, sigma, r0):
+    """``HARMONICNL`` (``fastoverlap/f90/fastclusters.f90:492-540``): RET[n, l]
+    = 4 pi int g_nl(r) exp(-(r^2 + rj^2)/2 sigma^2) i_l(r rj/sigma^2) r^2 dr for
+    the normalised oscillator functions
+    g_nl = N (r/r0)^l exp(-r^2/2 r0^2) L_n^{l+1/2}(r^2/r0^2), by the
+    reference's recurrence over n (degrees up to L - 2n; row 1 to L - 2)."""
+    ret = np.zeros((nmax + 1, L + 1))
+    rs = 1.0 / (r0 ** 2 + sigma ** 2)
+    ret[0, 0] = (math.sqrt(2.0 * math.sqrt(pi) * (r0 * rs) ** 3) * sigma ** 3
+                 * math.exp(-0.5 * rj ** 2 * rs) * 4 * pi)
+    f0 = math.sqrt(2.0) * r0 * rj * rs
+    for j in range(1, L + 1):
+        ret[0, j] = f0 / math.sqrt(1.0 + 2.0 * j) * ret[0, j - 1]
+    for n in range(1, nmax + 1):
+        for j in range(0, (L - 2 if n == 1 else L - 2 * n) + 1):
+            v = (math.sqrt(n + j + 0.5) * ret[n - 1, j]
+                 - (2.0 * j + 3.0) * sigma ** 2 / rj / r0 * ret[n - 1, j + 1]
+                 - math.sqrt(n + j + 1.5) * ret[n - 1, j + 2])
+            if n >= 2:
+                v += math.sqrt(n - 1.0) * ret[n - 2, j + 2]
+            ret[n, j] = v / math.sqrt(n)
+    return ret
+
+
+def harmonic_coeff_fortran(pos, Jmax, nmax, hwidth, kwidth):
+    """``HARMONICCOEFFS`` (``fastoverlap/f90/fastclusters.f90:654-687``):
+    c[n, l, m % (2J+1)] = sum_j RET_j[n, l] conj(Y_l^m(j))."""
+    pos = np.atleast_2d(np.asarray(pos, float))
+    theta, phi, r = calc_theta_phi_r(pos)
+    Y = sph_harm_table(Jmax, theta, phi)
+    D = np.stack([harmonic_nl(nmax, Jmax + 2 * nmax, rj, kwidth, hwidth)[:, :Jmax + 1]
+                  for rj in r], axis=-1)
+    return np.einsum("nlj,lmj->nlm", D, Y.conj())
+
+
+def harmonic_similarity(c1s, c2s, Jmax, soft=None):
+    """``CALCSIMILARITY`` (``fastoverlap/f90/fastclusters.f90:790-818``) of
+    per-group coefficient lists: (sum |I|^2, max of the real grid of
+    iSOFT(I) at bandwidth Jmax + 1) with I = sum_groups sum_n conj(c1) c2
+    (``DOTHARMONICCOEFFSPERM``, ``:764-788``)."""
+    I = sum(np.einsum("nlm,nlo->lmo", a.conj(), b) for a, b in zip(c1s, c2s))
+    soft = SOFTOracle(Jmax + 1) if soft is None else soft
+    return float((np.abs(I) ** 2).sum()), float(soft.iSOFT(I).real.max())
+
+
+def harmonic_compare_list(coords, Jmax, nmax, hwidth, kwidth, perm=None):
+    """``SphericalHarmonicAlignFortran.compareList`` (``sphericalAlignment.py:
+    622-663``, ``CALCOVERLAPMATRICES``): centred structures, symmetric
+    matrices of sum |I|^2 and maximum overlap, and their geometric-mean
+    normalisations."""
+    coords = np.array(coords, float)
+    coords -= coords.mean(1)[:, None, :]
+    n, N = coords.shape[:2]
+    perm = [np.arange(N)] if perm is None else perm
+    cs = [[harmonic_coeff_fortran(c[p], Jmax, nmax, hwidth, kwidth) for p in perm]
+          for c in coords]
+    soft = SOFTOracle(Jmax + 1)
+    avg, mx = np.zeros((n, n)), np.zeros((n, n))
+    for i in range(n):
+        for j in range(i, n):
+            avg[i, j], mx[i, j] = harmonic_similarity(cs[i], cs[j], Jmax, soft)
+            avg[j, i], mx[j, i] = avg[i, j], mx[i, j]
+    da, dm = avg.diagonal(), mx.diagonal()
+    return avg, mx, avg / np.sqrt(np.outer(da, da)), mx / np.sqrt(np.outer(dm, dm))
 
 
 # ---------------------------------------------------------------------------
@@ -1154,4 +1229,12 @@ PINNED = {
                       "times Y_l0; rotation covariance of |c| per (n, l)",
     "harmonic_so3_coeffs": "Hermitian identity of a pair with itself; inversion (-1)^l",
     "harmonic_align": "exact rotated, permuted copy -> 0; inverted copy -> 0 and inverted",
+    "harmonic_nl": "quadrature of the defining integral with the normalised oscillator "
+                   "functions (Laguerre polynomials from scipy)",
+    "harmonic_coeff_fortran": "single atom on the z axis: only m = 0, RET times Y_l0; "
+                              "rotation invariance of the power per (n, l)",
+    "harmonic_similarity": "sum |I|^2 invariant under rotating either structure (unitary "
+                           "D^l); maximum equals the brute-force grid maximum",
+    "harmonic_compare_list": "symmetric, unit normalised diagonal, rotated copy has the "
+                             "self-overlap norm",
 }

@@ -91,3 +91,45 @@ def test_harmonic_exact_copies_and_group(gpu):
             assert abs(dists[i, j] - o.dist) <= 1e-8
             np.testing.assert_allclose(aligned[1, i, j], o.X2, rtol=0, atol=1e-7)
     assert dists[0, 1] < 1e-6 and dists[1, 0] < 1e-6
+
+
+def test_fortran_basis_coefficients_match_oracle(gpu):
+    print("case: Fortran HARMONICCOEFFS basis, 18 atoms, Jmax 9, nmax 7", flush=True)
+    _, X = _cloud(56, 18)
+    with fo.SphericalBatch([18], [9], [0.32], 20, None,
+                           fo._lib.SPH_NO_INVERT | fo._lib.SPH_NO_CENTER,
+                           harmonic=(7, 1.25, 1)) as b:
+        b.set_positions(X, X)
+        b.run(fo._lib.STAGE_PREP | fo._lib.STAGE_COEFF)
+        c, _ = b.harmonic_coeffs(0)
+    o = O.harmonic_coeff_fortran(X, 9, 7, 1.25, 0.32)
+    np.testing.assert_allclose(c[0], o, rtol=0, atol=1e-10 * np.abs(o).max())
+
+
+def test_compare_list_matches_oracle(gpu):
+    print("case: compareList of five 20-atom structures, Jmax 8, nmax 6", flush=True)
+    rng = np.random.default_rng(57)
+    base = [_cloud(60 + k, 20)[1] for k in range(3)]
+    Rm = O.euler_matrix(1.0, 0.5, 2.5)
+    coords = [base[0], base[0] @ Rm.T, base[1], base[2], base[1][rng.permutation(20)] + 0.4]
+    A = fo.SphericalHarmonicAlignFortran(scale=0.3, Jmax=8, harmscale=1.1, nmax=6)
+    got = A.compareList(coords)
+    exp = O.harmonic_compare_list(coords, 8, 6, 1.1, 0.3)
+    for g, e in zip(got, exp):
+        np.testing.assert_allclose(g, e, rtol=1e-10, atol=0)
+    # a rotated copy has the self-overlap norm; a shifted, permuted copy too
+    assert got[0][0, 1] == pytest.approx(got[0][0, 0], rel=1e-10)
+    assert got[2][2, 4] == pytest.approx(1.0, rel=1e-10)
+
+
+def test_fortran_harmonic_align_matches_oracle(gpu, lj38):
+    print("case: LJ38 through SphericalHarmonicAlignFortran (Jmax 15, nmax 20)", flush=True)
+    p1, p2 = lj38
+    A = fo.SphericalHarmonicAlignFortran(scale=0.3, Jmax=15, harmscale=1.0, nmax=20)
+    d, X1, X2, rmat = A(p1, p2)
+    o = O.harmonic_align(p1, p2, 15, 20, scale=0.3, r0=1.0, basis="fortran")
+    assert abs(d - o.dist) <= 1e-8
+    np.testing.assert_array_equal(A.last["perm"], o.perm)
+    assert A.last["inverted"] == o.inverted
+    X2c = (p2 - p2.mean(0)) * (-1.0 if o.inverted else 1.0)
+    np.testing.assert_allclose(X2, (X2c @ rmat.T)[o.perm], rtol=0, atol=1e-8)

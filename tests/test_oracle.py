@@ -724,3 +724,52 @@ def test_harmonic_align_exact_copies():
     np.testing.assert_array_equal(perm[r.perm], np.arange(14))
     r = O.harmonic_align(X, -(X @ Rm.T)[perm], 8, 5, scale=0.3, r0=1.0)
     assert r.dist < 1e-6 and r.inverted
+
+
+def test_fortran_harmonic_recurrence_matches_quadrature():
+    from scipy.integrate import quad
+    from scipy.special import eval_genlaguerre, gamma, spherical_in
+    N, L, rj, sig, r0 = 4, 6, 1.1, 0.35, 1.3
+    R = O.harmonic_nl(N, L + 2 * N, rj, sig, r0)
+
+    def g(n, l, r):
+        nn = math.sqrt(2 * math.factorial(n) / (r0 ** 3 * gamma(n + l + 1.5)))
+        return nn * (r / r0) ** l * np.exp(-r * r / (2 * r0 * r0)) * \
+            eval_genlaguerre(n, l + 0.5, (r / r0) ** 2)
+    for n in range(N + 1):
+        # the oscillator functions are orthonormal under r^2 dr
+        assert quad(lambda r: g(n, 2, r) ** 2 * r * r, 0, 20)[0] == pytest.approx(1, abs=1e-10)
+        for l in range(0, L + 1, 2):
+            v = 4 * math.pi * quad(lambda r: g(n, l, r) * np.exp(-(r * r + rj * rj) / (2 * sig ** 2))
+                                   * spherical_in(l, r * rj / sig ** 2) * r * r, 0, 15,
+                                   limit=300, epsabs=0, epsrel=1e-12)[0]
+            assert R[n, l] == pytest.approx(v, rel=1e-10)
+
+
+def test_fortran_harmonic_coefficients_and_similarity():
+    rng = np.random.default_rng(61)
+    J, nmax, hw, kw = 6, 4, 1.2, 0.35
+    c = O.harmonic_coeff_fortran(np.array([[0.0, 0.0, 0.8]]), J, nmax, hw, kw)
+    R = O.harmonic_nl(nmax, J + 2 * nmax, 0.8, kw, hw)
+    for l in range(J + 1):
+        np.testing.assert_allclose(c[:, l, 0], R[:, l] * math.sqrt((2 * l + 1) / (4 * math.pi)),
+                                   rtol=1e-12)
+    X = rng.normal(size=(10, 3))
+    X -= X.mean(0)
+    Rm = O.euler_matrix(0.3, 1.2, 2.2)
+    cX = O.harmonic_coeff_fortran(X, J, nmax, hw, kw)
+    cR = O.harmonic_coeff_fortran(X @ Rm.T, J, nmax, hw, kw)
+    np.testing.assert_allclose((np.abs(cX) ** 2).sum(2), (np.abs(cR) ** 2).sum(2), rtol=1e-10)
+    # the norm is unchanged by rotating the partner; the maximum is the grid's
+    soft = O.SOFTOracle(J + 1)
+    n_self, m_self = O.harmonic_similarity([cX], [cX], J, soft)
+    n_rot, m_rot = O.harmonic_similarity([cX], [cR], J, soft)
+    assert n_rot == pytest.approx(n_self, rel=1e-10)
+    I = np.einsum("nlm,nlo->lmo", cX.conj(), cR)
+    assert m_rot == soft.iSOFT(I).real.max()
+    avg, mx, navg, nmx = O.harmonic_compare_list([X, X @ Rm.T, rng.normal(size=(10, 3))], J,
+                                                 nmax, hw, kw)
+    np.testing.assert_array_equal(avg, avg.T)
+    np.testing.assert_allclose(np.diag(navg), 1.0, rtol=1e-15)
+    np.testing.assert_allclose(np.diag(nmx), 1.0, rtol=1e-15)
+    assert avg[0, 1] == pytest.approx(avg[0, 0], rel=1e-10)
