@@ -53,8 +53,8 @@ void launch_beta_gemm(hipStream_t, int, int, int, long long, int, const SphPair*
 void launch_sph_select(hipStream_t, int, int, long long, int, const SphPair*, const double*,
                        const double2*, const double2*, const double2*, int, const PlaneCand*,
                        double2*, long long, const double*, SphPeak*, double);
-void launch_sph_rotopt(hipStream_t, int, int, const SphPair*, const double2*, const double2*, int,
-                       const SphPeak*, const double*, const double*, int, SphOpt*);
+void launch_sph_rotopt(hipStream_t, int, int, int, const SphPair*, const double2*, const double2*,
+                       int, const SphPeak*, const double*, int, SphOpt*);
 void launch_rec_table(hipStream_t, int, double*, double*, double2*, double*);
 void launch_sph_finish(hipStream_t, int, int, const SphPair*, const SphGroup*, const int*,
                        const double*, const double*, const SphOpt*, int, double*, int*, int*,
@@ -578,7 +578,7 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
   }
   mark(6);
   if (stages & FO_STAGE_ROTOPT)
-    launch_sph_rotopt(s, P, L.norient, b->pd, b->I64A, b->I64B, L.mode, b->peaks, b->rect->rec,
+    launch_sph_rotopt(s, P, L.norient, L.Lmax, b->pd, b->I64A, b->I64B, L.mode, b->peaks,
                       b->rect->seedu, kRecL, b->opt);
   mark(7);
   if (stages & FO_STAGE_FINISH)

@@ -1797,11 +1797,43 @@ struct FGH {
   double v[10];  // f, fa, fb, fy, faa, fab, fay, fbb, fby, fyy
 };
 
+// Recurrence coefficients of the unnormalised d from two small tables in
+// LDS instead of the per-entry table: with s_l(m) = sqrt(l^2 - m^2) and
+// e_l = s_l(m1) s_l(m2),
+//   Bu_l = (l+1)(2l+1) / e_{l+1},  Au_l = (l+1)/l e_l / e_{l+1},
+//   C_l = m1 m2 / (l(l+1))        (Au_0 = C_0 = 0),
+// the same values as k_rec_table's (to rounding).  st / ist hold s_l(m) and
+// 1 / s_l(m) (0 for m >= l) for l, m <= Lp1 - 1, row stride Lp1; al / cl
+// hold (l+1)/l and 1/(l(l+1)) (0 at l = 0).
+struct RecLds {
+  const double* st;
+  const double* ist;
+  const double* al;
+  const double* cl;
+  int Lp1;
+};
+FO_DEV size_t rec_lds_doubles(int Lp1) { return 2 * (size_t)Lp1 * Lp1 + 2 * (size_t)Lp1; }
+FO_DEV RecLds build_rec_lds(double* t, int Lp1) {
+  double *st = t, *ist = t + Lp1 * Lp1, *al = ist + Lp1 * Lp1, *cl = al + Lp1;
+  for (int e = threadIdx.x; e < Lp1 * Lp1; e += blockDim.x) {
+    const int l = e / Lp1, m = e % Lp1;
+    const double v = m < l ? sqrt((double)(l * l - m * m)) : 0.0;
+    st[e] = v;
+    ist[e] = m < l ? 1.0 / v : 0.0;
+  }
+  for (int l = threadIdx.x; l < Lp1; l += blockDim.x) {
+    al[l] = l > 0 ? (double)(l + 1) / l : 0.0;
+    cl[l] = l > 0 ? 1.0 / ((double)l * (l + 1)) : 0.0;
+  }
+  return RecLds{st, ist, al, cl, Lp1};
+}
+
 // `ph`: LDS for 3L + 2 phase factors.
 constexpr int kEvalUnroll = 2;  // degrees per batch of operand loads
+template <bool kLdsRec = false>
 FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a, double b,
                      double y, const double* __restrict__ rec, const double* __restrict__ seedu,
-                     int Lt, FGH& out, double* red, double2* ph) {
+                     int Lt, FGH& out, double* red, double2* ph, RecLds rl = RecLds{}) {
   const int W = 2 * L + 1, Wt = 2 * Lt + 1;
   const double cb = cos(b), sb = sin(b), c2 = cos(0.5 * b), s2 = sin(0.5 * b);
   // phase factors e^{-i m1 a} (m1 = 0..L) and e^{-i m2 y} (m2 = -L..L)
@@ -1882,9 +1914,22 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
 #pragma unroll
       for (int q = 0; q < kEvalUnroll; ++q) {
         Iv[q] = I[ids[q]];
-        Au[q] = rec[3 * ids[q]];
-        Bu[q] = rec[3 * ids[q] + 1];
-        C[q] = rec[3 * ids[q] + 2];
+        if constexpr (kLdsRec) {
+          // degree lq = l + q (wavefront-uniform); lanes below their first
+          // degree take harmless values (their steps are skipped)
+          const int lq = min(l + q, L), am1 = m1, am2 = m2 < 0 ? -m2 : m2;
+          const double* s0 = rl.st + lq * rl.Lp1;
+          const double* i1 = rl.ist + (lq + 1) * rl.Lp1;
+          const double ie = i1[am1] * i1[am2];  // 1 / e_{l+1}
+          const double el = s0[am1] * s0[am2];  // e_l
+          Bu[q] = (double)((lq + 1) * (2 * lq + 1)) * ie;
+          Au[q] = rl.al[lq] * el * ie;
+          C[q] = (double)(m1 * m2) * rl.cl[lq];
+        } else {
+          Au[q] = rec[3 * ids[q]];
+          Bu[q] = rec[3 * ids[q] + 1];
+          C[q] = rec[3 * ids[q] + 2];
+        }
         __builtin_amdgcn_sched_barrier(0);  // keep the loads in degree order
       }
 #pragma unroll
@@ -1941,6 +1986,86 @@ FO_DEV bool chol3_solve(const double H[6], double lam, const double g[3], double
   return true;
 }
 
+// Damped Newton iteration on f(R) from the grid peak: one evaluation site
+// (the start, then trial points with backtracking: the step is halved at
+// most 8 times), at most 40 accepted steps.
+struct NewtonState {
+  double x[3], xt[3], dx[3], cur[10];
+  double step, slope;
+  int nev, it, ls, have, stop;
+};
+
+FO_DEV void newton_init(NewtonState& s, const double R0[3]) {
+  for (int q = 0; q < 3; ++q) {
+    s.x[q] = s.xt[q] = R0[q];
+    s.dx[q] = 0.0;
+  }
+  s.step = 1.0;
+  s.slope = 0.0;
+  s.nev = s.it = s.ls = s.have = s.stop = 0;
+}
+
+// Consume the evaluation r at s.xt; sets s.stop or the next s.xt.
+FO_DEV void newton_update(NewtonState& s, const FGH& r) {
+  ++s.nev;
+  if (s.have) {
+    if (!(r.v[0] <= s.cur[0] + 1e-4 * s.step * s.slope)) {
+      // reject: halve the step
+      if (++s.ls >= 8) {
+        s.stop = 1;
+        return;
+      }
+      s.step *= 0.5;
+      for (int q = 0; q < 3; ++q) s.xt[q] = s.x[q] + s.step * s.dx[q];
+      return;
+    }
+    for (int q = 0; q < 3; ++q) s.x[q] = s.xt[q];
+    ++s.it;
+  }
+  for (int q = 0; q < 10; ++q) s.cur[q] = r.v[q];
+  s.have = 1;
+  if (s.it >= 40) {
+    s.stop = 1;
+    return;
+  }
+  const double fscale = fabs(s.cur[0]) + 1e-300;
+  const double g[3] = {s.cur[1], s.cur[2], s.cur[3]};
+  const double gn = fmax(fabs(g[0]), fmax(fabs(g[1]), fabs(g[2])));
+  if (gn <= 1e-12 * fscale) {
+    s.stop = 1;
+    return;
+  }
+  // Newton direction with Levenberg damping until positive definite
+  const double H[6] = {s.cur[4], s.cur[5], s.cur[6], s.cur[7], s.cur[8], s.cur[9]};
+  const double hs = fmax(fabs(H[0]), fmax(fabs(H[3]), fmax(fabs(H[5]), 1e-30)));
+  double dx[3];
+  bool ok = chol3_solve(H, 0.0, g, dx);
+  const bool pure = ok;  // undamped Newton step (positive definite Hessian)
+  for (int k = 0; !ok && k < 12; ++k) ok = chol3_solve(H, hs * pow(10.0, -8.0 + k), g, dx);
+  if (!ok)
+    for (int q = 0; q < 3; ++q) dx[q] = -g[q] / hs;
+  for (int q = 0; q < 3; ++q) s.dx[q] = dx[q];
+  s.slope = g[0] * dx[0] + g[1] * dx[1] + g[2] * dx[2];
+  const double dxn = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
+  if (-0.5 * s.slope <= 1e-13 * fscale || dxn <= 1e-12) {
+    s.stop = 1;
+    return;
+  }
+  // an undamped Newton step this short lands within O(|dx|^2) (~1e-10
+  // rad) of the optimum: take it without evaluating there, with the
+  // objective of the quadratic model (error O(|dx|^3))
+  if (pure && dxn <= 1e-5) {
+    for (int q = 0; q < 3; ++q) s.x[q] += dx[q];
+    s.cur[0] += 0.5 * s.slope;
+    ++s.it;
+    s.stop = 1;
+    return;
+  }
+  s.step = 1.0;
+  s.ls = 0;
+  for (int q = 0; q < 3; ++q) s.xt[q] = s.x[q] + dx[q];
+}
+
 // Converged when the gradient, the Newton model decrease -g.dx/2 or the step
 // reaches the level where rounding noise of f (a sum of ~K(L) terms)
 // dominates: |g| <= 1e-12 |f|, decrease <= 1e-13 |f|, step <= 1e-12 rad.
@@ -1949,88 +2074,44 @@ __global__ void __launch_bounds__(256, 3) k_sph_rotopt(const SphPair* __restrict
                                                     const double2* __restrict__ IA,
                                                     const double2* __restrict__ IB, int mode,
                                                     const SphPeak* __restrict__ peaks, int norient,
-                                                    const double* __restrict__ rec,
                                                     const double* __restrict__ seedu, int Lt,
-                                                    SphOpt* __restrict__ opt, int P) {
+                                                    int Lmax, SphOpt* __restrict__ opt, int P) {
   __shared__ double red[16 * 10];
   __shared__ double2 ph[3 * 64 + 2];
+  extern __shared__ __align__(16) double rtab[];  // rec_lds_doubles(Lmax + 2)
   int p, o;
   if (!xcd_decode(P, norient, p, o)) return;
   const SphPair d = pd[p];
   const int L = d.L;
+  const RecLds rl = build_rec_lds(rtab, Lmax + 2);
+  __syncthreads();
   const double sgn = (mode == 0 && o == 1) ? -1.0 : 1.0;
   const double2* I = ((mode == 1 && o == 1) ? IB : IA) + d.coef_off;
-  const SphPeak pk = peaks[(long long)p * norient + o];
-  double x[3] = {pk.R0[0], pk.R0[1], pk.R0[2]};
-  // one evaluation site: the first evaluation, then Newton trial points
-  // with backtracking (step halved at most 8 times); <= 40 accepted steps
-  FGH cur;
-  double xt[3] = {x[0], x[1], x[2]}, dx[3] = {0.0, 0.0, 0.0};
-  double step = 1.0, slope = 0.0;
-  int nev = 0, it = 0, ls = 0;
-  bool have = false;
+  // the iteration state lives in LDS, updated by thread 0 between
+  // evaluations, so no register holds it across an evaluation
+  __shared__ NewtonState ns;
+  if (threadIdx.x == 0) {
+    const SphPeak pk = peaks[(long long)p * norient + o];
+    newton_init(ns, pk.R0);
+  }
+  __syncthreads();
   for (;;) {
     FGH r;
-    eval_fgh(I, sgn, L, xt[0], xt[1], xt[2], rec, seedu, Lt, r, red, ph);
-    ++nev;
-    if (have) {
-      if (!(r.v[0] <= cur.v[0] + 1e-4 * step * slope)) {
-        // reject: halve the step
-        if (++ls >= 8) break;
-        step *= 0.5;
-        for (int q = 0; q < 3; ++q) xt[q] = x[q] + step * dx[q];
-        continue;
-      }
-      for (int q = 0; q < 3; ++q) x[q] = xt[q];
-      ++it;
-    }
-    cur = r;
-    have = true;
-    if (it >= 40) break;
-    const double fscale = fabs(cur.v[0]) + 1e-300;
-    const double g[3] = {cur.v[1], cur.v[2], cur.v[3]};
-    const double gn = fmax(fabs(g[0]), fmax(fabs(g[1]), fabs(g[2])));
-    if (gn <= 1e-12 * fscale) break;
-    // Newton direction with Levenberg damping until positive definite
-    const double H[6] = {cur.v[4], cur.v[5], cur.v[6], cur.v[7], cur.v[8], cur.v[9]};
-    const double hs = fmax(fabs(H[0]), fmax(fabs(H[3]), fmax(fabs(H[5]), 1e-30)));
-    double lam = 0.0;
-    bool ok = chol3_solve(H, lam, g, dx);
-    const bool pure = ok;  // undamped Newton step (positive definite Hessian)
-    for (int k = 0; !ok && k < 12; ++k) {
-      lam = hs * pow(10.0, -8.0 + k);
-      ok = chol3_solve(H, lam, g, dx);
-    }
-    if (!ok) {
-      dx[0] = -g[0] / hs;
-      dx[1] = -g[1] / hs;
-      dx[2] = -g[2] / hs;
-    }
-    slope = g[0] * dx[0] + g[1] * dx[1] + g[2] * dx[2];
-    const double dxn = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
-    if (-0.5 * slope <= 1e-13 * fscale || dxn <= 1e-12) break;
-    // an undamped Newton step this short lands within O(|dx|^2) (~1e-10
-    // rad) of the optimum: take it without evaluating there, with the
-    // objective of the quadratic model (error O(|dx|^3))
-    if (pure && dxn <= 1e-5) {
-      for (int q = 0; q < 3; ++q) x[q] += dx[q];
-      cur.v[0] += 0.5 * slope;
-      ++it;
-      break;
-    }
-    step = 1.0;
-    ls = 0;
-    for (int q = 0; q < 3; ++q) xt[q] = x[q] + dx[q];
+    eval_fgh<true>(I, sgn, L, ns.xt[0], ns.xt[1], ns.xt[2], nullptr, seedu, Lt, r, red, ph, rl);
+    if (threadIdx.x == 0) newton_update(ns, r);
+    __syncthreads();
+    if (ns.stop) break;
   }
   if (threadIdx.x == 0) {
     SphOpt r;
-    r.R[0] = x[0];
-    r.R[1] = x[1];
-    r.R[2] = x[2];
-    r.f = cur.v[0];
-    r.gnorm = fmax(fabs(cur.v[1]), fmax(fabs(cur.v[2]), fabs(cur.v[3])));  // last evaluated point
-    r.iters = it;
-    r.evals = nev;
+    r.R[0] = ns.x[0];
+    r.R[1] = ns.x[1];
+    r.R[2] = ns.x[2];
+    r.f = ns.cur[0];
+    // gradient at the last evaluated point
+    r.gnorm = fmax(fabs(ns.cur[1]), fmax(fabs(ns.cur[2]), fabs(ns.cur[3])));
+    r.iters = ns.it;
+    r.evals = ns.nev;
     opt[(long long)p * norient + o] = r;
   }
 }
@@ -2457,11 +2538,12 @@ void launch_sph_select(hipStream_t s, int P, int n, long long Kt, int norient, c
                      extra_rel);
 }
 
-void launch_sph_rotopt(hipStream_t s, int P, int norient, const SphPair* pd, const double2* IA,
-                       const double2* IB, int mode, const SphPeak* peaks, const double* rec,
+void launch_sph_rotopt(hipStream_t s, int P, int norient, int Lmax, const SphPair* pd,
+                       const double2* IA, const double2* IB, int mode, const SphPeak* peaks,
                        const double* seedu, int Lt, SphOpt* opt) {
-  hipLaunchKernelGGL(k_sph_rotopt, dim3(xcd_grid(P, norient)), dim3(256), 0, s, pd, IA, IB, mode,
-                     peaks, norient, rec, seedu, Lt, opt, P);
+  const size_t smem = sizeof(double) * (2 * (size_t)(Lmax + 2) * (Lmax + 2) + 2 * (Lmax + 2));
+  hipLaunchKernelGGL(k_sph_rotopt, dim3(xcd_grid(P, norient)), dim3(256), smem, s, pd, IA, IB,
+                     mode, peaks, norient, seedu, Lt, Lmax, opt, P);
 }
 
 void launch_sph_finish(hipStream_t s, int P, int lap_nmax, const SphPair* pd, const SphGroup* grp,
