@@ -145,6 +145,12 @@ _PROTOS = {
                                     C.c_double, C.c_int, C.c_int, C.c_int, _pi, _pi, _pd, _pd,
                                     _pi, _pd]),
     "fo_oh_operation": (C.c_int, [C.c_int, _pd]),
+    "fo_f90_cluster_align_harm": (C.c_int, [_pd, _pd, C.c_int, C.c_int, C.c_int, C.c_double,
+                                            _pd, C.c_int, C.c_int, _pi, _pi, _pd, _pd, _pd, _pi,
+                                            _pi]),
+    "fo_f90_harm_overlap_matrices": (C.c_int, [_pd, C.c_int, C.c_int, C.c_int, C.c_int,
+                                               C.c_double, C.c_double, C.c_int, _pi, _pi, _pd,
+                                               _pd]),
     "fo_f90_bulk_defaults": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, _pd, _pi,
                                        _pi]),
     "fo_f90_bulk_align_group": (C.c_int, [_pd, C.c_int, _pd, C.c_int, C.c_int, C.c_double,

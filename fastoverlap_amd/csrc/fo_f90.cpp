@@ -283,18 +283,13 @@ int bulk_align_pairs(fo_context* ctx, const fo_per_batch_desc& d1, const std::ve
   }
   return FO_OK;
 }
-}  // namespace
 
-extern "C" {
-
-// COORDSB (target) and COORDSA are returned centred; COORDSA is replaced by
-// the aligned, permuted (and possibly inverted) structure.  KWIDTH <= 0
-// selects the automatic width and returns it.  RMATBEST (column-major 3x3)
-// is the rotation applied to the (inverted, if *inverted) COORDSA.
-int fo_f90_cluster_align(double* coordsb, double* coordsa, int natoms, int L, double* kwidth,
-                         int invert, int ngroups, const int* npermsize, const int* permgroup,
-                         double* distance, double* dist2, double* rmatbest, int* bestperm,
-                         int* inverted) {
+// The cluster ALIGN of both coefficient methods: Gaussian overlap
+// (harm_nmax < 0) or the Fortran module's harmonic basis (ALIGNHARM).
+int cluster_align(double* coordsb, double* coordsa, int natoms, int L, double* kwidth,
+                  int harm_nmax, double hwidth, int invert, int ngroups, const int* npermsize,
+                  const int* permgroup, double* distance, double* dist2, double* rmatbest,
+                  int* bestperm, int* inverted) {
   if (!coordsa || !coordsb || !kwidth || natoms < 1)
     return fo::set_error(FO_ERR_INVALID, "bad argument");
   fo_context* ctx = nullptr;
@@ -316,12 +311,15 @@ int fo_f90_cluster_align(double* coordsb, double* coordsa, int natoms, int L, do
   d.group_atoms = atoms.data();
   d.flags = invert ? 0 : FO_SPH_NO_INVERT;
   std::lock_guard<std::mutex> lk(g_call_mu);
-  const std::vector<double> key = shape_key({(double)natoms, (double)L, sc, (double)d.flags}, sizes, atoms);
+  const std::vector<double> key = shape_key(
+      {(double)natoms, (double)L, sc, (double)d.flags, (double)harm_nmax, hwidth}, sizes, atoms);
   const bool reused = g_sph_cache.sph && g_sph_cache.key == key;
   if (!reused) {
     if (g_sph_cache.sph) fo_sph_batch_destroy(g_sph_cache.sph);
     g_sph_cache.sph = nullptr;
-    rc = fo_sph_batch_create(ctx, &d, &g_sph_cache.sph);
+    rc = harm_nmax >= 0 ? fo_sph_batch_create_harmonic(ctx, &d, harm_nmax, hwidth, 1,
+                                                       &g_sph_cache.sph)
+                        : fo_sph_batch_create(ctx, &d, &g_sph_cache.sph);
     if (rc) return rc;
     g_sph_cache.key = key;
   }
@@ -351,6 +349,96 @@ int fo_f90_cluster_align(double* coordsb, double* coordsa, int natoms, int L, do
   if (bestperm)
     for (int i = 0; i < natoms; ++i) bestperm[i] = perm[i] + 1;
   if (inverted) *inverted = inv;
+  return FO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// COORDSB (target) and COORDSA are returned centred; COORDSA is replaced by
+// the aligned, permuted (and possibly inverted) structure.  KWIDTH <= 0
+// selects the automatic width and returns it.  RMATBEST (column-major 3x3)
+// is the rotation applied to the (inverted, if *inverted) COORDSA.
+int fo_f90_cluster_align(double* coordsb, double* coordsa, int natoms, int L, double* kwidth,
+                         int invert, int ngroups, const int* npermsize, const int* permgroup,
+                         double* distance, double* dist2, double* rmatbest, int* bestperm,
+                         int* inverted) {
+  return cluster_align(coordsb, coordsa, natoms, L, kwidth, -1, 1.0, invert, ngroups, npermsize,
+                       permgroup, distance, dist2, rmatbest, bestperm, inverted);
+}
+
+// ALIGNHARM (f90/fastclusters.f90:271-391): as the cluster ALIGN, with the
+// overlap coefficients of the harmonic-oscillator basis of radial degree
+// <= n and length scale hwidth (HARMONICCOEFFS, :654-687).
+int fo_f90_cluster_align_harm(double* coordsb, double* coordsa, int natoms, int n, int L,
+                              double hwidth, double* kwidth, int invert, int ngroups,
+                              const int* npermsize, const int* permgroup, double* distance,
+                              double* dist2, double* rmatbest, int* bestperm, int* inverted) {
+  if (n < 0 || !(hwidth > 0)) return fo::set_error(FO_ERR_INVALID, "bad harmonic basis");
+  return cluster_align(coordsb, coordsa, natoms, L, kwidth, n, hwidth, invert, ngroups,
+                       npermsize, permgroup, distance, dist2, rmatbest, bestperm, inverted);
+}
+
+// CALCOVERLAPMATRICES (f90/fastclusters.f90:853-866): for every pair i <= j
+// of the nlist structures (coordinates used as given, column-major
+// [3 natoms][nlist]) the sum of |C|^2 (NORMS) and the maximum of the real
+// overlap grid at bandwidth L + 1 (MAXOVERS), with the harmonic basis; both
+// matrices symmetric, column-major [nlist][nlist].  All pairs run as one
+// batch.
+int fo_f90_harm_overlap_matrices(const double* coords, int natoms, int nlist, int n, int L,
+                                 double hwidth, double kwidth, int ngroups, const int* npermsize,
+                                 const int* permgroup, double* norms, double* maxovers) {
+  if (!coords || natoms < 1 || nlist < 1 || n < 0 || !(hwidth > 0) || !(kwidth > 0))
+    return fo::set_error(FO_ERR_INVALID, "bad argument");
+  fo_context* ctx = nullptr;
+  int rc = default_ctx(&ctx);
+  if (rc) return rc;
+  std::vector<int> sizes, atoms;
+  rc = groups_from_f90(natoms, ngroups, npermsize, permgroup, sizes, atoms);
+  if (rc) return rc;
+  std::vector<int> pi, pj;
+  for (int i = 0; i < nlist; ++i)
+    for (int j = i; j < nlist; ++j) {
+      pi.push_back(i);
+      pj.push_back(j);
+    }
+  const int P = (int)pi.size();
+  const size_t n3 = 3 * (size_t)natoms;
+  std::vector<int> nat(P, natoms), Ls(P, L), ngr(P, (int)sizes.size()), gs, ga;
+  std::vector<double> sc(P, kwidth), X1((size_t)P * n3), X2((size_t)P * n3);
+  for (int p = 0; p < P; ++p) {
+    gs.insert(gs.end(), sizes.begin(), sizes.end());
+    ga.insert(ga.end(), atoms.begin(), atoms.end());
+    std::copy(coords + pi[p] * n3, coords + (pi[p] + 1) * n3, X1.begin() + p * n3);
+    std::copy(coords + pj[p] * n3, coords + (pj[p] + 1) * n3, X2.begin() + p * n3);
+  }
+  fo_sph_batch_desc d{};
+  d.npairs = P;
+  d.grid_n = 2 * (L + 1);
+  d.natoms = nat.data();
+  d.L = Ls.data();
+  d.scale = sc.data();
+  d.ngroups = ngr.data();
+  d.group_sizes = gs.data();
+  d.group_atoms = ga.data();
+  d.flags = FO_SPH_NO_INVERT | FO_SPH_NO_CENTER;
+  fo_sph_batch* b = nullptr;
+  rc = fo_sph_batch_create_harmonic(ctx, &d, n, hwidth, 1, &b);
+  if (rc) return rc;
+  std::vector<double> nrm(P), val(P);
+  rc = fo_sph_batch_set_positions(b, X1.data(), X2.data());
+  if (!rc) rc = fo_sph_batch_run(b, FO_STAGE_PREP | FO_STAGE_COEFF | FO_STAGE_GRID, 0);
+  if (!rc) rc = fo_sph_batch_coef_norm2(b, nrm.data());
+  if (!rc) rc = fo_sph_batch_peaks(b, nullptr, nullptr, val.data(), nullptr, nullptr, nullptr,
+                                   nullptr);
+  fo_sph_batch_destroy(b);
+  if (rc) return rc;
+  for (int p = 0; p < P; ++p) {
+    const size_t a1 = pi[p] + (size_t)nlist * pj[p], a2 = pj[p] + (size_t)nlist * pi[p];
+    if (norms) norms[a1] = norms[a2] = nrm[p];
+    if (maxovers) maxovers[a1] = maxovers[a2] = val[p];
+  }
   return FO_OK;
 }
 

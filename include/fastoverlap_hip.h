@@ -367,6 +367,21 @@ int fo_f90_cluster_align(double* coordsb, double* coordsa, int natoms, int L, do
                          int invert, int ngroups, const int* npermsize, const int* permgroup,
                          double* distance, double* dist2, double* rmatbest, int* bestperm,
                          int* inverted);
+/* ALIGNHARM (f90/fastclusters.f90:271-391): the cluster ALIGN with the
+ * coefficients of the harmonic-oscillator basis (HARMONICCOEFFS, :654-687)
+ * of radial degree <= n and length scale hwidth; arguments otherwise as
+ * fo_f90_cluster_align. */
+int fo_f90_cluster_align_harm(double* coordsb, double* coordsa, int natoms, int n, int L,
+                              double hwidth, double* kwidth, int invert, int ngroups,
+                              const int* npermsize, const int* permgroup, double* distance,
+                              double* dist2, double* rmatbest, int* bestperm, int* inverted);
+/* CALCOVERLAPMATRICES (f90/fastclusters.f90:853-866): coords column-major
+ * [3 natoms][nlist] used as given; norms (sum |C|^2) and maxovers (maximum
+ * of the real overlap grid at bandwidth L + 1) column-major [nlist][nlist],
+ * symmetric. */
+int fo_f90_harm_overlap_matrices(const double* coords, int natoms, int nlist, int n, int L,
+                                 double hwidth, double kwidth, int ngroups, const int* npermsize,
+                                 const int* permgroup, double* norms, double* maxovers);
 /* ndisps = NDISPLACEMENTS (<= 1: the interpolated overlap maximum; above:
  * that many Gaussian-fitted peaks, each refined, as ALIGNCOEFFS,
  * f90/fastbulk.f90:501-565); ohcell 1 = the COMMONS flag OHCELLT, which also

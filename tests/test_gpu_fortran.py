@@ -2,7 +2,8 @@
 fastoverlap_amd/build.py) against the FP64 oracle: distance within 1e-8 and
 the 1-based BESTPERM equal to the oracle's permutation, for the cluster
 (LJ38) and the bulk (BLJ256) ALIGN, the bulk ALIGN with OHCELLT and the group
-alignment ALIGNGROUP (with CALCDEFAULTS)."""
+alignment ALIGNGROUP (with CALCDEFAULTS), and the harmonic-basis ALIGNHARM
+and CALCOVERLAPMATRICES."""
 
 import os
 import subprocess
@@ -62,3 +63,15 @@ def test_fortran_align_matches_dropin(gpu, lj38, blj256):
     odm, _ = O.bulk_align_group(frames, boxvec=box, perm=groups)
     dm = np.array([float(v) for v in out["GROUP"]]).reshape(3, 3, order="F")
     np.testing.assert_allclose(dm, odm, rtol=0, atol=1e-8)
+    # ALIGNHARM: the Fortran module's harmonic basis (degree 8, scale 1.0)
+    assert int(out["HARM"][1]) == 0
+    oh = O.harmonic_align(p1, p2, 14, 8, scale=0.3, r0=1.0, basis="fortran")
+    assert abs(float(out["HARM"][0]) - oh.dist) <= 1e-8
+    np.testing.assert_array_equal(np.array([int(v) for v in out["HPERM"]]) - 1, oh.perm)
+    # CALCOVERLAPMATRICES of (coords, finish, coords), centred
+    frames = [p1 - p1.mean(0), p2 - p2.mean(0), p1 - p1.mean(0)]
+    avg, mx, _, _ = O.harmonic_compare_list(frames, 10, 8, 1.0, 0.3)
+    norms = np.array([float(v) for v in out["NORMS"]]).reshape(3, 3, order="F")
+    maxov = np.array([float(v) for v in out["MAXOV"]]).reshape(3, 3, order="F")
+    np.testing.assert_allclose(norms, avg, rtol=1e-10)
+    np.testing.assert_allclose(maxov, mx, rtol=1e-10)
