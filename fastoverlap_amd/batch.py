@@ -249,8 +249,8 @@ class SphericalBatch:
         return dict(val32=v32.reshape(sh), val64=v64.reshape(sh), delta=dl.reshape(sh))
 
     def tuning(self):
-        """Launch shapes in use: (plane pairs per XCD chunk, highest degree
-        contracted with 16-row tiles, radial workgroups per pair)."""
+        """Launch shapes in use: (plane pairs per XCD chunk, a reserved entry
+        without effect, radial workgroups per pair)."""
         v = np.zeros(3, np.int32)
         call("fo_sph_batch_tuning", self.handle, iptr(v), 0)
         return tuple(int(x) for x in v)

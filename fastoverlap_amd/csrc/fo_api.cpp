@@ -31,7 +31,7 @@ void launch_twiddles(hipStream_t, int, double, double2*, float2*);
 void launch_sph_prep(hipStream_t, int, const SphPair*, const double*, const double*, double*,
                      double*, double*, int);
 void launch_sph_ylm(hipStream_t, int, const SphPair*, const int*, const double*, const double*,
-                    const double2*, const double*, double2*, double2*);
+                    const double2*, const double*, double2*, double2*, double*, double*);
 void launch_sph_radial(hipStream_t, int, int, long long, const SphPair*, const SphGroup*, const int*,
                        const double*, const double*, const double*, double*, int);
 void launch_harm_coeffs(hipStream_t, int, int, int, int, const SphPair*, const SphGroup*,
@@ -41,8 +41,8 @@ void launch_harm_coeffs(hipStream_t, int, int, int, int, const SphPair*, const S
 void launch_coef_norm2(hipStream_t, int, const SphPair*, const double2*, double*);
 int harm_block_lx(int L, int nmax, int basis);
 void launch_sph_contract(hipStream_t, int, int, int, const SphPair*, const SphGroup*,
-                         const double2*, const double2*, const double*, const double*, double2*,
-                         float2*, int);
+                         const double*, const double*, const double2*, const double2*,
+                         const double*, const double*, double2*, float2*);
 size_t isoft_plane_smem(int, int, int);
 int isoft_plane_use_fft(int, int, size_t);
 void launch_isoft_plane(hipStream_t, int, int, int, int, long long, const SphPair*, const float*,
@@ -473,6 +473,7 @@ struct fo_sph_batch {
   double *pos1 = nullptr, *pos2 = nullptr, *X1c = nullptr, *X2c = nullptr, *X2R = nullptr,
          *X2out = nullptr, *scale = nullptr, *R = nullptr;
   double2 *Y1 = nullptr, *Y2 = nullptr;         // Y_l^m (m >= 0) of the two structures
+  double* rad = nullptr;  // |x| of every grouped atom: structure 1, then structure 2
   double2 *I64A = nullptr, *I64B = nullptr, *scr = nullptr;
   float2 *I32A = nullptr, *I32B = nullptr;
   PlaneCand* cand = nullptr;
@@ -494,8 +495,8 @@ struct fo_sph_batch {
   bool cev_valid = false;       // the last timed run recorded cev
   // layout (byte offsets inside base) and ownership
   struct {
-    size_t pd, grp, gath, pos1, pos2, X1c, X2c, X2R, X2out, scale, R, Y1, Y2, I64A, I32A, I64B,
-        I32B, cand, abound, peaks, opt, res, perm, rowsol, scr, grid, S, gtab;
+    size_t pd, grp, gath, pos1, pos2, X1c, X2c, X2R, X2out, scale, R, Y1, Y2, rad, I64A, I32A,
+        I64B, I32B, cand, abound, peaks, opt, res, perm, rowsol, scr, grid, S, gtab;
   } off{};
   int gsum = 0;
   bool own_mem = false, own_stream = false;
@@ -503,9 +504,9 @@ struct fo_sph_batch {
   // FO_SPH_BETA_GEMM: even / odd-degree beta sums from k_beta_gemm
   // (split-BF16x3 on the matrix cores), [pair][beta][m1 <= Lmax][entry]
   float4* S = nullptr;
-  // launch shapes (arithmetic-neutral): plane pairs per XCD chunk, highest
-  // degree contracted with 16-row tiles (-1: none), radial workgroups per
-  // pair; 0 = default.  autotune: pick them on the first full run.
+  // launch shapes (arithmetic-neutral): plane pairs per XCD chunk, a
+  // reserved entry (kept for the interface, no effect), radial workgroups
+  // per pair; 0 = default.  autotune: pick them on the first full run.
   std::array<int, 3> tune{{0, 15, 0}};
   bool autotune = false, tuned = false;
   // harmonic-oscillator basis (harm_nmax >= 0): radial degree, length scale,
@@ -540,7 +541,8 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
       const SphPair* pdc = b->pd + C.p0;
       const double* scc = b->scale + C.p0;
       cmark(3 * c);
-      launch_sph_ylm(s, Pc, pdc, b->gath, b->X1c, b->X2c, b->rect->leg, b->rect->cm, b->Y1, b->Y2);
+      launch_sph_ylm(s, Pc, pdc, b->gath, b->X1c, b->X2c, b->rect->leg, b->rect->cm, b->Y1, b->Y2,
+                     b->rad, b->rad + b->hgath.size());
       cmark(3 * c + 1);
       if (b->harm_nmax >= 0) {
         launch_harm_coeffs(s, Pc, C.Lmax, b->maxgroups, L.maxgath, pdc, b->grp, b->gath, b->X1c,
@@ -552,8 +554,8 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
       launch_sph_radial(s, Pc, C.Lmax, C.maxjk, pdc, b->grp, b->gath, b->X1c, b->X2c, scc, b->R,
                         b->tune[2]);
       cmark(3 * c + 2);
-      launch_sph_contract(s, Pc, L.n, C.Lmax, pdc, b->grp, b->Y1, b->Y2, b->R, scc, b->I64A,
-                          b->I32A, b->tune[1]);
+      launch_sph_contract(s, Pc, L.n, C.Lmax, pdc, b->grp, b->rad, b->rad + b->hgath.size(),
+                          b->Y1, b->Y2, b->R, scc, b->I64A, b->I32A);
     }
     cmark(3 * b->chunks.size());
   }
@@ -690,7 +692,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
       G.size = sz;
       G.gath_off = ng;
       G.jk_off = jk;
-      G.rad_off = (long long)(Lp + 1) * jk;
+      G.rad_off = (long long)rad_slots(Lp) * jk;
       for (int i = 0; i < sz; ++i) {
         const int a = d->group_atoms[ga + ng + i];
         if (a < 0 || a >= N) return set_err(FO_ERR_INVALID, "group atom index out of range");
@@ -714,7 +716,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
     const long long lx = harm_nmax >= 0 ? harm_block_lx(Lp, harm_nmax, b->harm_basis) : 0;
     const long long rs = harm_nmax >= 0
                              ? 2 * ng * (lx + 1) * nn + 4 * (long long)sp.ngroups * plane_rows(Lp) * nn
-                             : (long long)(Lp + 1) * jk;
+                             : (long long)rad_slots(Lp) * jk;
     const long long ys = (long long)(Lp + 1) * (Lp + 2) / 2 * ng;
     b->maxgroups = std::max(b->maxgroups, sp.ngroups);
     const size_t cb = sizeof(double) * (size_t)(radtot + rs) + 2 * sizeof(double2) * (size_t)(ylmtot + ys);
@@ -792,6 +794,7 @@ int sph_batch_plan(fo_context* c, const fo_sph_batch_desc* d, fo_sph_batch** out
   o.R = lay.add<double>(std::max(radmax, 1LL));
   o.Y1 = lay.add<double2>(std::max(ylmmax, 1LL));
   o.Y2 = lay.add<double2>(std::max(ylmmax, 1LL));
+  o.rad = lay.add<double>(2 * b->hgath.size());
   o.I64A = lay.add<double2>(Ktot);
   o.I32A = lay.add<float2>(C32tot);
   o.I64B = general ? lay.add<double2>(Ktot) : 0;
@@ -848,6 +851,7 @@ int sph_batch_bind(fo_sph_batch* b, void* mem, hipStream_t stream) {
   b->R = (double*)(B + o.R);
   b->Y1 = (double2*)(B + o.Y1);
   b->Y2 = (double2*)(B + o.Y2);
+  b->rad = (double*)(B + o.rad);
   b->I64A = (double2*)(B + o.I64A);
   b->I32A = (float2*)(B + o.I32A);
   b->I64B = general ? (double2*)(B + o.I64B) : nullptr;
@@ -1091,16 +1095,9 @@ static int sph_autotune(fo_sph_batch* b) {
                            b->tab->tw32, b->I32A, b->I32B, L.mode, L.norient, b->cand, b->grid,
                            b->abound, b->tune[0], b->S, L.Lmax + 1);
       } else {
-        for (const CoefChunk& C : b->chunks) {
-          const SphPair* pdc = b->pd + C.p0;
-          const double* scc = b->scale + C.p0;
-          if (knob == 2)
-            launch_sph_radial(s, C.p1 - C.p0, C.Lmax, C.maxjk, pdc, b->grp, b->gath, b->X1c,
-                              b->X2c, scc, b->R, b->tune[2]);
-          else
-            launch_sph_contract(s, C.p1 - C.p0, L.n, C.Lmax, pdc, b->grp, b->Y1, b->Y2, b->R,
-                                scc, b->I64A, b->I32A, b->tune[1]);
-        }
+        for (const CoefChunk& C : b->chunks)
+          launch_sph_radial(s, C.p1 - C.p0, C.Lmax, C.maxjk, b->pd + C.p0, b->grp, b->gath,
+                            b->X1c, b->X2c, b->scale + C.p0, b->R, b->tune[2]);
       }
     };
     launch();
@@ -1112,10 +1109,11 @@ static int sph_autotune(fo_sph_batch* b) {
     FO_HIP(hipEventElapsedTime(ms, e0, e1));
     return FO_OK;
   };
-  const std::vector<int> cand[3] = {
-      {16, 1, 2, 4, 8, 32}, {15, -1, 7}, {1024, 64, 256, 4096}};
+  // knob 1 is reserved (it chose the contraction's tile class below degree
+  // 16, where one fused kernel now runs)
+  const std::vector<int> cand[3] = {{16, 1, 2, 4, 8, 32}, {}, {1024, 64, 256, 4096}};
   for (int knob = 0; knob < 3; ++knob) {
-    if (knob == 0 && !(L.P > 8)) continue;  // one chunk either way
+    if (cand[knob].empty() || (knob == 0 && !(L.P > 8))) continue;  // one chunk either way
     int best = b->tune[knob];
     float best_ms = 0.f;
     bool first = true;

@@ -32,6 +32,23 @@ FO_HD long long kidx(int l, int m1, int m2) {
 // triangular index of (l, m), 0 <= m <= l
 FO_HD int qlm(int l, int m) { return l * (l + 1) / 2 + m; }
 
+// ---- stored radial slots ---------------------------------------------------
+// The radial kernel stores, per (j, k) of a species group, the values of a
+// few degrees ("slots", [slot][j][k]).  Degrees 0..15 are contracted in
+// blocks of eight (0..7, 8..15), each from the two stored degrees at its top
+// by the downward recurrence; degrees >= 16 one per slot.  Slot degrees, in
+// increasing order:
+//   L <= 7 :  L, L+1                    (block 0 = 0..L)
+//   L <= 15:  7, 8, L, L+1              (block 1 = 8..L)
+//   L >= 16:  7, 8, 15, 16, 17, ..., L  (degree l >= 16 in slot l - 13)
+// Block b reads slots 2b (its top degree) and 2b + 1 (one above).
+FO_HD int rad_slots(int L) { return L <= 7 ? 2 : (L <= 15 ? 4 : L - 12); }
+FO_HD int rad_slot_deg(int s, int L) {
+  if (L <= 7) return L + s;
+  if (s < 2) return 7 + s;
+  return L <= 15 ? L + s - 2 : s + 13;
+}
+
 // ---- plane layout (FP32 operands of the iSOFT plane kernels) --------------
 // Rows (l, m1) with 0 <= m1 <= l at row l(l+1)/2 + m1 (l-major, so the rows
 // of degrees <= L are a prefix), n entries per row indexed by the gamma

@@ -189,10 +189,12 @@ static inline unsigned xcd_grid(int P, int per) { return (unsigned)(8 * ((P + 7)
 __global__ void k_sph_ylm(const SphPair* __restrict__ pd, const int* __restrict__ gath,
                           const double* __restrict__ X1c, const double* __restrict__ X2c,
                           const double2* __restrict__ leg, const double* __restrict__ cm,
-                          double2* __restrict__ Y1, double2* __restrict__ Y2) {
+                          double2* __restrict__ Y1, double2* __restrict__ Y2,
+                          double* __restrict__ rad1, double* __restrict__ rad2) {
   const SphPair d = pd[blockIdx.x];
   const double* X = blockIdx.y ? X2c : X1c;
   double2* Yp = (blockIdx.y ? Y2 : Y1) + d.ylm_off;
+  double* rad = (blockIdx.y ? rad2 : rad1) + d.gath_off;
   const int L = d.L, Ng = d.ngath;
   for (int w = threadIdx.x; w < Ng * (L + 1); w += blockDim.x) {
     const int m = w / Ng, g = w % Ng;
@@ -200,6 +202,7 @@ __global__ void k_sph_ylm(const SphPair* __restrict__ pd, const int* __restrict_
     const double x = X[3LL * (d.atom_off + a)], y = X[3LL * (d.atom_off + a) + 1],
                  z = X[3LL * (d.atom_off + a) + 2];
     const double rxy = sqrt(x * x + y * y), r = sqrt(x * x + y * y + z * z);
+    if (m == 0) rad[g] = r;  // |x| per grouped atom, for the fused contraction
     double ct = 1.0, st = 0.0, phi = 0.0;
     if (r > 0.0) {
       ct = z / r;
@@ -252,40 +255,45 @@ __global__ void k_leg_table(int Lt, double2* __restrict__ leg, double* __restric
 // (fastoverlap/sphericalAlignment.py:269-272), evaluated overflow-free as
 // [i_l(x) e^{-x}] exp(-(r1-r2)^2/4s^2) in FP64.  i_l(x) e^{-x}: power series
 // for x < 1; otherwise Miller's backward recurrence started at
-// l_s = ceil(sqrt(L^2 + 40x)) + 10 (where the minimal-solution
-// contamination is below 1e-17) and normalised by i_0(x) e^{-x} =
-// -expm1(-2x)/(2x).  Two passes of the same recurrence: the first finds
-// the normalisation at l = 0, the second stores the values as it goes.
-// Layout: R[rad_off + grp.rad_off + (l*Ns + j)*Ns + k].
+// l_s = ceil(sqrt(T^2 + 40x)) + 10 for the highest stored degree T (where
+// the minimal-solution contamination is below 1e-17) and normalised by
+// i_0(x) e^{-x} = -expm1(-2x)/(2x).  Two passes of the same recurrence: the
+// first finds the normalisation at l = 0, the second stores the slot
+// degrees as it passes them (rad_slot_deg).
+// Layout: R[rad_off + grp.rad_off + (slot*Ns + j)*Ns + k].
 // ---------------------------------------------------------------------------
+FO_DEV double radial_series(int l, double x, double exg) {
+  // i_l(x) e^{-x} g = x^l/(2l+1)!! sum_k (x^2/2)^k / (k! (2l+3)...(2l+2k+1)) e^{-x} g
+  const double h = 0.5 * x * x;
+  double lead = 1.0;
+  for (int i = 1; i <= l; ++i) lead *= x / (2.0 * i + 1.0);
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 40; ++k) {
+    term *= h / (k * (2.0 * l + 2.0 * k + 1.0));
+    sum += term;
+    if (term < 1e-18 * sum) break;
+  }
+  return lead * sum * exg;
+}
 FO_DEV void radial_store(int L, double x, double gauss, double* out, long long stride) {
+  const int ns = rad_slots(L);
   if (x == 0.0) {
-    out[0] = gauss;
-    for (int l = 1; l <= L; ++l) out[l * stride] = 0.0;
+    for (int s = 0; s < ns; ++s) out[s * stride] = rad_slot_deg(s, L) == 0 ? gauss : 0.0;
     return;
   }
   if (x < 1.0) {
-    const double ex = exp(-x), h = 0.5 * x * x;
-    double lead = 1.0;  // x^l / (2l+1)!!
-    for (int l = 0; l <= L; ++l) {
-      if (l > 0) lead *= x / (2.0 * l + 1.0);
-      double term = 1.0, sum = 1.0;
-      for (int k = 1; k < 40; ++k) {
-        term *= h / (k * (2.0 * l + 2.0 * k + 1.0));
-        sum += term;
-        if (term < 1e-18 * sum) break;
-      }
-      out[l * stride] = lead * sum * ex * gauss;
-    }
+    const double exg = exp(-x) * gauss;
+    for (int s = 0; s < ns; ++s) out[s * stride] = radial_series(rad_slot_deg(s, L), x, exg);
     return;
   }
   // i_0(x) e^{-x} times the Gaussian fixes the normalisation of Miller's
   // downward recurrence.  Pass 1 runs the recurrence to l = 0 without
   // storing; pass 2 repeats the identical arithmetic and stores f_l * scal
-  // as it goes (no array of L + 1 values held in registers, so more
+  // at the slot degrees (no array of values held in registers, so more
   // wavefronts per SIMD overlap their stores).
+  const int top = rad_slot_deg(ns - 1, L), bot = rad_slot_deg(0, L);
   const double i0 = -expm1(-2.0 * x) / (2.0 * x);
-  const int ls = (int)ceil(sqrt((double)L * L + 40.0 * x)) + 10;
+  const int ls = (int)ceil(sqrt((double)top * top + 40.0 * x)) + 10;
   const double ix = 1.0 / x;
   double fp1 = 0.0, f = 1.0;
 #pragma unroll 2
@@ -298,21 +306,18 @@ FO_DEV void radial_store(int L, double x, double gauss, double* out, long long s
   fp1 = 0.0;
   f = 1.0;
 #pragma unroll 2
-  for (int l = ls; l > L; --l) {
+  for (int l = ls; l > top; --l) {
     const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
     fp1 = f;
     f = fm;
   }
-  // degrees L .. 0 (L is uniform over the workgroup: a uniform loop)
-  double* o = out + (long long)L * stride;
-  for (int l = L; l >= 0; --l) {
-    *o = f * scal;
-    o -= stride;
-    if (l > 0) {
-      const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
-      fp1 = f;
-      f = fm;
-    }
+  // degrees top .. bot (uniform over the workgroup: L is)
+  int s = ns - 1;
+  for (int l = top; l >= bot; --l) {
+    while (s >= 0 && rad_slot_deg(s, L) == l) out[(s--) * stride] = f * scal;
+    const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
+    fp1 = f;
+    f = fm;
   }
 }
 __global__ void __launch_bounds__(256, 5) k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
@@ -435,7 +440,8 @@ FO_DEV void contract_mfma_body(const SphPair& d, int l, int b, const SphGroup* _
   for (int g = 0; g < d.ngroups; ++g) {
     const SphGroup G = grp[d.group_off + g];
     const int Ns = G.size, go = G.gath_off;
-    const double* Rl = R + d.rad_off + G.rad_off + (long long)l * Ns * Ns;
+    // degree l >= 16 is stored in radial slot l - 13
+    const double* Rl = R + d.rad_off + G.rad_off + (long long)(l - 13) * Ns * Ns;
     // staging roles of this thread:
     //   Y1 chunk:  j = j0 + t % kJC, complex rows t / kJC + kYR i, i < kNY1
     //              (each gives the real rows 2r and 2r + 1)
@@ -649,6 +655,360 @@ __global__ void __launch_bounds__(256, NCTMAX <= 2 ? 5 : 3) k_sph_contract_mfma(
   if (l > L) return;
   const double sc = scale_eff[pp];
   contract_mfma_body<NCTMAX>(d, l, b, grp, Y1, Y2, R, sc, I64, I32, n, S);
+}
+
+// Raw buffer access (gfx9 descriptor): a per-lane byte offset plus a
+// wavefront-uniform one (scalar register); offsets at or beyond `bytes`
+// read zero.
+FO_DEV __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
+}
+FO_DEV float4 load_f4(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const u4 w = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z),
+                     __uint_as_float(w.w));
+}
+
+FO_DEV double load_d(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  const u2 w = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  return __hiloint2double((int)w.y, (int)w.x);
+}
+FO_DEV double2 load_d2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const u4 w = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return make_double2(__hiloint2double((int)w.y, (int)w.x), __hiloint2double((int)w.w, (int)w.z));
+}
+
+// ---------------------------------------------------------------------------
+// Degrees 0..15: radial factors and contraction in one pass.  The radial
+// kernel stored K at the top degree h of each block of eight (block 0:
+// 0..min(7, L), block 1: 8..min(15, L)) and at h + 1 (slots 2b, 2b + 1).
+// One workgroup per (pair, four degrees): quarter s of block b covers
+// degrees h - 4s down to max(8b, h - 4s - 3).  It starts from the stored
+// pair and descends with the radial kernel's recurrence
+// K_{l-1} = K_{l+1} + (2l+1)/x K_l (i_l is the minimal solution, stable
+// downwards), so the K_l of degrees <= 15 never pass through HBM.  Entries
+// with x < 2^-40, where the stored values of degree 16 can be subnormal
+// (x = 0: an atom at the centre), take the power series instead.
+// Per species group and tile of KT atoms k of structure 2:
+//   j in chunks of kJC: the Y1 rows of the four degrees are staged in LDS
+//   (the next chunk is loaded into registers meanwhile); each lane steps
+//   the recurrence of the four (j, k) entries it supplies as the MFMA B
+//   operand and A_l += Y1'_l K_l accumulates per degree in registers.
+//   Waves: block 0 (2(l+1) <= 16 real rows) wave w owns column tile w
+//   (KT = 64); block 1 row tile w >> 1 and column tile w & 1 (KT = 32; both
+//   row tiles step the same entries).
+//   Then per degree: A_l -> LDS, the Y2_l tile -> LDS (its loads issued a
+//   degree ahead), C^l += A_l conj(Y2_l)^T as in k_sph_contract_mfma, one
+//   real or imaginary part of one column tile per wave (block 0: even
+//   degrees on waves 0-1, odd degrees on waves 2-3).
+// Each output's reduction order is fixed: j in order inside A, k in order
+// inside C, groups in order.
+// ---------------------------------------------------------------------------
+constexpr double kTinyX = 0x1p-40;
+constexpr int kJL = 16;  // atoms j per staged chunk
+constexpr int kNS4 = kJL / 4;  // MFMA k-steps per chunk
+// 1/(2l+1)!!, l = 0..15
+__constant__ double kInvDfact[16] = {
+    1.0, 1.0 / 3, 1.0 / 15, 1.0 / 105, 1.0 / 945, 1.0 / 10395, 1.0 / 135135, 1.0 / 2027025,
+    1.0 / 34459425, 1.0 / 654729075, 1.0 / 13749310575.0, 1.0 / 316234143225.0,
+    1.0 / 7905853580625.0, 1.0 / 213458046676875.0, 1.0 / 6190283353629375.0,
+    1.0 / 191898783962510625.0};
+
+template <int BLK>
+struct LowCfg {
+  static constexpr int NRT = BLK ? 2 : 1;     // real-row tiles of A per degree
+  static constexpr int KT = BLK ? 32 : 64;    // atoms k per tile
+  static constexpr int NQ = BLK ? 58 : 26;    // complex Y rows of four degrees (12..15, 4..7)
+  static constexpr int AR = BLK ? 16 : 8;     // complex rows m1 <= l
+  static constexpr int NMC = BLK ? 32 : 16;   // columns m = m2 + l (2l + 1 < NMC)
+  static constexpr int NC = BLK ? 4 : 2;      // C tile parts held per wave
+  static constexpr int NYI = (NQ * kJL + 255) / 256;  // staged Y1 entries per thread
+  static constexpr int NM = BLK ? 16 : 8;     // rows m2 = 0..l of Y2 (l < NM)
+  static constexpr int NYV = KT * NM / 256;           // staged Y2 entries per thread
+};
+
+template <int BLK>
+struct LowLds {
+  using Cf = LowCfg<BLK>;
+  union {
+    double y1[2 * Cf::NQ][kJL + 1];     // Y1 rows of a j chunk, [real row][j]
+    double2 ac[4][Cf::AR][Cf::KT + 1];  // A_l tiles [degree][m1][k]
+  } u;
+  double2 ys[4][Cf::KT][Cf::NM + 1];    // Y2_l rows m2 >= 0 [degree][k][m2]
+  double rj[kJL];                       // |x1_j| of the chunk's atoms (-1: none)
+  double irj[kJL];                      // 1/|x1_j| (0: none)
+};
+
+// A copy of v that the compiler cannot see through: lane-dependent index
+// arithmetic derived from it stays where it is used instead of being hoisted
+// out of the loops (where it would hold registers for the whole loop nest).
+FO_DEV int opaque_lane(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+template <int BLK>
+__global__ void __launch_bounds__(256, 2) k_sph_contract_low(
+    const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
+    const double* __restrict__ rad1, const double* __restrict__ rad2,
+    const double2* __restrict__ Y1, const double2* __restrict__ Y2,
+    const double* __restrict__ R, const double* __restrict__ scale_eff,
+    double2* __restrict__ I64, float2* __restrict__ I32, int n, int P) {
+  using Cf = LowCfg<BLK>;
+  constexpr int KT = Cf::KT;
+  constexpr double kBig = 0x1p600;  // stands for 1/r of an atom at the centre
+  __shared__ LowLds<BLK> S;
+  int pp, qs;
+  if (!xcd_decode(P, 2, pp, qs)) return;
+  const SphPair d = pd[pp];
+  const int lo = 8 * BLK;
+  if (lo > d.L) return;
+  const int h = min(lo + 7, d.L), top = h - 4 * qs;
+  if (top < lo) return;
+  const int bot = max(lo, top - 3), nd = top - bot + 1;
+  const double sc = scale_eff[pp];
+  const double i2s2 = 1.0 / (2.0 * sc * sc), i4s2 = 1.0 / (4.0 * sc * sc), s2x2 = 2.0 * sc * sc;
+  const int t = threadIdx.x, w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int ln = t & 63, col = ln & 15, kq = ln >> 4;
+  const int rt = Cf::NRT == 2 ? (w >> 1) : 0;  // A row tile of this wave
+  const int ct = Cf::NRT == 2 ? (w & 1) : w;   // A column tile
+  const int part = w & 1;                       // C part: 0 real, 1 imaginary
+  const int ct2 = BLK ? (w >> 1) : 0;           // C column tile
+  const int q0 = qlm(bot, 0);
+  // Every workgroup steps four degrees top - dd; those below bot (dd >= nd,
+  // only when L < 15) are computed into accumulators that are never
+  // stored, so the degree loops carry no branches.  LDS offset of this
+  // lane's A operand row per degree: real row 16 rt + col = 2 m1 + (Re, Im)
+  // (rows m1 > l read row l, degrees below bot read bot: never stored)
+  int arow[4];
+#pragma unroll
+  for (int dd = 0; dd < 4; ++dd) {
+    const int l = max(top - dd, bot);
+    arow[dd] = (2 * (qlm(l, min((16 * rt + col) >> 1, l)) - q0) + (col & 1)) * (kJL + 1);
+  }
+  v4d cacc[Cf::NC];
+#pragma unroll
+  for (int q = 0; q < Cf::NC; ++q) cacc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+  const int Ng = d.ngath;
+  // Y1 of the pair (reads past its end return zero); this thread stages
+  // row (t >> 4) + 16 i of the quarter and atom t & 15 of each chunk
+  const __amdgpu_buffer_rsrc_t ry1 = buf_rsrc(Y1 + d.ylm_off, (int)(plane_rows(d.L) * Ng * 16));
+  const int vy1 = ((t >> 4) * Ng + (t & 15)) * 16;
+  for (int g = 0; g < d.ngroups; ++g) {
+    const SphGroup G = grp[d.group_off + g];
+    const int Ns = G.size, go = G.gath_off;
+    const long long ss = (long long)Ns * Ns;
+    // stored K_h and K_{h+1} of the group ([j][k]; reads past the end: zero)
+    const __amdgpu_buffer_rsrc_t rq0 = buf_rsrc(R + d.rad_off + G.rad_off + 2 * BLK * ss, (int)(ss * 8));
+    const __amdgpu_buffer_rsrc_t rq1 =
+        buf_rsrc(R + d.rad_off + G.rad_off + (2 * BLK + 1) * ss, (int)(ss * 8));
+    const double2* Y2g = Y2 + d.ylm_off + go;
+    const double* ra1 = rad1 + d.gath_off + go;  // |x| of the group's atoms
+    const double* ra2 = rad2 + d.gath_off + go;
+    for (int k0 = 0; k0 < Ns; k0 += KT) {
+      const int kk = k0 + 16 * ct + col;  // this lane's B operand column
+      const bool kok = kk < Ns;
+      const double r2 = kok ? ra2[kk] : 0.0;
+      // 1/x = (1/r1) (1/(r2 i2s2)): the second factor per lane, 0 for
+      // columns beyond the group (their entries stay zero)
+      const double ir2 = !kok ? 0.0 : (r2 > 0.0 ? s2x2 / r2 : kBig);
+      // byte offset of entry (4 s4 + kq, kk) of a chunk, past the end for
+      // columns beyond the group
+      const int vq = kok ? (kq * Ns + kk) * 8 : 0x7ff00000;
+      v4d acc[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+      // registers of the next chunk: Y1 entries, the stored K_h, K_{h+1}
+      // of this lane's entries, one radius (-1: no atom)
+      double2 yb[Cf::NYI];
+      double kh[kNS4], kh1[kNS4], rjv = -1.0;
+#define FO_LOW_LOAD_CHUNK(J0)                                                               \
+  {                                                                                         \
+    const int j0_ = (J0);                                                                   \
+    _Pragma("unroll") for (int i = 0; i < Cf::NYI; ++i)                                     \
+      yb[i] = load_d2(ry1, vy1 + ((q0 + 16 * i) * Ng + go + j0_) * 16, 0);                  \
+    _Pragma("unroll") for (int s4 = 0; s4 < kNS4; ++s4) {                                   \
+      const int o_ = vq + (j0_ + 4 * s4) * Ns * 8;                                          \
+      kh[s4] = load_d(rq0, o_, 0);                                                          \
+      kh1[s4] = load_d(rq1, o_, 0);                                                         \
+    }                                                                                       \
+    if (t < kJL) rjv = j0_ + t < Ns ? ra1[j0_ + t] : -1.0;                                  \
+  }
+      const int nch = (Ns + kJL - 1) / kJL;
+      FO_LOW_LOAD_CHUNK(0);
+      // Y2_l rows m2 = 0..l of the four degrees -> LDS (once per tile):
+      // entry k = k0 + e % KT, m2 = e / KT (coalesced in k); the columns
+      // m2 < 0 are formed when read, Y_l^{-m} = (-1)^m conj(Y_l^m)
+      {
+        const int t_ = opaque_lane(t);
+        double2 yv[4][Cf::NYV];
+#pragma unroll
+        for (int dd = 0; dd < 4; ++dd) {
+          const int l_ = top - dd;
+#pragma unroll
+          for (int i = 0; i < Cf::NYV; ++i) {
+            const int e = t_ + 256 * i, k = k0 + e % KT, m2 = e / KT;
+            yv[dd][i] = (dd < nd && m2 <= l_ && k < Ns) ? Y2g[(long long)qlm(l_, m2) * Ng + k]
+                                                         : make_double2(0.0, 0.0);
+          }
+        }
+        __syncthreads();  // readers of the previous tile's Y2 are done
+#pragma unroll
+        for (int dd = 0; dd < 4; ++dd)
+#pragma unroll
+          for (int i = 0; i < Cf::NYV; ++i) {
+            const int e = t_ + 256 * i;
+            S.ys[dd][e % KT][e / KT] = yv[dd][i];  // rows m2 > l hold zeros
+          }
+      }
+      for (int c = 0; c < nch; ++c) {
+        const int j0 = c * kJL;
+        __syncthreads();  // readers of the previous chunk (or tile) are done
+#pragma unroll
+        for (int i = 0; i < Cf::NYI; ++i) {
+          const int qq = (t >> 4) + 16 * i;
+          if (qq < Cf::NQ) {
+            S.u.y1[2 * qq][t & 15] = yb[i].x;
+            S.u.y1[2 * qq + 1][t & 15] = yb[i].y;
+          }
+        }
+        if (t < kJL) {
+          S.rj[t] = rjv;
+          S.irj[t] = rjv > 0.0 ? 1.0 / rjv : (rjv == 0.0 ? kBig : 0.0);
+        }
+        double cur[kNS4], up[kNS4];
+#pragma unroll
+        for (int s4 = 0; s4 < kNS4; ++s4) {
+          cur[s4] = kh[s4];
+          up[s4] = kh1[s4];
+        }
+        __syncthreads();
+        if (c + 1 < nch) FO_LOW_LOAD_CHUNK(j0 + kJL)
+        // entries (j0 + 4 s4 + kq, kk): cx = 1/x (0 for empty entries);
+        // for x < 2^-40 the series, which is x^l/(2l+1)!! e^{-x} g to FP64
+        // precision there (its next term is below 2^-81 relative, and
+        // e^{-x} = 1 - x): cx = e^{-x} g, and the recurrence values are
+        // replaced at every degree
+        double cx[kNS4];
+        int tiny = 0;
+#pragma unroll
+        for (int s4 = 0; s4 < kNS4; ++s4) {
+          cx[s4] = S.irj[4 * s4 + kq] * ir2;
+          if (cx[s4] > 0x1p40) {
+            tiny |= 1 << s4;
+            const double r1 = S.rj[4 * s4 + kq], x = r1 * r2 * i2s2;
+            cx[s4] = (1.0 - x) * exp(-(r1 - r2) * (r1 - r2) * i4s2);
+          }
+        }
+        // degrees above this quarter: recurrence only
+        for (int l = h; l > top; --l) {
+          const double cl = 2.0 * l + 1.0;
+#pragma unroll
+          for (int s4 = 0; s4 < kNS4; ++s4) {
+            const double nx = up[s4] + cl * cx[s4] * cur[s4];
+            up[s4] = cur[s4];
+            cur[s4] = nx;
+          }
+        }
+#pragma unroll
+        for (int dd = 0; dd < 4; ++dd) {
+          const int l = top - dd;
+          if (tiny) {
+            const int la = max(l, 0);
+#pragma unroll
+            for (int s4 = 0; s4 < kNS4; ++s4)
+              if (tiny & (1 << s4))
+                cur[s4] = ipow(S.rj[4 * s4 + kq] * r2 * i2s2, la) * cx[s4] * kInvDfact[la];
+          }
+          double ya[kNS4];
+#pragma unroll
+          for (int s4 = 0; s4 < kNS4; ++s4) ya[s4] = (&S.u.y1[0][0])[arow[dd] + 4 * s4 + kq];
+#pragma unroll
+          for (int s4 = 0; s4 < kNS4; ++s4) acc[dd] = mfma64(ya[s4], cur[s4], acc[dd]);
+          const double cl = 2.0 * l + 1.0;
+#pragma unroll
+          for (int s4 = 0; s4 < kNS4; ++s4) {
+            const double nx = up[s4] + cl * cx[s4] * cur[s4];
+            up[s4] = cur[s4];
+            cur[s4] = nx;
+          }
+        }
+      }
+      // second product: the four A_l tiles -> LDS (aliasing the chunk
+      // buffer), then C^l += A_l conj(Y2_l)^T for every degree
+      {
+        const int tl = opaque_lane(t), cl_ = tl & 15, kq_ = (tl >> 4) & 3;
+        __syncthreads();  // readers of the last chunk are done
+#pragma unroll
+        for (int dd = 0; dd < 4; ++dd) {
+          const int l = top - dd;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int r = 16 * rt + 4 * i + kq_, m1 = r >> 1;
+            if (m1 <= l) ((double*)&S.u.ac[dd][m1][16 * ct + cl_])[r & 1] = acc[dd][i];
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int dd = 0; dd < 4; ++dd) {
+          const int l = top - dd;
+          if (BLK || (w >> 1) == (dd & 1)) {  // uniform
+            v4d& cq = cacc[BLK ? dd : (dd >> 1)];
+            // column m = 16 ct2 + col of C: m2 = m - l, read as row |m2|
+            // (times -1 for odd negative m2, conjugated for negative m2)
+            const int m2 = 16 * ct2 + cl_ - l, am2 = min(abs(m2), Cf::NM - 1);
+            const double sr = (m2 < 0 && (m2 & 1)) ? -1.0 : 1.0;
+            const double si = m2 < 0 ? -sr : sr;
+            const bool rok = cl_ <= l;
+            const int ar = min(cl_, Cf::AR - 1);
+#pragma unroll 4
+            for (int s = 0; s < KT / 4; ++s) {
+              const int kk2 = 4 * s + kq_;
+              const double2 av = rok ? S.u.ac[dd][ar][kk2] : make_double2(0.0, 0.0);
+              const double2 yr = S.ys[dd][kk2][am2];
+              const double a0 = part ? av.y : av.x, a1 = part ? -av.x : av.y;
+              cq = mfma64(a0, sr * yr.x, cq);
+              cq = mfma64(a1, si * yr.y, cq);
+            }
+          }
+        }
+      }
+#undef FO_LOW_LOAD_CHUNK
+    }
+  }
+  // C^l of the four degrees: both parts of an entry through LDS, then one
+  // lane writes it (FP64 compact and FP32 plane layouts)
+  const double norm = 4.0 * pow(kPi, 2.5) * sc * sc * sc;
+#pragma unroll
+  for (int dd = 0; dd < 4; ++dd) {
+    if (dd < nd) {
+      const int l = top - dd, M = 2 * l + 1, tl = opaque_lane(t);
+      __syncthreads();
+      if (BLK || (w >> 1) == (dd & 1)) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = 4 * i + ((tl >> 4) & 3);
+          if (r < Cf::AR)
+            ((double*)&S.u.ac[0][r][16 * ct2 + (tl & 15)])[part] = cacc[BLK ? dd : (dd >> 1)][i];
+        }
+      }
+      __syncthreads();
+      for (int e = tl; e < Cf::AR * Cf::NMC; e += 256) {
+        const int r = e / Cf::NMC, mc = e % Cf::NMC;
+        if (r <= l && mc < M) {
+          const int m2 = mc - l;
+          const double2 cv = S.u.ac[0][r][mc];
+          const double2 v = make_double2(norm * cv.x, norm * cv.y);
+          I64[d.coef_off + kidx(l, r, m2)] = v;
+          I32[d.c32_off + (long long)qlm(l, r) * n + plane_pos(plane_k(m2, n), n)] =
+              make_float2((float)v.x, (float)v.y);
+        }
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -886,19 +1246,6 @@ FO_DEV void two_pass_exchange(float2 (&v)[N2], float2 (&u)[N1], int lane, float2
       }
     wave_sync();
   }
-}
-
-// Raw buffer access (gfx9 descriptor): a per-lane byte offset plus a
-// wavefront-uniform one (scalar register), 16 bytes per lane; offsets at or
-// beyond `bytes` read zero.
-FO_DEV __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, int bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
-}
-FO_DEV float4 load_f4(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-  const u4 w = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-  return make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z),
-                     __uint_as_float(w.w));
 }
 
 template <int N1, int N2>
@@ -2411,8 +2758,9 @@ void launch_sph_prep(hipStream_t s, int P, const SphPair* pd, const double* pos1
 
 void launch_sph_ylm(hipStream_t s, int P, const SphPair* pd, const int* gath, const double* X1c,
                     const double* X2c, const double2* leg, const double* cm, double2* Y1,
-                    double2* Y2) {
-  hipLaunchKernelGGL(k_sph_ylm, dim3(P, 2), dim3(256), 0, s, pd, gath, X1c, X2c, leg, cm, Y1, Y2);
+                    double2* Y2, double* rad1, double* rad2) {
+  hipLaunchKernelGGL(k_sph_ylm, dim3(P, 2), dim3(256), 0, s, pd, gath, X1c, X2c, leg, cm, Y1, Y2,
+                     rad1, rad2);
 }
 
 void launch_sph_radial(hipStream_t s, int P, int Lmax, long long maxjk, const SphPair* pd,
@@ -2427,33 +2775,33 @@ void launch_sph_radial(hipStream_t s, int P, int Lmax, long long maxjk, const Sp
 }
 
 void launch_sph_contract(hipStream_t s, int P, int n, int Lmax, const SphPair* pd,
-                         const SphGroup* grp, const double2* Y1, const double2* Y2,
-                         const double* R, const double* scale_eff, double2* I64, float2* I32,
-                         int split) {
-  // one launch per column-tile class: degrees <= split (at most 15) run with
-  // the smaller tiles (16 rows: five workgroups per CU instead of three),
-  // the rest with 32-row tiles.  Each output's reduction order is the same
-  // in both tile shapes.
+                         const SphGroup* grp, const double* rad1, const double* rad2,
+                         const double2* Y1, const double2* Y2, const double* R,
+                         const double* scale_eff, double2* I64, float2* I32) {
+  // degrees 0..15: radial recurrence and contraction fused, one workgroup
+  // per (pair, block of eight degrees); degrees >= 16: one launch per
+  // column-tile class, workgroups per (pair, degree, block of 32 rows)
+  const dim3 b(256), gl(xcd_grid(P, 2));
+  hipLaunchKernelGGL(k_sph_contract_low<0>, gl, b, 0, s, pd, grp, rad1, rad2, Y1, Y2, R,
+                     scale_eff, I64, I32, n, P);
+  if (Lmax >= 8)
+    hipLaunchKernelGGL(k_sph_contract_low<1>, gl, b, 0, s, pd, grp, rad1, rad2, Y1, Y2, R,
+                       scale_eff, I64, I32, n, P);
   auto launch = [&](int K, int lo, int hi) {
     hi = std::min(hi, Lmax);
     if (lo > hi) return;
     int nb = 0;
     const int MB = K <= 2 ? contract_rows<2>() : contract_rows<4>();
     for (int l = lo; l <= hi; ++l) nb += (l + 1 + MB - 1) / MB;
-    const dim3 g(xcd_grid(P, nb)), b(256);
-    if (K == 2)
-      hipLaunchKernelGGL(k_sph_contract_mfma<2>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,
-                         I32, n, P, nb, lo, hi);
-    else if (K == 4)
+    const dim3 g(xcd_grid(P, nb));
+    if (K == 4)
       hipLaunchKernelGGL(k_sph_contract_mfma<4>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,
                          I32, n, P, nb, lo, hi);
     else
       hipLaunchKernelGGL(k_sph_contract_mfma<8>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,
                          I32, n, P, nb, lo, hi);
   };
-  split = std::min(split, 15);
-  if (split >= 0) launch(2, 0, split);
-  launch(4, split + 1, 31);
+  launch(4, 16, 31);
   launch(8, 32, 63);
 }
 

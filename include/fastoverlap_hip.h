@@ -159,7 +159,7 @@ int fo_sph_batch_norient(const fo_sph_batch* b);
 int fo_sph_batch_timing(fo_sph_batch* b, int enable, float* ms8);
 /* Launch shapes of the batch (no counterpart in the reference: the
  * project's autotuner): params[0] = pairs per XCD chunk of the plane kernel,
- * params[1] = highest degree contracted with 16-row tiles (-1: none),
+ * params[1] = reserved (accepted in [-1, 15], no effect),
  * params[2] = radial workgroups per pair (0: default).  set = 0 reads them
  * (after an FO_SPH_AUTOTUNE batch's first full run: the tuned ones), set = 1
  * imposes them.  Every shape gives identical results. */

@@ -134,17 +134,18 @@ def test_noalloc_refuses_what_it_cannot_hold(gpu):
     hip = _Hip()
     try:
         # both calls fail before any copy or launch; real buffers regardless
-        buf = hip.alloc(24 * 1000)
+        buf = hip.alloc(24 * 1500)
         c = W.config0()
         d, keep = _desc([c], 32)
         rc = _lib.load().fo_sph_align_noalloc(h, C.byref(d), buf, buf,
                                               None, None, None, None, None)
         assert rc == 5, rc  # FO_ERR_STATE
-        # one 1000-atom single-group pair at Jmax 15 needs ~130 MB of radial kernels
+        # one 1500-atom single-group pair at Jmax 15 needs ~72 MB of stored
+        # radial values (four per atom pair) alone
         _lib.call("fo_sph_prepare", h, 32)
         rng = np.random.default_rng(0)
-        X = rng.normal(size=(1000, 3)) * 4.0
-        big = dict(pos1=X, pos2=X, L=15, scale=0.3, groups=[np.arange(1000)])
+        X = rng.normal(size=(1500, 3)) * 4.0
+        big = dict(pos1=X, pos2=X, L=15, scale=0.3, groups=[np.arange(1500)])
         d, keep = _desc([big], 32)
         rc = _lib.load().fo_sph_align_noalloc(h, C.byref(d), buf, buf,
                                               None, None, None, None, None)
