@@ -709,6 +709,7 @@ FO_DEV double2 load_d2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
 // ---------------------------------------------------------------------------
 constexpr double kTinyX = 0x1p-40;
 constexpr int kJL = 16;  // atoms j per staged chunk
+static_assert(kJL == 16, "the chunk staging maps thread t to atom t & 15 of a chunk");
 constexpr int kNS4 = kJL / 4;  // MFMA k-steps per chunk
 // 1/(2l+1)!!, l = 0..15
 __constant__ double kInvDfact[16] = {
