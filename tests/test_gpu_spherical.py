@@ -58,6 +58,44 @@ def test_coefficients_match_oracle(gpu):
     np.testing.assert_allclose(I, ref, atol=1e-12 * np.abs(ref).max())
 
 
+def test_coefficients_across_degree_blocks_and_tiles(gpu):
+    """C^l of one batch whose pairs exercise the coefficient stage's cases:
+    Jmax below, inside and above the two eight-degree blocks that are
+    contracted from two stored radial degrees (0, 3, 7, 8, 11, 15, 16, 25),
+    species groups of 1 to 100 atoms (one or several atom tiles and chunks),
+    atoms exactly at and 1e-14 from the origin (the series branch), against
+    the FP64 oracle's GEMM-chain sum over groups."""
+    rng = np.random.default_rng(7)
+    spec = [(0, [5]), (3, [20]), (7, [30, 10]), (8, [17]), (11, [69, 1]), (15, [33]),
+            (16, [50]), (25, [100, 47])]
+    Ls = [L for L, _ in spec]
+    nat = [sum(g) for _, g in spec]
+    groups, X1s, X2s = [], [], []
+    for p, (L, gs) in enumerate(spec):
+        n = nat[p]
+        X1 = rng.normal(size=(n, 3)) * 1.3
+        X2 = rng.normal(size=(n, 3)) * 1.3
+        if p in (1, 6):
+            X1[0] = 0.0
+            X2[1] = 0.0
+            X1[2] = [1e-14, 0.0, 0.0]
+        off = np.cumsum([0] + gs)
+        groups.append([np.arange(off[i], off[i + 1]) for i in range(len(gs))])
+        X1s.append(X1)
+        X2s.append(X2)
+    say(f"case: coefficient stage, Jmax {Ls}, groups {[g for _, g in spec]}")
+    with SphericalBatch(nat, Ls, 0.35, 64, groups,
+                        _lib.SPH_NO_INVERT | _lib.SPH_NO_CENTER) as b:
+        b.set_positions(np.concatenate(X1s), np.concatenate(X2s))
+        b.run(_lib.STAGE_PREP | _lib.STAGE_COEFF)
+        cs = b.coeffs()
+    for p, L in enumerate(Ls):
+        got = compact_to_dense(cs[p], L)
+        ref = O.so3_coeffs_species(X1s[p], X2s[p], groups[p], L, 0.35, fast=True)
+        np.testing.assert_allclose(got, ref, atol=1e-12 * np.abs(ref).max(),
+                                   err_msg=f"pair {p} Jmax {L}")
+
+
 def test_isoft_general_matches_oracle(gpu):
     say("case: SOFT(8).iSOFT random coefficients")
     rng = np.random.default_rng(2)
