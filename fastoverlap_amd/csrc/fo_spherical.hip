@@ -692,18 +692,18 @@ FO_DEV double2 load_d2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
 // downwards), so the K_l of degrees <= 15 never pass through HBM.  Entries
 // with x < 2^-40, where the stored values of degree 16 can be subnormal
 // (x = 0: an atom at the centre), take the power series instead.
-// Per species group and tile of KT atoms k of structure 2:
-//   j in chunks of kJC: the Y1 rows of the four degrees are staged in LDS
-//   (the next chunk is loaded into registers meanwhile); each lane steps
-//   the recurrence of the four (j, k) entries it supplies as the MFMA B
-//   operand and A_l += Y1'_l K_l accumulates per degree in registers.
-//   Waves: block 0 (2(l+1) <= 16 real rows) wave w owns column tile w
-//   (KT = 64); block 1 row tile w >> 1 and column tile w & 1 (KT = 32; both
-//   row tiles step the same entries).
-//   Then per degree: A_l -> LDS, the Y2_l tile -> LDS (its loads issued a
-//   degree ahead), C^l += A_l conj(Y2_l)^T as in k_sph_contract_mfma, one
-//   real or imaginary part of one column tile per wave (block 0: even
-//   degrees on waves 0-1, odd degrees on waves 2-3).
+// Per species group and tile of KT = 64 atoms k of structure 2:
+//   j in chunks of kJL: the Y1 rows of the four degrees are staged in LDS
+//   (the next chunk is loaded into registers meanwhile); wave w owns the
+//   column tile of atoms k0 + 16 w .. + 15 and every row tile of A (one for
+//   block 0, whose 2(l+1) <= 16 real rows fit one, two for block 1), so
+//   each lane steps the recurrence of the four (j, k) entries it supplies
+//   as the MFMA B operand once for all row tiles, and A_l += Y1'_l K_l
+//   accumulates per degree in registers.
+//   Then two degrees at a time: their A_l tiles and Y2_l rows -> LDS,
+//   C^l += A_l conj(Y2_l)^T as in k_sph_contract_mfma, one real or
+//   imaginary part of one column tile per wave (block 0: even degrees on
+//   waves 0-1, odd degrees on waves 2-3).
 // Each output's reduction order is fixed: j in order inside A, k in order
 // inside C, groups in order.
 // ---------------------------------------------------------------------------
@@ -720,8 +720,8 @@ __constant__ double kInvDfact[16] = {
 
 template <int BLK>
 struct LowCfg {
-  static constexpr int NRT = BLK ? 2 : 1;     // real-row tiles of A per degree
-  static constexpr int KT = BLK ? 32 : 64;    // atoms k per tile
+  static constexpr int RTW = BLK ? 2 : 1;     // real-row tiles of A per degree (per wave)
+  static constexpr int KT = 64;               // atoms k per tile: one column tile per wave
   static constexpr int NQ = BLK ? 58 : 26;    // complex Y rows of four degrees (12..15, 4..7)
   static constexpr int AR = BLK ? 16 : 8;     // complex rows m1 <= l
   static constexpr int NMC = BLK ? 32 : 16;   // columns m = m2 + l (2l + 1 < NMC)
@@ -736,9 +736,9 @@ struct LowLds {
   using Cf = LowCfg<BLK>;
   union {
     double y1[2 * Cf::NQ][kJL + 1];     // Y1 rows of a j chunk, [real row][j]
-    double2 ac[4][Cf::AR][Cf::KT + 1];  // A_l tiles [degree][m1][k]
+    double2 ac[2][Cf::AR][Cf::KT + 1];  // A_l tiles of two degrees [degree][m1][k]
   } u;
-  double2 ys[4][Cf::KT][Cf::NM + 1];    // Y2_l rows m2 >= 0 [degree][k][m2]
+  double2 ys[2][Cf::KT][Cf::NM + 1];    // Y2_l rows m2 >= 0 of two degrees [degree][k][m2]
   double rj[kJL];                       // |x1_j| of the chunk's atoms (-1: none)
   double irj[kJL];                      // 1/|x1_j| (0: none)
 };
@@ -774,21 +774,23 @@ __global__ void __launch_bounds__(256, 2) k_sph_contract_low(
   const double i2s2 = 1.0 / (2.0 * sc * sc), i4s2 = 1.0 / (4.0 * sc * sc), s2x2 = 2.0 * sc * sc;
   const int t = threadIdx.x, w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int ln = t & 63, col = ln & 15, kq = ln >> 4;
-  const int rt = Cf::NRT == 2 ? (w >> 1) : 0;  // A row tile of this wave
-  const int ct = Cf::NRT == 2 ? (w & 1) : w;   // A column tile
+  const int ct = w;                             // A column tile of this wave (all row tiles)
   const int part = w & 1;                       // C part: 0 real, 1 imaginary
   const int ct2 = BLK ? (w >> 1) : 0;           // C column tile
   const int q0 = qlm(bot, 0);
   // Every workgroup steps four degrees top - dd; those below bot (dd >= nd,
   // only when L < 15) are computed into accumulators that are never
   // stored, so the degree loops carry no branches.  LDS offset of this
-  // lane's A operand row per degree: real row 16 rt + col = 2 m1 + (Re, Im)
-  // (rows m1 > l read row l, degrees below bot read bot: never stored)
-  int arow[4];
+  // lane's A operand rows per degree: real row 16 rt + col = 2 m1 + (Re, Im)
+  // of row tile rt (rows m1 > l read row l, degrees below bot read bot:
+  // never stored)
+  int arow[4][Cf::RTW];
 #pragma unroll
   for (int dd = 0; dd < 4; ++dd) {
     const int l = max(top - dd, bot);
-    arow[dd] = (2 * (qlm(l, min((16 * rt + col) >> 1, l)) - q0) + (col & 1)) * (kJL + 1);
+#pragma unroll
+    for (int rt = 0; rt < Cf::RTW; ++rt)
+      arow[dd][rt] = (2 * (qlm(l, min((16 * rt + col) >> 1, l)) - q0) + (col & 1)) * (kJL + 1);
   }
   v4d cacc[Cf::NC];
 #pragma unroll
@@ -819,9 +821,11 @@ __global__ void __launch_bounds__(256, 2) k_sph_contract_low(
       // byte offset of entry (4 s4 + kq, kk) of a chunk, past the end for
       // columns beyond the group
       const int vq = kok ? (kq * Ns + kk) * 8 : 0x7ff00000;
-      v4d acc[4];
+      v4d acc[4][Cf::RTW];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int rt = 0; rt < Cf::RTW; ++rt) acc[q][rt] = v4d{0.0, 0.0, 0.0, 0.0};
       // registers of the next chunk: Y1 entries, the stored K_h, K_{h+1}
       // of this lane's entries, one radius (-1: no atom)
       double2 yb[Cf::NYI];
@@ -840,31 +844,6 @@ __global__ void __launch_bounds__(256, 2) k_sph_contract_low(
   }
       const int nch = (Ns + kJL - 1) / kJL;
       FO_LOW_LOAD_CHUNK(0);
-      // Y2_l rows m2 = 0..l of the four degrees -> LDS (once per tile):
-      // entry k = k0 + e % KT, m2 = e / KT (coalesced in k); the columns
-      // m2 < 0 are formed when read, Y_l^{-m} = (-1)^m conj(Y_l^m)
-      {
-        const int t_ = opaque_lane(t);
-        double2 yv[4][Cf::NYV];
-#pragma unroll
-        for (int dd = 0; dd < 4; ++dd) {
-          const int l_ = top - dd;
-#pragma unroll
-          for (int i = 0; i < Cf::NYV; ++i) {
-            const int e = t_ + 256 * i, k = k0 + e % KT, m2 = e / KT;
-            yv[dd][i] = (dd < nd && m2 <= l_ && k < Ns) ? Y2g[(long long)qlm(l_, m2) * Ng + k]
-                                                         : make_double2(0.0, 0.0);
-          }
-        }
-        __syncthreads();  // readers of the previous tile's Y2 are done
-#pragma unroll
-        for (int dd = 0; dd < 4; ++dd)
-#pragma unroll
-          for (int i = 0; i < Cf::NYV; ++i) {
-            const int e = t_ + 256 * i;
-            S.ys[dd][e % KT][e / KT] = yv[dd][i];  // rows m2 > l hold zeros
-          }
-      }
       for (int c = 0; c < nch; ++c) {
         const int j0 = c * kJL;
         __syncthreads();  // readers of the previous chunk (or tile) are done
@@ -924,11 +903,15 @@ __global__ void __launch_bounds__(256, 2) k_sph_contract_low(
               if (tiny & (1 << s4))
                 cur[s4] = ipow(S.rj[4 * s4 + kq] * r2 * i2s2, la) * cx[s4] * kInvDfact[la];
           }
-          double ya[kNS4];
+          double ya[Cf::RTW][kNS4];
 #pragma unroll
-          for (int s4 = 0; s4 < kNS4; ++s4) ya[s4] = (&S.u.y1[0][0])[arow[dd] + 4 * s4 + kq];
+          for (int rt = 0; rt < Cf::RTW; ++rt)
 #pragma unroll
-          for (int s4 = 0; s4 < kNS4; ++s4) acc[dd] = mfma64(ya[s4], cur[s4], acc[dd]);
+            for (int s4 = 0; s4 < kNS4; ++s4) ya[rt][s4] = (&S.u.y1[0][0])[arow[dd][rt] + 4 * s4 + kq];
+#pragma unroll
+          for (int s4 = 0; s4 < kNS4; ++s4)
+#pragma unroll
+            for (int rt = 0; rt < Cf::RTW; ++rt) acc[dd][rt] = mfma64(ya[rt][s4], cur[s4], acc[dd][rt]);
           const double cl = 2.0 * l + 1.0;
 #pragma unroll
           for (int s4 = 0; s4 < kNS4; ++s4) {
@@ -938,24 +921,46 @@ __global__ void __launch_bounds__(256, 2) k_sph_contract_low(
           }
         }
       }
-      // second product: the four A_l tiles -> LDS (aliasing the chunk
-      // buffer), then C^l += A_l conj(Y2_l)^T for every degree
-      {
+      // second product, two degrees at a time: their Y2_l rows m2 = 0..l
+      // (entry k = k0 + e % KT, m2 = e / KT, coalesced in k; the columns
+      // m2 < 0 are formed when read, Y_l^{-m} = (-1)^m conj(Y_l^m)) and
+      // their A_l tiles -> LDS (the A tiles aliasing the chunk buffer), then
+      // C^l += A_l conj(Y2_l)^T
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
         const int tl = opaque_lane(t), cl_ = tl & 15, kq_ = (tl >> 4) & 3;
-        __syncthreads();  // readers of the last chunk are done
+        double2 yv[2][Cf::NYV];
 #pragma unroll
-        for (int dd = 0; dd < 4; ++dd) {
-          const int l = top - dd;
+        for (int q = 0; q < 2; ++q) {
+          const int dd = 2 * hh + q, l_ = top - dd;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int r = 16 * rt + 4 * i + kq_, m1 = r >> 1;
-            if (m1 <= l) ((double*)&S.u.ac[dd][m1][16 * ct + cl_])[r & 1] = acc[dd][i];
+          for (int i = 0; i < Cf::NYV; ++i) {
+            const int e = tl + 256 * i, k = k0 + e % KT, m2 = e / KT;
+            yv[q][i] = (dd < nd && m2 <= l_ && k < Ns) ? Y2g[(long long)qlm(l_, m2) * Ng + k]
+                                                        : make_double2(0.0, 0.0);
+          }
+        }
+        __syncthreads();  // readers of the last chunk / the previous pair of degrees are done
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int dd = 2 * hh + q, l = top - dd;
+#pragma unroll
+          for (int rt = 0; rt < Cf::RTW; ++rt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int r = 16 * rt + 4 * i + kq_, m1 = r >> 1;
+              if (m1 <= l) ((double*)&S.u.ac[q][m1][16 * ct + cl_])[r & 1] = acc[dd][rt][i];
+            }
+#pragma unroll
+          for (int i = 0; i < Cf::NYV; ++i) {
+            const int e = tl + 256 * i;
+            S.ys[q][e % KT][e / KT] = yv[q][i];  // rows m2 > l hold zeros
           }
         }
         __syncthreads();
 #pragma unroll
-        for (int dd = 0; dd < 4; ++dd) {
-          const int l = top - dd;
+        for (int q = 0; q < 2; ++q) {
+          const int dd = 2 * hh + q, l = top - dd;
           if (BLK || (w >> 1) == (dd & 1)) {  // uniform
             v4d& cq = cacc[BLK ? dd : (dd >> 1)];
             // column m = 16 ct2 + col of C: m2 = m - l, read as row |m2|
@@ -968,8 +973,8 @@ __global__ void __launch_bounds__(256, 2) k_sph_contract_low(
 #pragma unroll 4
             for (int s = 0; s < KT / 4; ++s) {
               const int kk2 = 4 * s + kq_;
-              const double2 av = rok ? S.u.ac[dd][ar][kk2] : make_double2(0.0, 0.0);
-              const double2 yr = S.ys[dd][kk2][am2];
+              const double2 av = rok ? S.u.ac[q][ar][kk2] : make_double2(0.0, 0.0);
+              const double2 yr = S.ys[q][kk2][am2];
               const double a0 = part ? av.y : av.x, a1 = part ? -av.x : av.y;
               cq = mfma64(a0, sr * yr.x, cq);
               cq = mfma64(a1, si * yr.y, cq);
