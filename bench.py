@@ -530,8 +530,8 @@ def main():
     # roofline of the dominant kernel (algorithmic flops / measured time)
     work = {
         "plane": ("k_isoft_fft", sum(isoft_flops(c["L"], c["grid_n"]) for c in pairs), FP32_PEAK),
-        "contract": ("k_sph_contract_mfma", sum(coeff_flops(c["groups"], c["L"]) for c in pairs),
-                     FP64_PEAK),
+        "contract": ("k_sph_contract_low+k_sph_contract_mfma",
+                     sum(coeff_flops(c["groups"], c["L"]) for c in pairs), FP64_PEAK),
         "rotopt": ("k_sph_rotopt", sum(float((e * rotopt_eval_flops(L)[:, None]).sum())
                                        for e, L in zip(evs, Ls)), FP64_PEAK),
     }
@@ -548,7 +548,8 @@ def main():
     compute = {"plane": "FP32 vector ALU; the published 157.3 TFLOP/s counts packed "
                         "(dual) FP32 issue, which this build keeps disabled, so 78.6 "
                         "TFLOP/s is its reachable single-issue ceiling",
-               "contract": "FP64 matrix cores (v_mfma_f64_16x16x4f64)",
+               "contract": "FP64 matrix cores (v_mfma_f64_16x16x4f64; degrees <= 15 with the "
+                           "radial recurrence fused in)",
                "rotopt": "FP64 vector ALU"}
     if dom in work:
         name, fl, peak = work[dom]

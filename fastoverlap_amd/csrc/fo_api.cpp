@@ -32,7 +32,7 @@ void launch_sph_prep(hipStream_t, int, const SphPair*, const double*, const doub
                      double*, double*, int);
 void launch_sph_ylm(hipStream_t, int, const SphPair*, const int*, const double*, const double*,
                     const double2*, const double*, double2*, double2*, double*, double*);
-void launch_sph_radial(hipStream_t, int, int, long long, const SphPair*, const SphGroup*, const int*,
+void launch_sph_radial(hipStream_t, int, int, long long, const SphPair*, const SphGroup*,
                        const double*, const double*, const double*, double*, int);
 void launch_harm_coeffs(hipStream_t, int, int, int, int, const SphPair*, const SphGroup*,
                         const int*, const double*, const double*, const double*, int, double, int,
@@ -551,8 +551,8 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
         cmark(3 * c + 2);
         continue;
       }
-      launch_sph_radial(s, Pc, C.Lmax, C.maxjk, pdc, b->grp, b->gath, b->X1c, b->X2c, scc, b->R,
-                        b->tune[2]);
+      launch_sph_radial(s, Pc, C.Lmax, C.maxjk, pdc, b->grp, b->rad, b->rad + b->hgath.size(), scc,
+                        b->R, b->tune[2]);
       cmark(3 * c + 2);
       launch_sph_contract(s, Pc, L.n, C.Lmax, pdc, b->grp, b->rad, b->rad + b->hgath.size(),
                           b->Y1, b->Y2, b->R, scc, b->I64A, b->I32A);
@@ -1096,8 +1096,8 @@ static int sph_autotune(fo_sph_batch* b) {
                            b->abound, b->tune[0], b->S, L.Lmax + 1);
       } else {
         for (const CoefChunk& C : b->chunks)
-          launch_sph_radial(s, C.p1 - C.p0, C.Lmax, C.maxjk, b->pd + C.p0, b->grp, b->gath,
-                            b->X1c, b->X2c, b->scale + C.p0, b->R, b->tune[2]);
+          launch_sph_radial(s, C.p1 - C.p0, C.Lmax, C.maxjk, b->pd + C.p0, b->grp, b->rad,
+                            b->rad + b->hgath.size(), b->scale + C.p0, b->R, b->tune[2]);
       }
     };
     launch();

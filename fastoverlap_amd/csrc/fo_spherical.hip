@@ -321,9 +321,8 @@ FO_DEV void radial_store(int L, double x, double gauss, double* out, long long s
   }
 }
 __global__ void __launch_bounds__(256, 5) k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
-                             const int* __restrict__ gath, const double* __restrict__ X1c,
-                             const double* __restrict__ X2c, const double* __restrict__ scale_eff,
-                             double* __restrict__ R) {
+                             const double* __restrict__ rad1, const double* __restrict__ rad2,
+                             const double* __restrict__ scale_eff, double* __restrict__ R) {
   const SphPair d = pd[blockIdx.y];
   const double s = scale_eff[blockIdx.y];
   const double i2s2 = 1.0 / (2.0 * s * s), i4s2 = 1.0 / (4.0 * s * s);
@@ -335,14 +334,11 @@ __global__ void __launch_bounds__(256, 5) k_sph_radial(const SphPair* __restrict
     int q = 0;
     while (q + 1 < d.ngroups && w >= g[q + 1].jk_off) ++q;
     const int Ns = g[q].size;
-    const long long jk = w - g[q].jk_off;
-    const int j = (int)(jk / Ns), k = (int)(jk % Ns);
-    const int a1 = gath[d.gath_off + g[q].gath_off + j];
-    const int a2 = gath[d.gath_off + g[q].gath_off + k];
-    const double* x1 = X1c + 3LL * (d.atom_off + a1);
-    const double* x2 = X2c + 3LL * (d.atom_off + a2);
-    const double r1 = sqrt(x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2]);
-    const double r2 = sqrt(x2[0] * x2[0] + x2[1] * x2[1] + x2[2] * x2[2]);
+    const int jk = (int)(w - g[q].jk_off);  // < Ns^2 <= 2^24
+    const int j = jk / Ns, k = jk - j * Ns;
+    // |x| of the two atoms, as the harmonics kernel recorded them
+    const double r1 = rad1[d.gath_off + g[q].gath_off + j];
+    const double r2 = rad2[d.gath_off + g[q].gath_off + k];
     const double x = r1 * r2 * i2s2;
     const double gauss = exp(-(r1 - r2) * (r1 - r2) * i4s2);
     radial_store(d.L, x, gauss, R + d.rad_off + g[q].rad_off + (long long)j * Ns + k,
@@ -2770,14 +2766,14 @@ void launch_sph_ylm(hipStream_t s, int P, const SphPair* pd, const int* gath, co
 }
 
 void launch_sph_radial(hipStream_t s, int P, int Lmax, long long maxjk, const SphPair* pd,
-                       const SphGroup* grp, const int* gath, const double* X1c,
-                       const double* X2c, const double* scale_eff, double* R, int bx_cap) {
+                       const SphGroup* grp, const double* rad1, const double* rad2,
+                       const double* scale_eff, double* R, int bx_cap) {
   // workgroups per pair: one per 256 (j, k) entries up to bx_cap (the rest
   // is a grid-stride loop); every entry is computed the same way
   const int bx = std::max(1, std::min(cdiv(maxjk, 256), bx_cap > 0 ? bx_cap : 1024));
   (void)Lmax;  // one kernel for every degree
-  hipLaunchKernelGGL(k_sph_radial, dim3(bx, P), dim3(256), 0, s, pd, grp, gath, X1c, X2c,
-                     scale_eff, R);
+  hipLaunchKernelGGL(k_sph_radial, dim3(bx, P), dim3(256), 0, s, pd, grp, rad1, rad2, scale_eff,
+                     R);
 }
 
 void launch_sph_contract(hipStream_t s, int P, int n, int Lmax, const SphPair* pd,
