@@ -2196,16 +2196,31 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
   }
   __syncthreads();
   double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  // entries (m1 >= 0, m2) in shell order: shell l0 = max(m1, |m2|) holds
-  // 4 l0 + 1 entries from e = l0 (2 l0 - 1) on (m1 = l0, m2 = -l0..l0, then
-  // m1 < l0 with m2 = +-l0), so the lanes of a wavefront start their degree
-  // recurrences at nearly the same l and the loop over l is wavefront-uniform
-  const int ne = (L + 1) * W;
+  // The sum runs over the entries (m1 >= 0, m2).  d and its b-derivatives
+  // obey d_{m1,m2} = (-1)^{m1-m2} d_{m2,m1} = d_{-m2,-m1}, so one recurrence
+  // serves two entries: item (l0, m2), |m2| <= l0 (the row m1 = l0 of shell
+  // l0), also carries (m2, l0) for 0 <= m2 < l0 (sign (-1)^{l0-m2}) or
+  // (-m2, -l0) for -l0 < m2 < 0; the entries (0, -l0), l0 >= 1, are items of
+  // their own after the (L+1)^2 shell items.  Items in shell order, so the
+  // lanes of a wavefront start their recurrences at nearly the same l and
+  // the loop over l is wavefront-uniform.
+  const int nw = (L + 1) * (L + 1), ne = nw + L;
   for (int e0 = 0; e0 < ne; e0 += blockDim.x) {
     const int e = e0 + threadIdx.x;
     const bool act = e < ne;
     int l0 = L + 1, m1 = 0, m2 = 0;
-    if (act) shell_entry(e, l0, m1, m2);
+    if (act && e < nw) {
+      l0 = (int)sqrt((double)e);
+      while (l0 * l0 > e) --l0;
+      while ((l0 + 1) * (l0 + 1) <= e) ++l0;
+      m1 = l0;
+      m2 = e - l0 * l0 - l0;
+    } else if (act) {
+      l0 = e - nw + 1;
+      m2 = -l0;
+    }
+    const bool hasp = act && e < nw && m2 > -l0 && m2 < l0;
+    const int pm1 = m2 >= 0 ? m2 : -m2, pm2 = m2 >= 0 ? l0 : -l0;
     const int mu = abs(m1 - m2), nu = abs(m1 + m2);
     // edge element g = s2^mu c2^nu and its b-derivatives
     const double fct = seedu[(m1 + Lt) * Wt + (m2 + Lt)];
@@ -2220,18 +2235,26 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
                               (mu * (nu + 1.0) + nu * (mu + 1.0)) * smu * cnu +
                               nu * (nu - 1) * smu * s2 * s2 * cnu2);
     double c0x = 0, c0y = 0, c1x = 0, c1y = 0, c2x = 0, c2y = 0;
+    double q0x = 0, q0y = 0, q1x = 0, q1y = 0, q2x = 0, q2y = 0;  // partner
     double d0 = fct * g0, d1 = fct * g1, dd2 = fct * g2;
     double p0 = 0, p1 = 0, p2 = 0;
     double sl = (l0 & 1) ? sgn : 1.0;
-    auto step = [&](int l, const double2& Iv, double Au, double Bu, double C) {
+    auto step = [&](int l, const double2& Iv, const double2& Pv, double Au, double Bu, double C) {
       // conj(sgn^l I) * d
       const double ix = sl * Iv.x, iy = -sl * Iv.y;
+      const double jx = sl * Pv.x, jy = -sl * Pv.y;
       c0x += d0 * ix;
       c0y += d0 * iy;
       c1x += d1 * ix;
       c1y += d1 * iy;
       c2x += dd2 * ix;
       c2y += dd2 * iy;
+      q0x += d0 * jx;
+      q0y += d0 * jy;
+      q1x += d1 * jx;
+      q1y += d1 * jy;
+      q2x += dd2 * jx;
+      q2y += dd2 * jy;
       sl *= sgn;
       if (l == L) return;
       const double bc = Bu * (cb - C);
@@ -2245,24 +2268,33 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
       d1 = n1;
       dd2 = n2;
     };
-    // operands (coefficient, recurrence terms to l + 1) of kEvalUnroll
-    // degrees at a time, all loads issued before the steps.  The loaded degree is
-    // max(l, l0) clamped to L (the table covers every l <= Lt; terms of
+    // operands (coefficients of the item and its partner, recurrence terms
+    // to l + 1) of kEvalUnroll degrees at a time, all loads issued before
+    // the steps.  The loaded degree is max(l, l0) clamped to L (terms of
     // degree L are not used).
-    // compact index of degree l + 1 from that of l (same m1, m2)
     auto next_id = [&](int id, int l) { return next_kidx(id, l, l0, m1, L); };
-    const int lw = __builtin_amdgcn_readfirstlane(l0);  // smallest l0 of the wavefront
+    // smallest l0 of the wavefront (the items (0, -l0) that follow the
+    // shells can share a wavefront with shell L)
+    int lmin = l0;
+    for (int off = 32; off > 0; off >>= 1) lmin = min(lmin, __shfl_xor(lmin, off, 64));
+    const int lw = __builtin_amdgcn_readfirstlane(lmin);
     int id = act ? (int)kidx(l0, m1, m2) : 0;
+    int pid = hasp ? (int)kidx(l0, pm1, pm2) : 0;
     for (int l = lw; l <= L; l += kEvalUnroll) {  // wavefront-uniform
-      int ids[kEvalUnroll];
-      double2 Iv[kEvalUnroll];
+      int ids[kEvalUnroll], pids[kEvalUnroll];
+      double2 Iv[kEvalUnroll], Pv[kEvalUnroll];
       double Au[kEvalUnroll], Bu[kEvalUnroll], C[kEvalUnroll];
       ids[0] = id;
+      pids[0] = pid;
 #pragma unroll
-      for (int q = 1; q < kEvalUnroll; ++q) ids[q] = next_id(ids[q - 1], l + q - 1);
+      for (int q = 1; q < kEvalUnroll; ++q) {
+        ids[q] = next_id(ids[q - 1], l + q - 1);
+        pids[q] = hasp ? next_kidx(pids[q - 1], l + q - 1, l0, pm1, L) : 0;
+      }
 #pragma unroll
       for (int q = 0; q < kEvalUnroll; ++q) {
         Iv[q] = I[ids[q]];
+        Pv[q] = I[pids[q]];
         if constexpr (kLdsRec) {
           // degree lq = l + q (wavefront-uniform); lanes below their first
           // degree take harmless values (their steps are skipped)
@@ -2283,27 +2315,37 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
       }
 #pragma unroll
       for (int q = 0; q < kEvalUnroll; ++q)
-        if (l + q <= L && l + q >= l0) step(l + q, Iv[q], Au[q], Bu[q], C[q]);
+        if (l + q <= L && l + q >= l0) step(l + q, Iv[q], Pv[q], Au[q], Bu[q], C[q]);
       id = next_id(ids[kEvalUnroll - 1], l + kEvalUnroll - 1);
+      pid = hasp ? next_kidx(pids[kEvalUnroll - 1], l + kEvalUnroll - 1, l0, pm1, L) : 0;
     }
-    const double2 ea = ph[m1], ey = ph[2 * L + 1 + m2];
-    const double cp = ea.x * ey.x - ea.y * ey.y, sp = ea.x * ey.y + ea.y * ey.x;
-    const double w = (m1 == 0) ? 1.0 : 2.0;
-    // Re(c E) and Re(-i c E) = Im(c E)
-    const double r0 = c0x * cp - c0y * sp, i0 = c0x * sp + c0y * cp;
-    const double r1 = c1x * cp - c1y * sp, i1 = c1x * sp + c1y * cp;
-    const double r2 = c2x * cp - c2y * sp;
-    const double M1 = m1, M2 = m2;
-    acc[0] -= w * r0;
-    acc[1] -= w * M1 * i0;          // d/da: Re(-i m1 c E) = m1 Im(cE)
-    acc[2] -= w * r1;
-    acc[3] -= w * M2 * i0;
-    acc[4] -= w * (-M1 * M1 * r0);
-    acc[5] -= w * M1 * i1;
-    acc[6] -= w * (-M1 * M2 * r0);
-    acc[7] -= w * r2;
-    acc[8] -= w * M2 * i1;
-    acc[9] -= w * (-M2 * M2 * r0);
+    // the item's entry (m1, m2) and its partner (pm1, pm2): weight 2 for
+    // m1 > 0 (the rows m1 < 0 by Hermitian symmetry), the partner's sign
+    const double psg = (m2 >= 0 && ((l0 - m2) & 1)) ? -1.0 : 1.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int a1 = k ? pm1 : m1, a2 = k ? pm2 : m2;
+      const double w = k ? (hasp ? (a1 == 0 ? psg : 2.0 * psg) : 0.0) : (a1 == 0 ? 1.0 : 2.0);
+      const double x0 = k ? q0x : c0x, y0 = k ? q0y : c0y, x1 = k ? q1x : c1x;
+      const double y1 = k ? q1y : c1y, x2 = k ? q2x : c2x, y2 = k ? q2y : c2y;
+      const double2 ea = ph[a1], ey = ph[2 * L + 1 + a2];
+      const double cp = ea.x * ey.x - ea.y * ey.y, sp = ea.x * ey.y + ea.y * ey.x;
+      // Re(c E) and Re(-i c E) = Im(c E)
+      const double r0 = x0 * cp - y0 * sp, i0 = x0 * sp + y0 * cp;
+      const double r1 = x1 * cp - y1 * sp, i1 = x1 * sp + y1 * cp;
+      const double r2 = x2 * cp - y2 * sp;
+      const double M1 = a1, M2 = a2;
+      acc[0] -= w * r0;
+      acc[1] -= w * M1 * i0;          // d/da: Re(-i m1 c E) = m1 Im(cE)
+      acc[2] -= w * r1;
+      acc[3] -= w * M2 * i0;
+      acc[4] -= w * (-M1 * M1 * r0);
+      acc[5] -= w * M1 * i1;
+      acc[6] -= w * (-M1 * M2 * r0);
+      acc[7] -= w * r2;
+      acc[8] -= w * M2 * i1;
+      acc[9] -= w * (-M2 * M2 * r0);
+    }
   }
   block_sum_n<10>(acc, red);
   // the sums are uniform over the workgroup: keep them in scalar registers
