@@ -24,6 +24,7 @@ ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "unsuppo
 SPH_NO_INVERT, SPH_KEEP_GRID, SPH_NO_CENTER, SPH_GENERAL, SPH_DFT, SPH_AUTOTUNE = \
     1, 2, 4, 8, 16, 32
 SPH_BETA_GEMM = 64
+SPH_EXACT_PLANES = 128
 STAGE_PREP, STAGE_COEFF, STAGE_GRID, STAGE_ROTOPT, STAGE_FINISH, STAGE_ALL = 1, 2, 4, 8, 16, 31
 # periodic flags / stages
 PER_KEEP_GRID, PER_INPUT_COEFFS, PER_KEEP_COEFFS, PER_KEEP_FFT = 2, 4, 8, 16

@@ -52,7 +52,7 @@ void launch_beta_gemm(hipStream_t, int, int, int, long long, int, const SphPair*
                       const float2*, float4*, int, double*);
 void launch_sph_select(hipStream_t, int, int, long long, int, const SphPair*, const double*,
                        const double2*, const double2*, const double2*, int, const PlaneCand*,
-                       double2*, long long, const double*, SphPeak*, double);
+                       double2*, long long, const double*, SphPeak*, double, int);
 void launch_sph_rotopt(hipStream_t, int, int, int, const SphPair*, const double2*, const double2*,
                        int, const SphPeak*, const double*, int, SphOpt*);
 void launch_rec_table(hipStream_t, int, double*, double*, double2*, double*);
@@ -574,7 +574,8 @@ static int sph_enqueue(fo_sph_batch* b, int stages) {
     // split-BF16x3 products carry up to 3.1 * 2^-16 of each |d I| beyond FP32
     launch_sph_select(s, P, L.n, b->tab->Kt, L.norient, b->pd, b->tab->tab64, b->tab->tw64,
                       b->I64A, b->I64B, L.mode, b->cand, b->scr, b->scr_stride, b->abound,
-                      b->peaks, b->S ? 3.1 / 65536.0 : 0.0);
+                      b->peaks, b->S ? 3.1 / 65536.0 : 0.0,
+                      (b->flags & FO_SPH_EXACT_PLANES) ? 1 : 0);
   } else {
     mark(5);
   }

@@ -1677,7 +1677,7 @@ __global__ void __launch_bounds__(256) k_isoft_plane(
 }
 
 // ---------------------------------------------------------------------------
-// FP64 evaluation of the overlap grid at one point (workgroup cooperative):
+// FP64 evaluation of the overlap grid at grid points (workgroup cooperative):
 //   g(a,b,c) = sum_{m1>=0,m2} w_m1 Re( S64[m1,m2](b) e^{i(m1 alpha + m2 gamma)} )
 // with S64 = sum_l sgn^l dtilde64 I64 (alignment mode) or the B set.
 // ---------------------------------------------------------------------------
@@ -1690,32 +1690,6 @@ struct GridCtx {
   const double2* IB;  // B set (general mode) or nullptr
   double sgn;         // +1 / -1 (alignment mode)
 };
-
-FO_DEV double grid_eval64(const GridCtx& c, int ia, int ib, int ig, double* red) {
-  const int L = c.L, W = 2 * L + 1, n = c.n;
-  const double* tb = c.tab64 + (long long)ib * c.Kt;
-  const double2* I = c.IB ? c.IB : c.I;
-  const double sg = c.IB ? 1.0 : c.sgn;
-  double acc = 0.0;
-  for (int e = threadIdx.x; e < (L + 1) * W; e += blockDim.x) {
-    const int m1 = e / W, m2 = e % W - L;
-    const int l0 = max(m1, abs(m2));
-    double sx = 0.0, sy = 0.0;
-    double sl = (l0 & 1) ? sg : 1.0;
-    for (int l = l0; l <= L; ++l) {
-      const long long id = kidx(l, m1, m2);
-      const double dv = tb[id] * sl;
-      sx += dv * I[id].x;
-      sy += dv * I[id].y;
-      sl *= sg;
-    }
-    int ph = (int)(((long long)m1 * ia + (long long)m2 * ig) % n);
-    if (ph < 0) ph += n;
-    const double2 w = c.tw64[ph];
-    acc += (m1 == 0 ? 1.0 : 2.0) * (sx * w.x - sy * w.y);
-  }
-  return block_sum(acc, red);
-}
 
 // FP64 grid values at up to kEvalBatch points whose beta indices take at most
 // kEvalBeta distinct values, in one sweep over the coefficients (each
@@ -1812,7 +1786,7 @@ FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int
             x = sx[bb];
             y = sy[bb];
           }
-        int ph = (int)(((long long)m1 * pa[q] + (long long)m2 * pg[q]) % n);
+        int ph = (m1 * pa[q] + m2 * pg[q]) % n;  // |m1 pa + m2 pg| < 2 L n
         if (ph < 0) ph += n;
         const double2 w = c.tw64[ph];
         acc[q] += wm * (x * w.x - y * w.y);
@@ -1826,52 +1800,113 @@ FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int
 }
 
 // Full FP64 evaluation of one beta plane; returns its argmax (ties -> lower
-// flat index).  Uses global scratch Sg[(L+1)W], Tg[(L+1)n].
-FO_DEV void plane_eval64(const GridCtx& c, int ib, double2* Sg, double2* Tg, double* red,
-                         int* redi, double& bestv, int& besti) {
-  const int L = c.L, W = 2 * L + 1, n = c.n;
+// flat index).  Three stages, each thread-parallel:
+//   S[m1][m2] = sum_l s_l d^l_{m1 m2}(beta) I^l_{m1 m2}   (entries in shell
+//     order, two degrees of loads in flight, as in grid_eval64_batch),
+//   T[m1][g]  = sum_m2 S[m1][m2] e^{i m2 g 2pi/n}          (thread per (m1, g)),
+//   v(a, g)   = Re T[0][g] + 2 sum_{m1>=1} Re(T[m1][g] e^{i m1 a 2pi/n})
+//     (thread per (g, kPlaneA values of a): each T load serves them all).
+// S lives in the LDS buffer sS when (L+1)(2L+1) <= kSelS (L <= 31), else
+// in the global scratch Sg; T in the global scratch Tg[(L+1)n]; the n
+// twiddles are staged in the LDS buffer stw (n <= 256).  Phases advance by
+// integer steps mod n, so every twiddle is a table entry.
+constexpr int kSelS = 2048;
+constexpr int kPlaneA = 8;
+FO_DEV void plane_eval64(const GridCtx& c, int ib, double2* Sg, double2* Tg, double2* sS,
+                         double2* stw, double* red, int* redi, double& bestv, int& besti) {
+  const int L = c.L, W = 2 * L + 1, n = c.n, t = threadIdx.x, nt = blockDim.x;
   const double* tb = c.tab64 + (long long)ib * c.Kt;
   const double2* I = c.IB ? c.IB : c.I;
   const double sg = c.IB ? 1.0 : c.sgn;
-  for (int e = threadIdx.x; e < (L + 1) * W; e += blockDim.x) {
-    const int m1 = e / W, m2 = e % W - L;
-    const int l0 = max(m1, abs(m2));
+  const bool sl_lds = (L + 1) * W <= kSelS;
+  for (int k = t; k < n; k += nt) stw[k] = c.tw64[k];
+  // (1) beta sums
+  const int ne = (L + 1) * W;
+  for (int e0 = 0; e0 < ne; e0 += nt) {
+    const int e = e0 + t;
+    const bool act = e < ne;
+    int l0 = L + 1, m1 = 0, m2 = 0;
+    if (act) shell_entry(e, l0, m1, m2);
+    const int lw = __builtin_amdgcn_readfirstlane(l0);  // smallest l0 of the wavefront
     double sx = 0.0, sy = 0.0, sl = (l0 & 1) ? sg : 1.0;
-    for (int l = l0; l <= L; ++l) {
-      const long long id = kidx(l, m1, m2);
-      const double dv = tb[id] * sl;
-      sx += dv * I[id].x;
-      sy += dv * I[id].y;
-      sl *= sg;
+    int id = act ? (int)kidx(l0, m1, m2) : 0;
+    for (int l = lw; l <= L; l += 2) {  // wavefront-uniform
+      int ids[2];
+      ids[0] = id;
+      ids[1] = next_kidx(id, l, l0, m1, L);
+      double2 Iv[2];
+      double dv[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        Iv[q] = I[ids[q]];
+        dv[q] = tb[ids[q]];
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (l + q <= L && l + q >= l0) {
+          const double d = dv[q] * sl;
+          sx += d * Iv[q].x;
+          sy += d * Iv[q].y;
+          sl *= sg;
+        }
+      }
+      id = next_kidx(ids[1], l + 1, l0, m1, L);
     }
-    Sg[e] = make_double2(sx, sy);
+    if (act) (sl_lds ? sS : Sg)[m1 * W + m2 + L] = make_double2(sx, sy);
   }
   __threadfence_block();
   __syncthreads();
-  for (int e = threadIdx.x; e < (L + 1) * n; e += blockDim.x) {
-    const int m1 = e / n, g = e % n;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int m2i = 0; m2i < W; ++m2i) {
-      int ph = (int)(((long long)(m2i - L) * g) % n);
-      if (ph < 0) ph += n;
-      acc = cadd(acc, cmul(Sg[m1 * W + m2i], c.tw64[ph]));
+  // (2) gamma sums
+  auto gamma = [&](const double2* S) {
+    for (int e = t; e < (L + 1) * n; e += nt) {
+      const int m1 = e / n, g = e % n;
+      int ph = (L * g) % n;
+      ph = ph ? n - ph : 0;  // (-L g) mod n
+      const double2* row = S + m1 * W;
+      double2 acc = make_double2(0.0, 0.0);
+#pragma unroll 4
+      for (int m2i = 0; m2i < W; ++m2i) {
+        acc = cadd(acc, cmul(row[m2i], stw[ph]));
+        ph += g;
+        if (ph >= n) ph -= n;
+      }
+      Tg[e] = acc;
     }
-    Tg[e] = acc;
-  }
+  };
+  if (sl_lds)
+    gamma(sS);
+  else
+    gamma(Sg);
   __threadfence_block();
   __syncthreads();
+  // (3) alpha sums and the plane's argmax
   double bv = -INFINITY;
   int bi = 0x7fffffff;
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    const int a = e / n, g = e % n;
-    double acc = Tg[g].x;
-    for (int m1 = 1; m1 <= L; ++m1) {
-      const int ph = (int)(((long long)m1 * a) % n);
-      const double2 v = Tg[m1 * n + g];
-      const double2 w = c.tw64[ph];
-      acc += 2.0 * (v.x * w.x - v.y * w.y);
+  const int nch = (n + kPlaneA - 1) / kPlaneA;
+  for (int it = t; it < n * nch; it += nt) {
+    const int g = it % n, a0 = (it / n) * kPlaneA;
+    double acc[kPlaneA];
+    int ph[kPlaneA];
+    const double t0 = Tg[g].x;
+#pragma unroll
+    for (int j = 0; j < kPlaneA; ++j) {
+      acc[j] = t0;
+      ph[j] = 0;
     }
-    argmax_merge(bv, bi, acc, (a * n + ib) * n + g);
+#pragma unroll 2
+    for (int m1 = 1; m1 <= L; ++m1) {
+      const double2 v = Tg[m1 * n + g];
+#pragma unroll
+      for (int j = 0; j < kPlaneA; ++j) {
+        ph[j] += a0 + j < n ? a0 + j : 0;  // (values of a >= n are not used)
+        if (ph[j] >= n) ph[j] -= n;
+        const double2 w = stw[ph[j]];
+        acc[j] += 2.0 * (v.x * w.x - v.y * w.y);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kPlaneA; ++j)
+      if (a0 + j < n) argmax_merge(bv, bi, acc[j], ((a0 + j) * n + ib) * n + g);
   }
   block_argmax(bv, bi, red, redi);
   bestv = bv;
@@ -1886,14 +1921,14 @@ FO_DEV void plane_eval64(const GridCtx& c, int ib, double2* Sg, double2* Tg, dou
 // at most a few units of 2^-24 A each; delta = (8(L+1) + 32) 2^-24 A covers
 // their sum.  Every grid point within delta of the FP32 maximum is
 // re-evaluated in FP64; planes whose unlisted values could be within delta
-// are re-evaluated in full.
+// are re-evaluated in full (all_planes: every plane, the exhaustive check).
 
 __global__ void __launch_bounds__(256) k_sph_select(
     const SphPair* __restrict__ pd, int n, long long Kt, const double* __restrict__ tab64,
     const double2* __restrict__ tw64, const double2* __restrict__ IA,
     const double2* __restrict__ IB, int mode, int norient, const PlaneCand* __restrict__ cand,
     double2* __restrict__ scratch, long long scratch_stride, const double* __restrict__ abound,
-    SphPeak* __restrict__ peaks, int P, double extra_rel) {
+    SphPeak* __restrict__ peaks, int P, double extra_rel, int all_planes) {
   __shared__ double red[16 * kEvalBatch];
   __shared__ int redi[16];
   __shared__ int s_cnt, s_ovf;
@@ -1902,6 +1937,7 @@ __global__ void __launch_bounds__(256) k_sph_select(
   __shared__ int s_pa[kEvalBatch], s_pg[kEvalBatch], s_pb[kEvalBatch], s_beta[kEvalBeta];
   __shared__ int s_np, s_nb;
   __shared__ double s_val[kEvalBatch];
+  __shared__ double2 s_planeS[kSelS], s_planeTw[256];  // full-plane evaluation only
   int p, o;
   if (!xcd_decode(P, norient, p, o)) return;
   const SphPair d = pd[p];
@@ -1943,7 +1979,7 @@ __global__ void __launch_bounds__(256) k_sph_select(
   for (int b = t; b < n; b += blockDim.x) {
     float rest = pc[b].rest[0];
     for (int r = 1; r < kCandPerPlane; ++r) rest = fmaxf(rest, pc[b].rest[r]);
-    if ((double)rest >= thr) {
+    if (all_planes || (double)rest >= thr) {
       int q = atomicAdd(&s_ovf, 1);
       s_ovfl[q] = b;
     } else {
@@ -1991,7 +2027,7 @@ __global__ void __launch_bounds__(256) k_sph_select(
     for (int b = 0; b < n; ++b) {
       double v;
       int i;
-      plane_eval64(c, b, Sg, Tg, red, redi, v, i);
+      plane_eval64(c, b, Sg, Tg, s_planeS, s_planeTw, red, redi, v, i);
       if (v > best || (v == best && i < besti)) {
         second = best;
         best = v;
@@ -2046,7 +2082,7 @@ __global__ void __launch_bounds__(256) k_sph_select(
     for (int q = 0; q < novf; ++q) {
       double v;
       int i;
-      plane_eval64(c, s_ovfl[q], Sg, Tg, red, redi, v, i);
+      plane_eval64(c, s_ovfl[q], Sg, Tg, s_planeS, s_planeTw, red, redi, v, i);
       if (v > best || (v == best && i < besti)) {
         second = best;
         best = v;
@@ -2924,10 +2960,10 @@ void launch_sph_select(hipStream_t s, int P, int n, long long Kt, int norient, c
                        const double* tab64, const double2* tw64, const double2* IA,
                        const double2* IB, int mode, const PlaneCand* cand, double2* scratch,
                        long long scratch_stride, const double* abound, SphPeak* peaks,
-                       double extra_rel) {
+                       double extra_rel, int all_planes) {
   hipLaunchKernelGGL(k_sph_select, dim3(xcd_grid(P, norient)), dim3(256), 0, s, pd, n, Kt, tab64,
                      tw64, IA, IB, mode, norient, cand, scratch, scratch_stride, abound, peaks, P,
-                     extra_rel);
+                     extra_rel, all_planes);
 }
 
 void launch_sph_rotopt(hipStream_t s, int P, int norient, int Lmax, const SphPair* pd,

@@ -74,6 +74,9 @@ int fo_context_memory(fo_context* ctx, unsigned long long* used, unsigned long l
 #define FO_SPH_BETA_GEMM 64    /* beta contraction as a split-BF16x3 GEMM on the matrix
                                   cores (FP32 accumulation) instead of FP32 in the plane
                                   kernel; alignment mode, power-of-two grid sides */
+#define FO_SPH_EXACT_PLANES 128 /* every beta plane of the grid evaluated in FP64 for the
+                                  argmax (exhaustive check of the bounded FP32 selection;
+                                  about grid_n times the selection's work) */
 
 /* stages for fo_sph_batch_run (bit mask) */
 #define FO_STAGE_PREP 1        /* centroids, kernel width */

@@ -466,6 +466,51 @@ def test_c3_large_jmax_against_fixture(gpu, arm):
     say(f"C3 {arm}: grid error on the top values <= {worst:.2e} of the grid maximum")
 
 
+def test_exhaustive_fp64_planes_against_oracle(gpu):
+    """FO_SPH_EXACT_PLANES evaluates every beta plane of both orientations in
+    FP64 with the full-plane routine that the bounded selection otherwise
+    runs only for planes whose unlisted FP32 values come within the error
+    bound of the maximum.  C2 pairs of both grid classes (beta sums in LDS)
+    against the oracle's FP64 grids: the oracle's argmax (or an FP64 tie,
+    judged by value), the FP64 maximum within 1e-12 of the oracle's, and the
+    default selection's answers.  One C3 pair (Jmax 40, n = 128: beta sums in
+    global scratch) against its fixture's argmax and maximum."""
+    for pidx in (1, 5, 23):  # n = 32 / Jmax 15; n = 64 / Jmax 20; n = 64 / Jmax 25
+        c = W.config2(pidx)
+        n = c["grid_n"]
+        say(f"case: C2 pair {pidx} n={n} L={c['L']} N={len(c['pos1'])}, every plane in FP64")
+        o = _oracle_pair(c, keep_grids=True)
+        res = {}
+        for flag in (0, _lib.SPH_EXACT_PLANES):
+            with SphericalBatch([len(c["pos1"])], [c["L"]], 0.3, n, [c["groups"]], flag) as b:
+                b.set_positions(c["pos1"], c["pos2"])
+                b.run()
+                res[flag] = (b.results(), b.peaks())
+        for flag, (r, pk) in res.items():
+            _check_pair(f"pair {pidx} flags {flag}", o, r["dist"][0], r["perm"][0],
+                        r["inverted"][0], pk["ind"][0])
+        r, pk = res[_lib.SPH_EXACT_PLANES]
+        assert (pk["noverflow"][0] == n).all(), pk["noverflow"]
+        for k in range(2):
+            top = o.grids[k].real[o.ind[k]]
+            assert abs(pk["value"][0][k] - top) <= 1e-12 * abs(top), (pidx, k)
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "C3_pairs.npz"))
+    L, n, s = int(z["L"]), int(z["grid_n"]), float(z["scale"])
+    sz = z["group_sizes"][0]
+    say(f"case: C3 pair 0 N=1000 L={L} n={n}, every plane in FP64")
+    with SphericalBatch([1000], [L], [s], n, [[np.arange(sz[0]), np.arange(sz[0], sz[0] + sz[1])]],
+                        _lib.SPH_EXACT_PLANES) as b:
+        b.set_positions([z["pos1"][0]], [z["pos2"][0]])
+        b.run()
+        r, pk = b.results(), b.peaks()
+    for k in range(2):
+        assert tuple(pk["ind"][0][k]) == tuple(z["ind"][0][k]), (k, pk["ind"][0][k])
+        assert abs(pk["value"][0][k] - z["top_val"][0][k][0]) <= 1e-12 * z["gmax"][0][k], k
+    np.testing.assert_array_equal(r["perm"][0], z["perm"][0])
+    assert abs(r["dist"][0] - float(z["dist"][0])) <= DIST_TOL
+
+
 def test_graph_replay_matches_eager(gpu):
     pairs = [W.config2(p) for p in range(100, 108)]
     sel = [c for c in pairs if c["grid_n"] == 64]
