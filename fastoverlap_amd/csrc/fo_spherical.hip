@@ -1722,23 +1722,30 @@ FO_DEV int next_kidx(int id, int l, int l0, int m1, int L) {
   return (l >= l0 && l < L) ? id + (2 * l + 1) * (2 * l + 1) + 2 * m1 + 4 * l + 4 : id;
 }
 
+// The entries are swept in blocks of 64 (one wavefront) in shell order, so
+// the first blocks carry the longest degree loops; a wavefront that finishes
+// its block takes the next untaken one (LDS counter ctr), which evens out the
+// wavefronts' loop counts.  Each block's sums are reduced over its wavefront
+// into part[blk][q] (LDS, (L+1)(2L+1)/64 + 1 rows) and the blocks are added
+// in block order, so the result does not depend on which wavefront took
+// which block.
 FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int* pg,
                               const int* pb, int nbeta, const int* betas, double* out,
-                              double* red) {
+                              double* part, int* ctr) {
   const int L = c.L, W = 2 * L + 1, n = c.n;
   const double2* I = c.IB ? c.IB : c.I;
   const double sg = c.IB ? 1.0 : c.sgn;
   const double* tb[kEvalBeta];
 #pragma unroll
   for (int b = 0; b < kEvalBeta; ++b) tb[b] = c.tab64 + (long long)betas[b < nbeta ? b : 0] * c.Kt;
-  double acc[kEvalBatch];
-#pragma unroll
-  for (int q = 0; q < kEvalBatch; ++q) acc[q] = 0.0;
   // entries (m1 >= 0, m2) in shell order (shell_entry), so the degree loop of
   // a wavefront is uniform and the operands of two degrees are loaded at once
-  const int ne = (L + 1) * W;
-  for (int e0 = 0; e0 < ne; e0 += blockDim.x) {
-    const int e = e0 + threadIdx.x;
+  const int ne = (L + 1) * W, nblk = (ne + 63) >> 6;
+  const int lane = threadIdx.x & 63, nwv = blockDim.x >> 6;
+  if (threadIdx.x == 0) *ctr = nwv;
+  __syncthreads();
+  for (int blk = threadIdx.x >> 6; blk < nblk;) {  // wavefront-uniform
+    const int e = (blk << 6) + lane;
     const bool act = e < ne;
     int l0 = L + 1, m1 = 0, m2 = 0;
     if (act) shell_entry(e, l0, m1, m2);
@@ -1775,8 +1782,10 @@ FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int
       id = next_kidx(ids[1], l + 1, l0, m1, L);
     }
     const double wm = m1 == 0 ? 1.0 : 2.0;
+    double acc[kEvalBatch];
 #pragma unroll
     for (int q = 0; q < kEvalBatch; ++q) {
+      acc[q] = 0.0;
       if (q < np) {
         const int b = pb[q];
         double x = sx[0], y = sy[0];
@@ -1789,14 +1798,24 @@ FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int
         int ph = (m1 * pa[q] + m2 * pg[q]) % n;  // |m1 pa + m2 pg| < 2 L n
         if (ph < 0) ph += n;
         const double2 w = c.tw64[ph];
-        acc[q] += wm * (x * w.x - y * w.y);
+        acc[q] = wm * (x * w.x - y * w.y);
       }
     }
-  }
-  block_sum_n<kEvalBatch>(acc, red);
 #pragma unroll
-  for (int q = 0; q < kEvalBatch; ++q)
-    if (q < np) out[q] = acc[q];
+    for (int q = 0; q < kEvalBatch; ++q) {
+      const double v = wave_sum(acc[q]);
+      if (lane == 0) part[blk * kEvalBatch + q] = v;
+    }
+    int nb = 0;
+    if (lane == 0) nb = atomicAdd(ctr, 1);
+    blk = __builtin_amdgcn_readfirstlane(nb);
+  }
+  __syncthreads();
+  if (threadIdx.x < np) {
+    double v = 0.0;
+    for (int b = 0; b < nblk; ++b) v += part[b * kEvalBatch + threadIdx.x];
+    out[threadIdx.x] = v;
+  }
 }
 
 // Full FP64 evaluation of one beta plane; returns its argmax (ties -> lower
@@ -1935,7 +1954,9 @@ __global__ void __launch_bounds__(256) k_sph_select(
   __shared__ int s_cand[kMaxCand];
   __shared__ int s_ovfl[256];
   __shared__ int s_pa[kEvalBatch], s_pg[kEvalBatch], s_pb[kEvalBatch], s_beta[kEvalBeta];
-  __shared__ int s_np, s_nb;
+  __shared__ int s_np, s_nb, s_blk;
+  // s_planeS also holds the block sums of grid_eval64_batch (L <= 63)
+  static_assert(2 * kSelS >= kEvalBatch * ((64 * 127 + 63) / 64), "block sums");
   __shared__ double s_val[kEvalBatch];
   __shared__ double2 s_planeS[kSelS], s_planeTw[256];  // full-plane evaluation only
   int p, o;
@@ -2063,7 +2084,7 @@ __global__ void __launch_bounds__(256) k_sph_select(
       }
       __syncthreads();
       const int np = s_np;
-      grid_eval64_batch(c, np, s_pa, s_pg, s_pb, s_nb, s_beta, s_val, red);
+      grid_eval64_batch(c, np, s_pa, s_pg, s_pb, s_nb, s_beta, s_val, (double*)s_planeS, &s_blk);
       __syncthreads();
       for (int q = 0; q < np; ++q) {
         const int f = s_cand[q0 + q];
@@ -2117,7 +2138,8 @@ __global__ void __launch_bounds__(256) k_sph_select(
       }
   }
   __syncthreads();
-  grid_eval64_batch(c, q0 + 6, s_pa, s_pg, s_pb, s_nb, s_beta, s_val, red);
+  grid_eval64_batch(c, q0 + 6, s_pa, s_pg, s_pb, s_nb, s_beta, s_val, (double*)s_planeS,
+                    &s_blk);
   __syncthreads();
   if (merged) best = s_val[0];
   double off[3];
@@ -2213,12 +2235,15 @@ FO_DEV RecLds build_rec_lds(double* t, int Lp1) {
   return RecLds{st, ist, al, cl, Lp1};
 }
 
-// `ph`: LDS for 3L + 2 phase factors.
+// `ph`: LDS for 3L + 2 phase factors; `part`: LDS for (kFghBlocks + 1) * 10
+// doubles; `ctr`: an LDS int.
 constexpr int kEvalUnroll = 2;  // degrees per batch of operand loads
+constexpr int kFghBlocks = (64 * 64 + 63 + 63) / 64;  // item blocks at L = 63
 template <bool kLdsRec = false>
 FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a, double b,
                      double y, const double* __restrict__ rec, const double* __restrict__ seedu,
-                     int Lt, FGH& out, double* red, double2* ph, RecLds rl = RecLds{}) {
+                     int Lt, FGH& out, double* part, double2* ph, int* ctr,
+                     RecLds rl = RecLds{}) {
   const int W = 2 * L + 1, Wt = 2 * Lt + 1;
   const double cb = cos(b), sb = sin(b), c2 = cos(0.5 * b), s2 = sin(0.5 * b);
   // phase factors e^{-i m1 a} (m1 = 0..L) and e^{-i m2 y} (m2 = -L..L)
@@ -2230,32 +2255,43 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
       sincos(-((q - L - 1 - L) * y), &sp, &cp);
     ph[q] = make_double2(cp, sp);
   }
+  if (threadIdx.x == 0) *ctr = blockDim.x >> 6;
   __syncthreads();
-  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   // The sum runs over the entries (m1 >= 0, m2).  d and its b-derivatives
   // obey d_{m1,m2} = (-1)^{m1-m2} d_{m2,m1} = d_{-m2,-m1}, so one recurrence
   // serves two entries: item (l0, m2), |m2| <= l0 (the row m1 = l0 of shell
   // l0), also carries (m2, l0) for 0 <= m2 < l0 (sign (-1)^{l0-m2}) or
-  // (-m2, -l0) for -l0 < m2 < 0; the entries (0, -l0), l0 >= 1, are items of
-  // their own after the (L+1)^2 shell items.  Items in shell order, so the
-  // lanes of a wavefront start their recurrences at nearly the same l and
-  // the loop over l is wavefront-uniform.
-  const int nw = (L + 1) * (L + 1), ne = nw + L;
-  for (int e0 = 0; e0 < ne; e0 += blockDim.x) {
-    const int e = e0 + threadIdx.x;
+  // (-m2, -l0) for -l0 < m2 < 0; the entry (0, -l0), l0 >= 1, is an item of
+  // its own.  Items in shell order: shell l0 >= 1 starts at item
+  // l0^2 + l0 - 1 with its 2 l0 + 1 row items, then (0, -l0); so the lanes
+  // of a wavefront start their recurrences at nearly the same l and the loop
+  // over l is wavefront-uniform.  Blocks of 64 items (one wavefront): the
+  // first blocks carry the longest loops, and a wavefront that finishes its
+  // block takes the next untaken one (LDS counter ctr).  Each block's sums
+  // are reduced over its wavefront into part[blk][10] (LDS, kFghBlocks
+  // rows) and the blocks are added in block order, so the result does not
+  // depend on which wavefront took which block.
+  const int ne = (L + 1) * (L + 1) + L, nblk = (ne + 63) >> 6;
+  const int lane = threadIdx.x & 63;
+  for (int blk = threadIdx.x >> 6; blk < nblk;) {  // wavefront-uniform
+    const int e = (blk << 6) + lane;
     const bool act = e < ne;
     int l0 = L + 1, m1 = 0, m2 = 0;
-    if (act && e < nw) {
-      l0 = (int)sqrt((double)e);
-      while (l0 * l0 > e) --l0;
-      while ((l0 + 1) * (l0 + 1) <= e) ++l0;
-      m1 = l0;
-      m2 = e - l0 * l0 - l0;
+    bool row = false;  // an item of the row m1 = l0
+    if (act && e == 0) {
+      l0 = 0;
+      row = true;
     } else if (act) {
-      l0 = e - nw + 1;
-      m2 = -l0;
+      l0 = (int)((sqrt(4.0 * e + 5.0) - 1.0) * 0.5);
+      while (l0 * l0 + l0 - 1 > e) --l0;
+      while ((l0 + 1) * (l0 + 1) + l0 <= e) ++l0;
+      const int r = e - (l0 * l0 + l0 - 1);
+      row = r <= 2 * l0;
+      m1 = row ? l0 : 0;
+      m2 = row ? r - l0 : -l0;
     }
-    const bool hasp = act && e < nw && m2 > -l0 && m2 < l0;
+    const bool hasp = row && m2 > -l0 && m2 < l0;
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int pm1 = m2 >= 0 ? m2 : -m2, pm2 = m2 >= 0 ? l0 : -l0;
     const int mu = abs(m1 - m2), nu = abs(m1 + m2);
     // edge element g = s2^mu c2^nu and its b-derivatives
@@ -2382,12 +2418,26 @@ FO_DEV void eval_fgh(const double2* __restrict__ I, double sgn, int L, double a,
       acc[8] -= w * M2 * i1;
       acc[9] -= w * (-M2 * M2 * r0);
     }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+      const double v = wave_sum(acc[q]);
+      if (lane == 0) part[blk * 10 + q] = v;
+    }
+    int nb = 0;
+    if (lane == 0) nb = atomicAdd(ctr, 1);
+    blk = __builtin_amdgcn_readfirstlane(nb);
   }
-  block_sum_n<10>(acc, red);
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    double v = 0.0;
+    for (int b = 0; b < nblk; ++b) v += part[b * 10 + threadIdx.x];
+    part[kFghBlocks * 10 + threadIdx.x] = v;
+  }
+  __syncthreads();
   // the sums are uniform over the workgroup: keep them in scalar registers
 #pragma unroll
   for (int q = 0; q < 10; ++q) {
-    const long long b = __double_as_longlong(acc[q]);
+    const long long b = __double_as_longlong(part[kFghBlocks * 10 + q]);
     const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
     const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
     out.v[q] = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
@@ -2503,8 +2553,9 @@ __global__ void __launch_bounds__(256, 3) k_sph_rotopt(const SphPair* __restrict
                                                     const SphPeak* __restrict__ peaks, int norient,
                                                     const double* __restrict__ seedu, int Lt,
                                                     int Lmax, SphOpt* __restrict__ opt, int P) {
-  __shared__ double red[16 * 10];
+  __shared__ double part[(kFghBlocks + 1) * 10];
   __shared__ double2 ph[3 * 64 + 2];
+  __shared__ int s_blk;
   extern __shared__ __align__(16) double rtab[];  // rec_lds_doubles(Lmax + 2)
   int p, o;
   if (!xcd_decode(P, norient, p, o)) return;
@@ -2524,7 +2575,8 @@ __global__ void __launch_bounds__(256, 3) k_sph_rotopt(const SphPair* __restrict
   __syncthreads();
   for (;;) {
     FGH r;
-    eval_fgh<true>(I, sgn, L, ns.xt[0], ns.xt[1], ns.xt[2], nullptr, seedu, Lt, r, red, ph, rl);
+    eval_fgh<true>(I, sgn, L, ns.xt[0], ns.xt[1], ns.xt[2], nullptr, seedu, Lt, r, part, ph,
+                   &s_blk, rl);
     if (threadIdx.x == 0) newton_update(ns, r);
     __syncthreads();
     if (ns.stop) break;
@@ -3045,10 +3097,11 @@ __global__ void __launch_bounds__(256) k_sph_energy(int L, const double2* __rest
                                                     const double* __restrict__ rec,
                                                     const double* __restrict__ seedu, int Lt,
                                                     double* __restrict__ out) {
-  __shared__ double red[16 * 10];
+  __shared__ double part[(kFghBlocks + 1) * 10];
   __shared__ double2 ph[3 * 64 + 2];
+  __shared__ int s_blk;
   FGH r;
-  eval_fgh(Iconj, 1.0, L, a, b, y, rec, seedu, Lt, r, red, ph);
+  eval_fgh(Iconj, 1.0, L, a, b, y, rec, seedu, Lt, r, part, ph, &s_blk);
   if (threadIdx.x == 0)
     for (int q = 0; q < 10; ++q) out[q] = r.v[q];
 }
