@@ -1752,20 +1752,29 @@ FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int
     double sx[kEvalBeta], sy[kEvalBeta];
 #pragma unroll
     for (int b = 0; b < kEvalBeta; ++b) sx[b] = sy[b] = 0.0;
-    double sl = (l0 & 1) ? sg : 1.0;
+    // The column entries of a shell (m1 < l0, m2 = +-l0) read the table at
+    // the transposed entry, d_{m1 m2} = (-1)^{m1-m2} d_{m2 m1}: row m2, so
+    // the lanes of a wavefront read consecutive table words (the rows of
+    // the compact layout are contiguous, its columns 2l+1 words apart).
+    const bool col = m1 < l0;
+    const int tm1 = col ? m2 : m1, tm2 = col ? m1 : m2;
+    double sl = ((l0 & 1) ? sg : 1.0) * ((col && ((m1 - m2) & 1)) ? -1.0 : 1.0);
     const int lw = __builtin_amdgcn_readfirstlane(l0);  // smallest l0 of the wavefront
     int id = act ? (int)kidx(l0, m1, m2) : 0;
+    int td = act ? (int)kidx(l0, tm1, tm2) : 0;
     for (int l = lw; l <= L; l += 2) {  // wavefront-uniform
-      int ids[2];
+      int ids[2], tds[2];
       ids[0] = id;
       ids[1] = next_kidx(id, l, l0, m1, L);
+      tds[0] = td;
+      tds[1] = next_kidx(td, l, l0, tm1, L);
       double2 Iv[2];
       double dv[2][kEvalBeta];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         Iv[q] = I[ids[q]];
 #pragma unroll
-        for (int b = 0; b < kEvalBeta; ++b) dv[q][b] = b < nbeta ? tb[b][ids[q]] : 0.0;
+        for (int b = 0; b < kEvalBeta; ++b) dv[q][b] = b < nbeta ? tb[b][tds[q]] : 0.0;
       }
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
@@ -1780,6 +1789,7 @@ FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int
         }
       }
       id = next_kidx(ids[1], l + 1, l0, m1, L);
+      td = next_kidx(tds[1], l + 1, l0, tm1, L);
     }
     const double wm = m1 == 0 ? 1.0 : 2.0;
     double acc[kEvalBatch];
