@@ -1698,6 +1698,7 @@ struct GridCtx {
 // the tree of block_sum.
 constexpr int kEvalBatch = 8;
 constexpr int kEvalBeta = 4;
+constexpr int kSelUnroll = 3;  // degrees per batch of operand loads
 
 // Entry e of the (m1 >= 0, m2) entries of degree <= L in shell order: shell
 // l0 = max(m1, |m2|) holds 4 l0 + 1 entries from e = l0 (2 l0 - 1) on
@@ -1762,22 +1763,25 @@ FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int
     const int lw = __builtin_amdgcn_readfirstlane(l0);  // smallest l0 of the wavefront
     int id = act ? (int)kidx(l0, m1, m2) : 0;
     int td = act ? (int)kidx(l0, tm1, tm2) : 0;
-    for (int l = lw; l <= L; l += 2) {  // wavefront-uniform
-      int ids[2], tds[2];
+    for (int l = lw; l <= L; l += kSelUnroll) {  // wavefront-uniform
+      int ids[kSelUnroll], tds[kSelUnroll];
       ids[0] = id;
-      ids[1] = next_kidx(id, l, l0, m1, L);
       tds[0] = td;
-      tds[1] = next_kidx(td, l, l0, tm1, L);
-      double2 Iv[2];
-      double dv[2][kEvalBeta];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
+      for (int q = 1; q < kSelUnroll; ++q) {
+        ids[q] = next_kidx(ids[q - 1], l + q - 1, l0, m1, L);
+        tds[q] = next_kidx(tds[q - 1], l + q - 1, l0, tm1, L);
+      }
+      double2 Iv[kSelUnroll];
+      double dv[kSelUnroll][kEvalBeta];
+#pragma unroll
+      for (int q = 0; q < kSelUnroll; ++q) {
         Iv[q] = I[ids[q]];
 #pragma unroll
         for (int b = 0; b < kEvalBeta; ++b) dv[q][b] = b < nbeta ? tb[b][tds[q]] : 0.0;
       }
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
+      for (int q = 0; q < kSelUnroll; ++q) {
         if (l + q <= L && l + q >= l0) {
 #pragma unroll
           for (int b = 0; b < kEvalBeta; ++b) {
@@ -1788,8 +1792,8 @@ FO_DEV void grid_eval64_batch(const GridCtx& c, int np, const int* pa, const int
           sl *= sg;
         }
       }
-      id = next_kidx(ids[1], l + 1, l0, m1, L);
-      td = next_kidx(tds[1], l + 1, l0, tm1, L);
+      id = next_kidx(ids[kSelUnroll - 1], l + kSelUnroll - 1, l0, m1, L);
+      td = next_kidx(tds[kSelUnroll - 1], l + kSelUnroll - 1, l0, tm1, L);
     }
     const double wm = m1 == 0 ? 1.0 : 2.0;
     double acc[kEvalBatch];
