@@ -1111,8 +1111,13 @@ static int sph_autotune(fo_sph_batch* b) {
     return FO_OK;
   };
   // knob 1 is reserved (it chose the contraction's tile class below degree
-  // 16, where one fused kernel now runs)
+  // 16, where one fused kernel now runs).  The first candidate of each knob
+  // is the default shape; another replaces the best so far only when its
+  // stage runs at least kTuneMargin faster, so timing noise (a few percent
+  // between launches) cannot trade the default for an equal or slower
+  // shape.
   const std::vector<int> cand[3] = {{16, 1, 2, 4, 8, 32}, {}, {1024, 64, 256, 4096}};
+  constexpr float kTuneMargin = 0.03f;
   for (int knob = 0; knob < 3; ++knob) {
     if (cand[knob].empty() || (knob == 0 && !(L.P > 8))) continue;  // one chunk either way
     int best = b->tune[knob];
@@ -1123,7 +1128,7 @@ static int sph_autotune(fo_sph_batch* b) {
       float ms = 0.f;
       rc = time_stage(knob, &ms);
       if (rc) break;
-      if (first || ms < best_ms) {
+      if (first || ms < best_ms * (1.f - kTuneMargin)) {
         best = v;
         best_ms = ms;
         first = false;
