@@ -257,9 +257,9 @@ __global__ void k_leg_table(int Lt, double2* __restrict__ leg, double* __restric
 // for x < 1; otherwise Miller's backward recurrence started at
 // l_s = ceil(sqrt(T^2 + 40x)) + 10 for the highest stored degree T (where
 // the minimal-solution contamination is below 1e-17) and normalised by
-// i_0(x) e^{-x} = -expm1(-2x)/(2x).  Two passes of the same recurrence: the
-// first finds the normalisation at l = 0, the second stores the slot
-// degrees as it passes them (rad_slot_deg).
+// i_0(x) e^{-x} = -expm1(-2x)/(2x).  The first pass finds the normalisation
+// at l = 0; the second resumes at T and stores the slot degrees as it
+// passes them (rad_slot_deg).
 // Layout: R[rad_off + grp.rad_off + (slot*Ns + j)*Ns + k].
 // ---------------------------------------------------------------------------
 FO_DEV double radial_series(int l, double x, double exg) {
@@ -288,29 +288,31 @@ FO_DEV void radial_store(int L, double x, double gauss, double* out, long long s
   }
   // i_0(x) e^{-x} times the Gaussian fixes the normalisation of Miller's
   // downward recurrence.  Pass 1 runs the recurrence to l = 0 without
-  // storing; pass 2 repeats the identical arithmetic and stores f_l * scal
-  // at the slot degrees (no array of values held in registers, so more
-  // wavefronts per SIMD overlap their stores).
+  // storing, keeping its state at the top slot degree; pass 2 resumes from
+  // that state (the identical values) and stores f_l * scal at the slot
+  // degrees (no array of values held in registers, so more wavefronts per
+  // SIMD overlap their stores).
   const int top = rad_slot_deg(ns - 1, L), bot = rad_slot_deg(0, L);
   const double i0 = -expm1(-2.0 * x) / (2.0 * x);
   const int ls = (int)ceil(sqrt((double)top * top + 40.0 * x)) + 10;
   const double ix = 1.0 / x;
   double fp1 = 0.0, f = 1.0;
 #pragma unroll 2
-  for (int l = ls; l > 0; --l) {
-    const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
-    fp1 = f;
-    f = fm;
-  }
-  const double scal = i0 / f * gauss;
-  fp1 = 0.0;
-  f = 1.0;
-#pragma unroll 2
   for (int l = ls; l > top; --l) {
     const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
     fp1 = f;
     f = fm;
   }
+  const double ft = f, ft1 = fp1;  // f_top, f_{top+1}
+#pragma unroll 2
+  for (int l = top; l > 0; --l) {
+    const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
+    fp1 = f;
+    f = fm;
+  }
+  const double scal = i0 / f * gauss;
+  f = ft;
+  fp1 = ft1;
   // degrees top .. bot (uniform over the workgroup: L is)
   int s = ns - 1;
   for (int l = top; l >= bot; --l) {
