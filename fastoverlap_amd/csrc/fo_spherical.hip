@@ -2544,10 +2544,11 @@ FO_DEV void newton_update(NewtonState& s, const FGH& r) {
     s.stop = 1;
     return;
   }
-  // an undamped Newton step this short lands within O(|dx|^2) (~1e-10
-  // rad) of the optimum: take it without evaluating there, with the
-  // objective of the quadratic model (error O(|dx|^3))
-  if (pure && dxn <= 1e-5) {
+  // an undamped Newton step this short lands within O(|dx|^2) (~1e-8 rad)
+  // of the optimum: take it without evaluating there, with the objective of
+  // the quadratic model (error O(|dx|^3), ~1e-12 of f).  The rotation only
+  // seeds the assignment; the distance comes from the quaternion fit.
+  if (pure && dxn <= 1e-4) {
     for (int q = 0; q < 3; ++q) s.x[q] += dx[q];
     s.cur[0] += 0.5 * s.slope;
     ++s.it;
