@@ -325,12 +325,20 @@ FO_DEV void radial_store(int L, double x, double gauss, double* out, long long s
 __global__ void __launch_bounds__(256, 5) k_sph_radial(const SphPair* __restrict__ pd, const SphGroup* __restrict__ grp,
                              const double* __restrict__ rad1, const double* __restrict__ rad2,
                              const double* __restrict__ scale_eff, double* __restrict__ R) {
+  // the pair's first kRadGroups group records are staged in LDS (one load
+  // latency for the whole workgroup instead of a dependent chain per entry)
+  constexpr int kRadGroups = 16;
+  __shared__ SphGroup sgrp[kRadGroups];
   const SphPair d = pd[blockIdx.y];
+  if (d.ngroups < 1) return;  // whole workgroup: no (j, k) entries
   const double s = scale_eff[blockIdx.y];
   const double i2s2 = 1.0 / (2.0 * s * s), i4s2 = 1.0 / (4.0 * s * s);
-  const SphGroup* g = grp + d.group_off;
-  long long total = 0;
-  for (int q = 0; q < d.ngroups; ++q) total += (long long)g[q].size * g[q].size;
+  const SphGroup* gsrc = grp + d.group_off;
+  if ((int)threadIdx.x < min(d.ngroups, kRadGroups)) sgrp[threadIdx.x] = gsrc[threadIdx.x];
+  __syncthreads();
+  const SphGroup* g = d.ngroups <= kRadGroups ? sgrp : gsrc;
+  const SphGroup last = g[d.ngroups - 1];
+  const long long total = last.jk_off + (long long)last.size * last.size;
   for (long long w = blockIdx.x * (long long)blockDim.x + threadIdx.x; w < total;
        w += (long long)gridDim.x * blockDim.x) {
     int q = 0;
