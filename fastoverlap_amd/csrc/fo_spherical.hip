@@ -275,15 +275,20 @@ FO_DEV double radial_series(int l, double x, double exg) {
   }
   return lead * sum * exg;
 }
+// The values are stored non-temporally: 1.7 GB per C2 step that the
+// contraction reads back from HBM anyway, so they need not displace L2 lines
+// (0.93 -> 0.84 ms; the contraction's time unchanged).
 FO_DEV void radial_store(int L, double x, double gauss, double* out, long long stride) {
   const int ns = rad_slots(L);
   if (x == 0.0) {
-    for (int s = 0; s < ns; ++s) out[s * stride] = rad_slot_deg(s, L) == 0 ? gauss : 0.0;
+    for (int s = 0; s < ns; ++s)
+      __builtin_nontemporal_store(rad_slot_deg(s, L) == 0 ? gauss : 0.0, &out[s * stride]);
     return;
   }
   if (x < 1.0) {
     const double exg = exp(-x) * gauss;
-    for (int s = 0; s < ns; ++s) out[s * stride] = radial_series(rad_slot_deg(s, L), x, exg);
+    for (int s = 0; s < ns; ++s)
+      __builtin_nontemporal_store(radial_series(rad_slot_deg(s, L), x, exg), &out[s * stride]);
     return;
   }
   // i_0(x) e^{-x} times the Gaussian fixes the normalisation of Miller's
@@ -316,7 +321,7 @@ FO_DEV void radial_store(int L, double x, double gauss, double* out, long long s
   // degrees top .. bot (uniform over the workgroup: L is)
   int s = ns - 1;
   for (int l = top; l >= bot; --l) {
-    while (s >= 0 && rad_slot_deg(s, L) == l) out[(s--) * stride] = f * scal;
+    while (s >= 0 && rad_slot_deg(s, L) == l) __builtin_nontemporal_store(f * scal, &out[(s--) * stride]);
     const double fm = fp1 + (2.0 * l + 1.0) * ix * f;
     fp1 = f;
     f = fm;
