@@ -2959,12 +2959,18 @@ void launch_sph_contract(hipStream_t s, int P, int n, int Lmax, const SphPair* p
     if (K == 4)
       hipLaunchKernelGGL(k_sph_contract_mfma<4>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,
                          I32, n, P, nb, lo, hi);
+    else if (K == 6)
+      hipLaunchKernelGGL(k_sph_contract_mfma<6>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,
+                         I32, n, P, nb, lo, hi);
     else
       hipLaunchKernelGGL(k_sph_contract_mfma<8>, g, b, 0, s, pd, grp, Y1, Y2, R, scale_eff, I64,
                          I32, n, P, nb, lo, hi);
   };
+  // column-tile classes: 2l+1 <= 64 / 96 / 128 (the six-tile class holds two
+  // waves per SIMD and two workgroups per CU, the eight-tile class one)
   launch(4, 16, 31);
-  launch(8, 32, 63);
+  launch(6, 32, 47);
+  launch(8, 48, 63);
 }
 
 // two-pass factorisation n = N1 N2 of the power-of-two grids (0: direct DFT)
