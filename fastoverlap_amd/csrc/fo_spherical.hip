@@ -2689,28 +2689,31 @@ __global__ void __launch_bounds__(256) k_sph_finish(
     __threadfence_block();
     __syncthreads();
   }
-  // quaternion fit on all atoms
-  double c[6] = {0, 0, 0, 0, 0, 0};
-  for (int a = t; a < N; a += blockDim.x)
-    for (int k = 0; k < 3; ++k) {
-      c[k] += x1[3 * a + k];
-      c[3 + k] += xr[3 * pm[a] + k];
-    }
-  for (int k = 0; k < 6; ++k) c[k] = block_sum(c[k], red) / (double)N;
-  double q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int a = t; a < N; a += blockDim.x) {
-    const double* B = xr + 3 * pm[a];
-    qmat_accum(q, x1[3 * a] - c[0], x1[3 * a + 1] - c[1], x1[3 * a + 2] - c[2], B[0] - c[3],
-               B[1] - c[4], B[2] - c[5]);
-  }
-  for (int k = 0; k < 10; ++k) q[k] = block_sum(q[k], red);
+  // quaternion fit on all atoms, by the first wavefront (its sums do not
+  // depend on the block size)
   __shared__ double sMR[9];
   __shared__ double sdist;
-  if (t == 0) {
-    double MR[9], dist;
-    kearsley_solve(q, &dist, MR);
-    for (int k = 0; k < 9; ++k) sMR[k] = MR[k];
-    sdist = dist;
+  if (t < 64) {
+    double c[6] = {0, 0, 0, 0, 0, 0};
+    for (int a = t; a < N; a += 64)
+      for (int k = 0; k < 3; ++k) {
+        c[k] += x1[3 * a + k];
+        c[3 + k] += xr[3 * pm[a] + k];
+      }
+    for (int k = 0; k < 6; ++k) c[k] = wave_sum(c[k]) / (double)N;
+    double q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = t; a < N; a += 64) {
+      const double* B = xr + 3 * pm[a];
+      qmat_accum(q, x1[3 * a] - c[0], x1[3 * a + 1] - c[1], x1[3 * a + 2] - c[2], B[0] - c[3],
+                 B[1] - c[4], B[2] - c[5]);
+    }
+    for (int k = 0; k < 10; ++k) q[k] = wave_sum(q[k]);
+    if (t == 0) {
+      double MR[9], dist;
+      kearsley_solve(q, &dist, MR);
+      for (int k = 0; k < 9; ++k) sMR[k] = MR[k];
+      sdist = dist;
+    }
   }
   __syncthreads();
   double* xo = X2out + 3LL * d.atom_off;
@@ -3067,13 +3070,16 @@ void launch_sph_finish(hipStream_t s, int P, int lap_nmax, const SphPair* pd, co
                        int norient, double* X2R, int* perm, int* rowsol, double* X2out,
                        SphResult* res) {
   // one wavefront per pair: the assignment search is a chain of small
-  // reductions, cheapest without cross-wave barriers
+  // reductions, cheapest without cross-wave barriers, until the largest
+  // group reaches 192 atoms: then four, whose column scans are shorter (the
+  // assignment and the fit do not depend on the block size)
   // positions of the largest group in LDS beside the assignment arrays when
   // that stays within 64 KiB
   const size_t lap = (lap_lds_bytes(lap_nmax) + 15) & ~(size_t)15;
   const size_t pos = 6 * sizeof(double) * (size_t)lap_nmax;
   const int pos_lds = lap + pos <= 65536 ? 1 : 0;
-  hipLaunchKernelGGL(k_sph_finish, dim3(P), dim3(64), lap + (pos_lds ? pos : 0), s, pd, grp,
+  const int nthr = lap_nmax >= 192 ? 256 : 64;
+  hipLaunchKernelGGL(k_sph_finish, dim3(P), dim3(nthr), lap + (pos_lds ? pos : 0), s, pd, grp,
                      gath, X1c, X2c, opt, norient, lap_nmax, pos_lds, X2R, perm, rowsol, X2out,
                      res);
 }
